@@ -1,0 +1,193 @@
+/*
+ * sfh.h -- C ABI of libsfh_hip.so, the MI355X (gfx950) implementation of the
+ * two data-parallel paths of StarFormationHistories.jl:
+ *
+ *   P2  template-mixture likelihood: composite!, loglikelihood,
+ *       ∇loglikelihood(!), fg! -- plus a batched form over many coefficient
+ *       vectors (HMC / ensemble walkers) on the matrix cores;
+ *   P1  smoothed Hess-diagram template deposition: bin_cmd_smooth (the kernel
+ *       loop of partial_cmd_smooth), single and batched over templates.
+ *
+ * Plain C: pointers, sizes, status codes.  Every function returns SFH_OK (0)
+ * or a negative SFH_ERR_* code; sfh_last_error(ctx) then describes it.  The
+ * reference throws ArgumentError from @argcheck on shape mismatches; those
+ * cases return SFH_ERR_ARGUMENT here.
+ *
+ * Pointer conventions
+ *   - Functions without a _device suffix take HOST pointers, copy what they
+ *     need, and return when the result is in host memory.
+ *   - *_device functions take DEVICE pointers on the context's device, enqueue
+ *     work on the context's stream and return without synchronising.
+ *
+ * Layouts (identical to the reference's flattened layout)
+ *   models   : nbins x ntemplates, column-major, one template per column
+ *              (stack_models, src/fitting/utilities.jl:12-13).  FP32.
+ *   X        : ntemplates x B, column-major, one coefficient vector per column.
+ *   Hess mat : nx x ny, column-major (colour bin fastest), nx = length of the
+ *              colour edge range - 1 (src/StarFormationHistories.jl:584).
+ *
+ * Threading: calls on one context are serialised by a mutex; distinct
+ * contexts are independent (one stream each), mirroring the reference's
+ * per-task buffers (src/fitting/hmc_sample.jl:127).
+ */
+#ifndef SFH_H
+#define SFH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFH_OK              0
+#define SFH_ERR_ARGUMENT   -1   /* shape / value checks (@argcheck -> ArgumentError) */
+#define SFH_ERR_DEVICE     -2   /* HIP runtime or launch failure */
+#define SFH_ERR_ALLOC      -3   /* device or pinned allocation failed */
+#define SFH_ERR_DOMAIN     -4   /* input outside the reference's domain (e.g. non-finite
+                                   kernel centre: Julia's round(Int, NaN) throws InexactError) */
+
+typedef struct sfh_ctx sfh_ctx;
+typedef struct sfh_templates sfh_templates;
+typedef struct sfh_data sfh_data;
+
+/* Uniform bin edges: `first`, `step`, `length` edges -- the
+ * Tuple{AbstractRange,AbstractRange} of `edges` in bin_cmd_smooth /
+ * partial_cmd_smooth (src/StarFormationHistories.jl:451-476, :372-374). */
+typedef struct {
+    double first;
+    double step;
+    int64_t length;
+} sfh_range;
+
+/* Batched precision ladder (sfh_fg_batched*). */
+#define SFH_PREC_FP32    0  /* FP32 templates x FP32 coefficients, FP32 MFMA */
+#define SFH_PREC_BF16X2  1  /* split-BF16 hi+lo on both operands (3 products), FP32 accumulation */
+#define SFH_PREC_BF16X3  2  /* 3-piece templates x 4-piece coefficients (9 products), FP64
+                               promotion of the leading product every few K-steps */
+
+/* ---------------------------------------------------------------- context */
+
+/* No reference counterpart (the reference is one Julia process).  `stream` is
+ * a hipStream_t to enqueue on, or NULL to create a private stream. */
+int sfh_ctx_create(int device, void* stream, sfh_ctx** out);
+int sfh_ctx_destroy(sfh_ctx* ctx);
+const char* sfh_last_error(const sfh_ctx* ctx);
+int sfh_ctx_synchronize(sfh_ctx* ctx);
+/* Per-kernel timing with HIP events on the context's stream. */
+int sfh_ctx_set_timing(sfh_ctx* ctx, int enable);
+int sfh_ctx_kernel_stats(sfh_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
+int sfh_ctx_reset_stats(sfh_ctx* ctx);
+int sfh_version(void);
+
+/* ------------------------------------------------------------- templates */
+
+/* Upload models (nbins x ntemplates, column-major, FP32) -- the flattened
+ * `models::AbstractMatrix` argument of composite!/∇loglikelihood!/fg!
+ * (src/fitting/fitting_base.jl:55-65,265-285; src/fitting/solvers.jl:20). */
+int sfh_templates_create(sfh_ctx* ctx, const float* models, int64_t nbins,
+                         int64_t ntemplates, sfh_templates** out);
+/* Same from a device pointer (e.g. templates written by the deposition kernel). */
+int sfh_templates_create_device(sfh_ctx* ctx, const float* d_models, int64_t nbins,
+                                int64_t ntemplates, sfh_templates** out);
+int sfh_templates_destroy(sfh_templates* tpl);
+int sfh_templates_shape(const sfh_templates* tpl, int64_t* nbins, int64_t* ntemplates);
+/* Device pointer of the resident FP32 models (nbins x ntemplates). */
+const float* sfh_templates_device_ptr(const sfh_templates* tpl);
+
+/* Observed Hess diagram n_i (any real counts), FP64. */
+int sfh_data_create(sfh_ctx* ctx, const double* data, int64_t nbins, sfh_data** out);
+int sfh_data_create_device(sfh_ctx* ctx, const double* d_data, int64_t nbins, sfh_data** out);
+int sfh_data_destroy(sfh_data* data);
+
+/* ---------------------------------------------- P2: one coefficient vector */
+
+/* composite!(composite::AbstractVector, coeffs, models::AbstractMatrix)
+ * -- src/fitting/fitting_base.jl:55-65.  composite = models * coeffs. */
+int sfh_composite(sfh_ctx* ctx, const sfh_templates* tpl, const double* coeffs,
+                  double* composite);
+int sfh_composite_device(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_coeffs,
+                         double* d_composite);
+
+/* loglikelihood(composite, data) -- src/fitting/fitting_base.jl:84-96.
+ * `eps` is eps(promote_type(eltype(composite), eltype(data))), the clamp
+ * max(m_i, eps); the result is -Inf when the sum is exactly 0 (:95). */
+int sfh_loglikelihood(sfh_ctx* ctx, const double* composite, const double* data,
+                      int64_t n, double eps, double* out);
+
+/* loglikelihood(coeffs, models::AbstractMatrix, data::AbstractVector)
+ * -- src/fitting/fitting_base.jl:116-125. */
+int sfh_loglikelihood_coeffs(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
+                             const double* coeffs, double eps, double* out);
+
+/* ∇loglikelihood(models::AbstractMatrix, composite, data)
+ * -- src/fitting/fitting_base.jl:177-182 (composite left unchanged). */
+int sfh_grad_loglikelihood(sfh_ctx* ctx, const sfh_templates* tpl, const double* composite,
+                           const sfh_data* data, double eps, double* G);
+
+/* ∇loglikelihood!(G, composite::AbstractVector, models::AbstractMatrix, data)
+ * -- src/fitting/fitting_base.jl:265-285.  Overwrites composite with
+ * 1 - n_i/max(m_i, eps) exactly as the reference does (:274-280). */
+int sfh_grad_loglikelihood_inplace(sfh_ctx* ctx, const sfh_templates* tpl, double* composite,
+                                   const sfh_data* data, double eps, double* G);
+
+/* fg!(F, G, coeffs, models, data, composite) -- src/fitting/solvers.jl:20-38.
+ * want_F selects the objective; G may be NULL (gradient not wanted).  On
+ * return *negLL = -logL (when want_F), G = -∇logL, and composite (if not
+ * NULL) holds what the reference leaves there: 1 - n/max(m,eps) when G was
+ * requested, else m. */
+int sfh_fg(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
+           const double* coeffs, int want_F, double eps, double* negLL, double* G,
+           double* composite);
+/* Device-resident form: d_negLL (1 double), d_G (ntemplates) and
+ * d_composite (nbins) may each be NULL. */
+int sfh_fg_device(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
+                  const double* d_coeffs, int want_F, double eps, double* d_negLL,
+                  double* d_G, double* d_composite);
+
+/* --------------------------------------- P2: many coefficient vectors (new) */
+
+/* B independent fg! evaluations, one per column of X (ntemplates x B), the
+ * call pattern of LogDensityProblems.logdensity_and_gradient(::HMCModel)
+ * (src/fitting/hmc_sample.jl:24-37) and MCMCModel (src/fitting/mcmc_sample.jl:12-23)
+ * over many walkers.  negLL[b] = -logL(X[:,b]); G[:,b] = -∇logL (G may be
+ * NULL).  The final log-likelihood reduction is FP64 in every precision. */
+int sfh_fg_batched(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
+                   const double* X, int64_t B, int precision, double eps,
+                   double* negLL, double* G);
+int sfh_fg_batched_device(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
+                          const double* d_X, int64_t B, int precision, double eps,
+                          double* d_negLL, double* d_G);
+
+/* --------------------------------------------------- P1: template deposition */
+
+/* bin_cmd_smooth(colors, mags, color_err, mag_err, cov_mult; weights, edges)
+ * -- src/StarFormationHistories.jl:574-621.  cov_mult in {-1,0,1} as the
+ * reference checks (:579): 0 deposits GaussianPSFAsymmetric in pixel space
+ * (:586-610, :191-265, :348-364); +-1 deposits GaussianPSFCovariant in real
+ * space (:611-618, :272-338, :366-408).  out receives the FP64 nx x ny matrix. */
+int sfh_bin_cmd_smooth(sfh_ctx* ctx, const double* colors, const double* mags,
+                       const double* color_err, const double* mag_err,
+                       const double* weights, int64_t npoints, int cov_mult,
+                       sfh_range xedges, sfh_range yedges, double* out);
+
+/* Many templates in one launch (no reference counterpart: the reference
+ * calls partial_cmd_smooth once per template).  Point arrays are the
+ * concatenation of every template's points; d_offsets (ntemplates+1 int64,
+ * device) delimits them.  d_cmult, if not NULL, gives a real-valued
+ * covariance multiplier per point (the covariant formula with any c);
+ * otherwise cov_mult applies to all points.  Writes FP32 templates into
+ * d_out, column t at d_out + t*ld_out (ld_out >= nx*ny), the layout
+ * sfh_templates_create_device takes.  *pairs_out (may be NULL) receives the
+ * number of (point, bin) pairs evaluated (synchronises). */
+int sfh_bin_cmd_smooth_batch_device(sfh_ctx* ctx, const double* d_colors, const double* d_mags,
+                                    const double* d_color_err, const double* d_mag_err,
+                                    const double* d_weights, const double* d_cmult,
+                                    const int64_t* d_offsets, int64_t ntemplates,
+                                    double cov_mult, sfh_range xedges, sfh_range yedges,
+                                    float* d_out, int64_t ld_out, int64_t* pairs_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFH_H */

@@ -1,0 +1,32 @@
+"""MI355X-native implementation of StarFormationHistories.jl's two
+data-parallel paths: template-mixture likelihood (composite!, loglikelihood,
+∇loglikelihood(!), fg!, batched over walkers) and smoothed Hess-diagram
+template deposition (bin_cmd_smooth / partial_cmd_smooth).
+
+All numerical work runs in libsfh_hip.so (hand-written HIP for gfx950,
+C ABI in include/sfh.h).  Importing this package loads that library and
+raises if it has not been built; there is no CPU fallback.
+"""
+from . import _lib
+
+_lib.load()
+
+from ._lib import ArgumentError, DeviceError, InexactError  # noqa: E402
+from .core import (Context, HIPData, HIPTemplates, composite_, default_context, fg_,  # noqa: E402
+                   fg_batched, fg_batched_device, fg_device, grad_loglikelihood,
+                   grad_loglikelihood_, loglikelihood, stack_models)
+from .templates import (Histogram, PointBatch, Range, bin_cmd_smooth, calculate_edges,  # noqa: E402
+                        calculate_weights, deposit_batch, edges_range, exp_photerr,
+                        histogram_data, histogram_pix, interpolate_mini, Martin2016_complete,
+                        midpoints, mini_spacing, partial_cmd_smooth, partial_cmd_smooth_points,
+                        templates_from_device)
+
+__all__ = [
+    "ArgumentError", "DeviceError", "InexactError", "Context", "HIPData", "HIPTemplates",
+    "composite_", "default_context", "fg_", "fg_batched", "fg_batched_device", "fg_device",
+    "grad_loglikelihood", "grad_loglikelihood_", "loglikelihood", "stack_models", "Histogram",
+    "PointBatch", "Range", "bin_cmd_smooth", "calculate_edges", "calculate_weights",
+    "deposit_batch", "edges_range", "exp_photerr", "histogram_data", "histogram_pix",
+    "interpolate_mini", "Martin2016_complete", "midpoints", "mini_spacing", "partial_cmd_smooth",
+    "partial_cmd_smooth_points", "templates_from_device",
+]

@@ -1,0 +1,115 @@
+"""ctypes binding of libsfh_hip.so (declared in include/sfh.h).
+
+The shared library is the only compute path of this package: if it is
+missing, importing the package raises.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libsfh_hip.so")
+
+SFH_OK = 0
+SFH_ERR_ARGUMENT = -1
+SFH_ERR_DEVICE = -2
+SFH_ERR_ALLOC = -3
+SFH_ERR_DOMAIN = -4
+
+SFH_PREC_FP32 = 0
+SFH_PREC_BF16X2 = 1
+SFH_PREC_BF16X3 = 2
+
+PRECISIONS = {"fp32": SFH_PREC_FP32, "bf16x2": SFH_PREC_BF16X2, "bf16x3": SFH_PREC_BF16X3}
+
+
+class ArgumentError(ValueError):
+    """Shape / value check failed (the reference's @argcheck ArgumentError)."""
+
+
+class InexactError(ValueError):
+    """Input outside the reference's domain (e.g. round(Int, NaN))."""
+
+
+class DeviceError(RuntimeError):
+    """HIP runtime, launch or allocation failure."""
+
+
+class sfh_range(C.Structure):
+    _fields_ = [("first", C.c_double), ("step", C.c_double), ("length", C.c_int64)]
+
+
+_vp = C.c_void_p
+_i64 = C.c_int64
+_dbl = C.c_double
+_int = C.c_int
+_p64 = C.POINTER(C.c_int64)
+_pd = C.POINTER(C.c_double)
+
+# (name, restype, argtypes) for every symbol of include/sfh.h
+SIGNATURES = [
+    ("sfh_version", _int, []),
+    ("sfh_ctx_create", _int, [_int, _vp, C.POINTER(_vp)]),
+    ("sfh_ctx_destroy", _int, [_vp]),
+    ("sfh_last_error", C.c_char_p, [_vp]),
+    ("sfh_ctx_synchronize", _int, [_vp]),
+    ("sfh_ctx_set_timing", _int, [_vp, _int]),
+    ("sfh_ctx_kernel_stats", _int, [_vp, C.c_char_p, _pd, _p64]),
+    ("sfh_ctx_reset_stats", _int, [_vp]),
+    ("sfh_templates_create", _int, [_vp, _vp, _i64, _i64, C.POINTER(_vp)]),
+    ("sfh_templates_create_device", _int, [_vp, _vp, _i64, _i64, C.POINTER(_vp)]),
+    ("sfh_templates_destroy", _int, [_vp]),
+    ("sfh_templates_shape", _int, [_vp, _p64, _p64]),
+    ("sfh_templates_device_ptr", _vp, [_vp]),
+    ("sfh_data_create", _int, [_vp, _vp, _i64, C.POINTER(_vp)]),
+    ("sfh_data_create_device", _int, [_vp, _vp, _i64, C.POINTER(_vp)]),
+    ("sfh_data_destroy", _int, [_vp]),
+    ("sfh_composite", _int, [_vp, _vp, _vp, _vp]),
+    ("sfh_composite_device", _int, [_vp, _vp, _vp, _vp]),
+    ("sfh_loglikelihood", _int, [_vp, _vp, _vp, _i64, _dbl, _pd]),
+    ("sfh_loglikelihood_coeffs", _int, [_vp, _vp, _vp, _vp, _dbl, _pd]),
+    ("sfh_grad_loglikelihood", _int, [_vp, _vp, _vp, _vp, _dbl, _vp]),
+    ("sfh_grad_loglikelihood_inplace", _int, [_vp, _vp, _vp, _vp, _dbl, _vp]),
+    ("sfh_fg", _int, [_vp, _vp, _vp, _vp, _int, _dbl, _pd, _vp, _vp]),
+    ("sfh_fg_device", _int, [_vp, _vp, _vp, _vp, _int, _dbl, _vp, _vp, _vp]),
+    ("sfh_fg_batched", _int, [_vp, _vp, _vp, _vp, _i64, _int, _dbl, _vp, _vp]),
+    ("sfh_fg_batched_device", _int, [_vp, _vp, _vp, _vp, _i64, _int, _dbl, _vp, _vp]),
+    ("sfh_bin_cmd_smooth", _int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _int, sfh_range, sfh_range, _vp]),
+    ("sfh_bin_cmd_smooth_batch_device", _int,
+     [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, sfh_range, sfh_range, _vp, _i64, _p64]),
+]
+
+_lib = None
+
+
+def load():
+    """Load libsfh_hip.so and declare every entry point; raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"libsfh_hip.so not found at {LIB_PATH}; build it with __graft_entry__.build() "
+            "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int, ctx=None):
+    if rc == SFH_OK:
+        return
+    msg = ""
+    if ctx is not None and _lib is not None:
+        raw = _lib.sfh_last_error(ctx)
+        msg = raw.decode() if raw else ""
+    if rc == SFH_ERR_ARGUMENT:
+        raise ArgumentError(msg or "ArgumentError")
+    if rc == SFH_ERR_DOMAIN:
+        raise InexactError(msg or "InexactError")
+    raise DeviceError(f"sfh error {rc}: {msg}")

@@ -1,0 +1,297 @@
+// sfh_deposit.hip -- smoothed Hess-diagram template deposition on gfx950:
+// bin_cmd_smooth (src/StarFormationHistories.jl:574-621), one template or many
+// templates per launch.
+//
+// Two kernels per call:
+//   k_dep_prep    one thread per point: the reference's FP64 footprint / index
+//                 math (bit-exact, no FMA contraction) -> a 48-byte PointDesc.
+//   k_dep_cov / k_dep_asym
+//                 bin-owner gather: a workgroup owns 256 consecutive "slots" of
+//                 one template, a slot being one row and 8 consecutive columns,
+//                 held by one thread in registers.  Point descriptors are staged
+//                 through LDS 64 at a time; every thread adds its own bins'
+//                 contributions, so there are no atomics and every bin sums its
+//                 points in point order (deterministic).  Waves whose rows miss a
+//                 footprint skip it uniformly.  Contributions accumulate in FP32
+//                 per 64-point chunk and are flushed into FP64 accumulators.
+// The pair work is VALU/transcendental bound; HBM traffic is the descriptors in
+// and one write of each template out.
+#include "sfh_internal.h"
+#include "sfh_psf.h"
+
+namespace sfh {
+
+using psf::PointDesc;
+
+constexpr int kDepBlock = 256;
+constexpr int kCPT = 8;          // columns per thread
+constexpr int kChunk = 64;       // descriptors staged per LDS round
+
+struct DepRange {
+    double first, step;
+    int64_t length;
+};
+
+__global__ __launch_bounds__(256) void k_dep_prep(
+    const double* __restrict__ colors, const double* __restrict__ mags,
+    const double* __restrict__ cerr, const double* __restrict__ merr,
+    const double* __restrict__ weights, const double* __restrict__ cmult, double cov_mult,
+    int64_t npoints, psf::Range ex, psf::Range ey, int asym, PointDesc* __restrict__ out,
+    unsigned long long* __restrict__ pairs, int* __restrict__ bad)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long cnt = 0;
+    if (p < npoints) {
+        const double x = colors[p], y = mags[p], sx = cerr[p], sy = merr[p], A = weights[p];
+        const double c = cmult ? cmult[p] : cov_mult;
+        PointDesc d;
+        if (!(isfinite(x) && isfinite(y) && isfinite(sx) && isfinite(sy) && isfinite(A) && isfinite(c))) {
+            d = psf::empty_desc();
+            atomicOr(bad, 1);
+        } else if (asym) {
+            cnt = (unsigned long long)psf::prep_asymmetric(x, y, sx, sy, A, ex, ey, &d);
+        } else {
+            cnt = (unsigned long long)psf::prep_covariant(x, y, sx, sy, c, A, ex, ey, &d);
+        }
+        out[p] = d;
+    }
+    // wave-level sum of pair counts, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(pairs, cnt);
+}
+
+__device__ inline void stage_chunk(const PointDesc* __restrict__ desc, int64_t pc, int n, PointDesc* sdesc)
+{
+    const float4* src = reinterpret_cast<const float4*>(desc + pc);
+    float4* dst = reinterpret_cast<float4*>(sdesc);
+    for (int k = threadIdx.x; k < n * 3; k += kDepBlock) dst[k] = src[k];
+}
+
+template <bool kAsym>
+__global__ __launch_bounds__(kDepBlock) void k_dep_gather(
+    const PointDesc* __restrict__ desc, const int64_t* __restrict__ offsets, int nx, int ny,
+    int ncg, int nblk_per_tpl, float* __restrict__ out32, double* __restrict__ out64,
+    int64_t ld_out)
+{
+    __shared__ PointDesc sdesc[kChunk];
+    const int64_t t = blockIdx.x / nblk_per_tpl;
+    const int blk = blockIdx.x - (int)(t * nblk_per_tpl);
+    const int64_t p0 = offsets[t], p1 = offsets[t + 1];
+    const int slot = blk * kDepBlock + threadIdx.x;
+    const int row = slot / ncg;
+    const int col0 = (slot - row * ncg) * kCPT;
+    const bool valid = row < ny;
+    const int wslot = blk * kDepBlock + (threadIdx.x & ~63);
+    const int row_lo = wslot / ncg, row_hi = (wslot + 63) / ncg;
+
+    double acc[kCPT];
+    float part[kCPT];
+#pragma unroll
+    for (int c = 0; c < kCPT; ++c) {
+        acc[c] = 0.0;
+        part[c] = 0.0f;
+    }
+    for (int64_t pc = p0; pc < p1; pc += kChunk) {
+        const int n = (int)((p1 - pc) < kChunk ? (p1 - pc) : kChunk);
+        __syncthreads();
+        stage_chunk(desc, pc, n, sdesc);
+        __syncthreads();
+        for (int q = 0; q < n; ++q) {
+            const PointDesc d = sdesc[q];
+            if (d.ya > row_hi || d.yb < row_lo || d.xa > d.xb) continue;   // wave-uniform
+            if (!valid || row < d.ya || row > d.yb || col0 > d.xb || col0 + kCPT - 1 < d.xa) continue;
+            const bool xser = d.ex < psf::kSeriesEps;
+            const psf::Series sx = psf::series_coeffs(d.ex);
+            if (kAsym) {
+                // A/4 * D(u_i, ex) * D(v_j, ey)
+                const float v = fmaf((float)(row - d.ya), d.dv, d.ay);
+                float dy;
+                if (d.ey < psf::kSeriesEps) {
+                    dy = psf::d_series(v, psf::series_coeffs(d.ey));
+                } else {
+                    dy = psf::d_direct(v, d.ey);
+                }
+                const float fy = d.pref * dy;
+#pragma unroll
+                for (int c = 0; c < kCPT; ++c) {
+                    const int col = col0 + c;
+                    const float u = fmaf((float)(col - d.xa), d.du, d.ax);
+                    const float dx = xser ? psf::d_series(u, sx) : psf::d_direct(u, d.ex);
+                    part[c] += (col >= d.xa && col <= d.xb) ? fy * dx : 0.0f;
+                }
+            } else {
+                // sum_k w_k exp(-t_k^2/2) D(u_i + g t_k, ex), GL nodes t_k = t_j + {-1,0,1}*ey
+                const float tj = fmaf((float)(row - d.ya), d.dv, d.ay);
+                const float t0 = tj - d.ey, t2 = tj + d.ey;
+                const float w0 = psf::kW0 * psf::fast_exp(-0.5f * t0 * t0);
+                const float w1 = psf::kW1 * psf::fast_exp(-0.5f * tj * tj);
+                const float w2 = psf::kW0 * psf::fast_exp(-0.5f * t2 * t2);
+                const float h0 = d.g * t0, h1 = d.g * tj, h2 = d.g * t2;
+#pragma unroll
+                for (int c = 0; c < kCPT; ++c) {
+                    const int col = col0 + c;
+                    const float u = fmaf((float)(col - d.xa), d.du, d.ax);
+                    float v;
+                    if (xser) {
+                        v = w0 * psf::d_series(u + h0, sx) + w1 * psf::d_series(u + h1, sx) +
+                            w2 * psf::d_series(u + h2, sx);
+                    } else {
+                        v = w0 * psf::d_direct(u + h0, d.ex) + w1 * psf::d_direct(u + h1, d.ex) +
+                            w2 * psf::d_direct(u + h2, d.ex);
+                    }
+                    part[c] += (col >= d.xa && col <= d.xb) ? d.pref * v : 0.0f;
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kCPT; ++c) {
+            acc[c] += (double)part[c];
+            part[c] = 0.0f;
+        }
+    }
+    if (!valid) return;
+    const int64_t base = t * ld_out + (int64_t)row * nx;
+#pragma unroll
+    for (int c = 0; c < kCPT; ++c) {
+        const int col = col0 + c;
+        if (col < nx) {
+            if (out32) out32[base + col] = (float)acc[c];
+            if (out64) out64[base + col] = acc[c];
+        }
+    }
+}
+
+static int validate_ranges(sfh_ctx* ctx, sfh_range xe, sfh_range ye)
+{
+    SFH_ARG_CHECK(ctx, xe.length >= 2 && ye.length >= 2, "edge ranges need at least 2 edges");
+    SFH_ARG_CHECK(ctx, (xe.length - 1) * (ye.length - 1) > 1, "length(image.weights) > 1 (:367)");
+    SFH_ARG_CHECK(ctx, std::isfinite(xe.step) && std::isfinite(ye.step) && xe.step != 0.0 && ye.step != 0.0,
+                  "edge steps must be finite and non-zero");
+    SFH_ARG_CHECK(ctx, xe.length < (1 << 30) && ye.length < (1 << 30), "grid too large");
+    return SFH_OK;
+}
+
+// Core launcher on device arrays.  out32 / out64: column t at base + t*ld_out.
+static int deposit(sfh_ctx* ctx, const double* colors, const double* mags, const double* cerr,
+                   const double* merr, const double* weights, const double* cmult,
+                   const int64_t* d_offsets, int64_t ntemplates, int64_t npoints_total,
+                   double cov_mult, sfh_range xe, sfh_range ye, float* out32, double* out64,
+                   int64_t ld_out, int64_t* pairs_out)
+{
+    if (int rc = validate_ranges(ctx, xe, ye)) return rc;
+    const int nx = (int)(xe.length - 1), ny = (int)(ye.length - 1);
+    SFH_ARG_CHECK(ctx, ld_out >= (int64_t)nx * ny, "ld_out < nx*ny");
+    const bool asym = (cov_mult == 0.0) && cmult == nullptr;
+    void *pdesc = nullptr, *pcnt = nullptr;
+    if (int rc = scratch(ctx, "dep_desc", (size_t)(npoints_total > 0 ? npoints_total : 1) * sizeof(PointDesc), &pdesc))
+        return rc;
+    if (int rc = scratch(ctx, "dep_counters", 64, &pcnt)) return rc;
+    SFH_HIP_TRY(ctx, hipMemsetAsync(pcnt, 0, 64, ctx->stream));
+    unsigned long long* d_pairs = (unsigned long long*)pcnt;
+    int* d_bad = (int*)((char*)pcnt + 16);
+    psf::Range rx{xe.first, xe.step, xe.length}, ry{ye.first, ye.step, ye.length};
+    if (npoints_total > 0) {
+        TimedLaunch tl(ctx, "deposit_prep");
+        hipLaunchKernelGGL(k_dep_prep, dim3((unsigned)cdiv(npoints_total, 256)), dim3(256), 0, ctx->stream,
+                           colors, mags, cerr, merr, weights, cmult, cov_mult, npoints_total, rx, ry,
+                           asym ? 1 : 0, (PointDesc*)pdesc, d_pairs, d_bad);
+    }
+    if (int rc = check_launch(ctx, "k_dep_prep")) return rc;
+    const int ncg = (int)cdiv(nx, kCPT);
+    const int nblk = (int)cdiv((int64_t)ncg * ny, kDepBlock);
+    const int64_t grid = (int64_t)nblk * ntemplates;
+    SFH_ARG_CHECK(ctx, grid < (int64_t)1 << 31, "too many templates for one launch");
+    if (grid > 0) {
+        if (asym) {
+            TimedLaunch tl(ctx, "deposit_asym");
+            hipLaunchKernelGGL(k_dep_gather<true>, dim3((unsigned)grid), dim3(kDepBlock), 0, ctx->stream,
+                               (const PointDesc*)pdesc, d_offsets, nx, ny, ncg, nblk, out32, out64, ld_out);
+        } else {
+            TimedLaunch tl(ctx, "deposit_cov");
+            hipLaunchKernelGGL(k_dep_gather<false>, dim3((unsigned)grid), dim3(kDepBlock), 0, ctx->stream,
+                               (const PointDesc*)pdesc, d_offsets, nx, ny, ncg, nblk, out32, out64, ld_out);
+        }
+    }
+    if (int rc = check_launch(ctx, "k_dep_gather")) return rc;
+    // Non-finite inputs: the reference's round(Int, NaN) throws InexactError.
+    int bad = 0;
+    unsigned long long pairs = 0;
+    SFH_HIP_TRY(ctx, hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    SFH_HIP_TRY(ctx, hipMemcpyAsync(&pairs, d_pairs, sizeof(pairs), hipMemcpyDeviceToHost, ctx->stream));
+    SFH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (bad) return set_error(ctx, SFH_ERR_DOMAIN, "non-finite kernel parameter (InexactError in round(Int, ·))");
+    if (pairs_out) *pairs_out = (int64_t)pairs;
+    return SFH_OK;
+}
+
+}  // namespace sfh
+
+using namespace sfh;
+
+extern "C" {
+
+int sfh_bin_cmd_smooth(sfh_ctx* ctx, const double* colors, const double* mags,
+                       const double* color_err, const double* mag_err, const double* weights,
+                       int64_t npoints, int cov_mult, sfh_range xedges, sfh_range yedges,
+                       double* out)
+{
+    if (!ctx) return SFH_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (int rc = bind(ctx)) return rc;
+    SFH_ARG_CHECK(ctx, cov_mult == -1 || cov_mult == 0 || cov_mult == 1, "cov_mult in (-1, 0, 1) (:579)");
+    SFH_ARG_CHECK(ctx, npoints >= 0, "npoints must be >= 0");
+    SFH_ARG_CHECK(ctx, out && (npoints == 0 || (colors && mags && color_err && mag_err && weights)),
+                  "null pointer");
+    if (int rc = validate_ranges(ctx, xedges, yedges)) return rc;
+    const int64_t nbins = (xedges.length - 1) * (yedges.length - 1);
+    const int64_t np = npoints > 0 ? npoints : 1;
+    void* p = nullptr;
+    if (int rc = scratch(ctx, "bcs_in", (size_t)np * 5 * sizeof(double) + 2 * sizeof(int64_t), &p)) return rc;
+    double* d = (double*)p;
+    const double* srcs[5] = {colors, mags, color_err, mag_err, weights};
+    for (int k = 0; k < 5 && npoints > 0; ++k)
+        SFH_HIP_TRY(ctx, hipMemcpyAsync(d + k * np, srcs[k], (size_t)npoints * sizeof(double),
+                                        hipMemcpyHostToDevice, ctx->stream));
+    int64_t* doff = (int64_t*)(d + 5 * np);
+    int64_t hoff[2] = {0, npoints};
+    SFH_HIP_TRY(ctx, hipMemcpyAsync(doff, hoff, sizeof(hoff), hipMemcpyHostToDevice, ctx->stream));
+    void* po = nullptr;
+    if (int rc = scratch(ctx, "bcs_out", (size_t)nbins * sizeof(double), &po)) return rc;
+    int rc = deposit(ctx, d, d + np, d + 2 * np, d + 3 * np, d + 4 * np, nullptr, doff, 1, npoints,
+                     (double)cov_mult, xedges, yedges, nullptr, (double*)po, nbins, nullptr);
+    if (rc) return rc;
+    SFH_HIP_TRY(ctx, hipMemcpyAsync(out, po, (size_t)nbins * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SFH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SFH_OK;
+}
+
+int sfh_bin_cmd_smooth_batch_device(sfh_ctx* ctx, const double* d_colors, const double* d_mags,
+                                    const double* d_color_err, const double* d_mag_err,
+                                    const double* d_weights, const double* d_cmult,
+                                    const int64_t* d_offsets, int64_t ntemplates, double cov_mult,
+                                    sfh_range xedges, sfh_range yedges, float* d_out, int64_t ld_out,
+                                    int64_t* pairs_out)
+{
+    if (!ctx) return SFH_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (int rc = bind(ctx)) return rc;
+    SFH_ARG_CHECK(ctx, ntemplates >= 1, "ntemplates must be >= 1");
+    SFH_ARG_CHECK(ctx, d_offsets && d_out, "null pointer");
+    SFH_ARG_CHECK(ctx, std::isfinite(cov_mult), "cov_mult must be finite");
+    // The gather kernel trusts the offsets to index the descriptor array, so
+    // check them all on the host first (ntemplates+1 integers).
+    std::vector<int64_t> hoff((size_t)ntemplates + 1);
+    SFH_HIP_TRY(ctx, hipMemcpyAsync(hoff.data(), d_offsets, hoff.size() * sizeof(int64_t),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+    SFH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    SFH_ARG_CHECK(ctx, hoff[0] == 0, "offsets[0] must be 0");
+    for (int64_t t = 0; t < ntemplates; ++t)
+        SFH_ARG_CHECK(ctx, hoff[t + 1] >= hoff[t], "offsets must be non-decreasing (t=%lld)", (long long)t);
+    const int64_t last = hoff[ntemplates];
+    SFH_ARG_CHECK(ctx, last == 0 || (d_colors && d_mags && d_color_err && d_mag_err && d_weights),
+                  "null point array");
+    return deposit(ctx, d_colors, d_mags, d_color_err, d_mag_err, d_weights, d_cmult, d_offsets,
+                   ntemplates, last, cov_mult, xedges, yedges, d_out, nullptr, ld_out, pairs_out);
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+// sfh_internal.h -- context, handles, error plumbing and launch timing shared
+// by the HIP translation units of libsfh_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sfh.h"
+
+namespace sfh {
+
+// A grow-only device buffer owned by a context (no shrinking: the resident
+// working set of a fit is reused call after call).
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct KernelStat {
+    double total_ms = 0.0;
+    int64_t launches = 0;
+};
+
+struct PendingTiming {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
+}  // namespace sfh
+
+struct sfh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::recursive_mutex mu;
+    std::string last_error;
+    bool timing = false;
+    std::map<std::string, sfh::KernelStat> stats;
+    std::vector<sfh::PendingTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+    // named scratch buffers
+    std::map<std::string, sfh::DevBuf> scratch;
+    // small pinned host staging buffer
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+};
+
+struct sfh_templates {
+    sfh_ctx* ctx = nullptr;
+    int64_t nbins = 0, ntemplates = 0;
+    float* d_models = nullptr;          // nbins x ntemplates, column-major
+    // Batched-path layouts, built lazily on first batched call.
+    bool batched_ready = false;
+    int64_t nb_pad = 0, nt_pad = 0;     // padded bins / templates
+    float* d_a1 = nullptr;              // composite A operand, fragment-tiled FP32
+    uint16_t* d_a2 = nullptr;           // gradient A operand, fragment-tiled BF16 pieces
+    int a2_pieces = 0;
+    int32_t* d_scale_exp = nullptr;     // per-template power-of-two exponent
+};
+
+struct sfh_data {
+    sfh_ctx* ctx = nullptr;
+    int64_t nbins = 0;
+    double* d_data = nullptr;
+};
+
+namespace sfh {
+
+int set_error(sfh_ctx* ctx, int code, const char* fmt, ...);
+
+#define SFH_HIP_TRY(ctx, expr)                                                     \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess)                                                      \
+            return ::sfh::set_error((ctx), SFH_ERR_DEVICE, "%s failed: %s (%s:%d)", \
+                                    #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
+    } while (0)
+
+#define SFH_ARG_CHECK(ctx, cond, ...)                                              \
+    do {                                                                           \
+        if (!(cond)) return ::sfh::set_error((ctx), SFH_ERR_ARGUMENT, __VA_ARGS__); \
+    } while (0)
+
+// Make the context's device current for the calling thread.
+int bind(sfh_ctx* ctx);
+
+// Grow-only named scratch buffer.
+int scratch(sfh_ctx* ctx, const char* name, size_t bytes, void** out);
+// Pinned host staging buffer of at least `bytes`.
+int pinned(sfh_ctx* ctx, size_t bytes, void** out);
+
+// Launch timing: call before / after each kernel launch.
+void timing_begin(sfh_ctx* ctx, const char* name, hipEvent_t* start_out);
+void timing_end(sfh_ctx* ctx, const char* name, hipEvent_t start);
+void resolve_timing(sfh_ctx* ctx);
+
+// Check the last launch error.
+int check_launch(sfh_ctx* ctx, const char* name);
+
+struct TimedLaunch {
+    sfh_ctx* ctx;
+    const char* name;
+    hipEvent_t start{};
+    TimedLaunch(sfh_ctx* c, const char* n) : ctx(c), name(n) { timing_begin(ctx, name, &start); }
+    ~TimedLaunch() { timing_end(ctx, name, start); }
+};
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Launchers implemented per translation unit.
+int launch_composite(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_coeffs,
+                     double* d_partials, int nsplit);
+
+}  // namespace sfh

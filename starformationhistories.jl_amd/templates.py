@@ -1,0 +1,348 @@
+"""Smoothed Hess-diagram templates on the GPU, behind the reference's API.
+
+Mirrors the template path of src/StarFormationHistories.jl (v1.3.1):
+partial_cmd_smooth (:829-911) prepares the isochrone on the host (numpy,
+FP64, the reference's operation order) and hands the kernel centres, widths
+and weights to bin_cmd_smooth (:574-621), which runs on the GPU through
+libsfh_hip.so.  ``partial_cmd_smooth_batch`` deposits many templates in one
+launch and leaves them on the device for the likelihood path.
+
+Index arguments (y_index, color_indices) are 0-based here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import ArgumentError, sfh_range
+from .core import Context, HIPTemplates, default_context
+
+# ------------------------------------------------------------------ geometry
+
+
+@dataclass(frozen=True)
+class Range:
+    """A uniform range (Julia StepRangeLen): ``first + k*step`` for k < length."""
+    first: float
+    step: float
+    length: int
+
+    def values(self) -> np.ndarray:
+        return self.first + np.arange(self.length, dtype=np.float64) * self.step
+
+    def __len__(self):
+        return self.length
+
+    @property
+    def last(self):
+        return self.first + (self.length - 1) * self.step
+
+    def as_c(self) -> sfh_range:
+        return sfh_range(float(self.first), float(self.step), int(self.length))
+
+
+def edges_range(start: float, stop: float, length: int) -> Range:
+    """``range(start, stop; length)`` (Float64).  Julia keeps the step in twice
+    precision; (stop-start)/(length-1) agrees to ~1 ulp."""
+    if length < 2:
+        raise ArgumentError("range needs length >= 2 here")
+    return Range(float(start), (float(stop) - float(start)) / (length - 1), int(length))
+
+
+def calculate_edges(edges=None, xlim=None, ylim=None, nbins=None, xwidth=None, ywidth=None):
+    """calculate_edges (src/StarFormationHistories.jl:451-476).  ``nbins`` is
+    the number of EDGES per axis, as in the reference's range(...; length)."""
+    if edges is not None:
+        ex, ey = edges
+        if not (isinstance(ex, Range) and isinstance(ey, Range)):
+            raise ArgumentError("`edges` must be a tuple of two Range objects")
+        return edges
+    xlim, ylim = sorted(xlim), sorted(ylim)
+    if nbins is None:
+        if xwidth is not None and ywidth is not None:
+            nbins = (int(round((xlim[1] - xlim[0]) / xwidth)), int(round((ylim[1] - ylim[0]) / ywidth)))
+        else:
+            raise ArgumentError("If the keyword arguments `edges` and `nbins` are not provided, then "
+                                "`xwidth` and `ywidth` must be provided.")
+    return (edges_range(xlim[0], xlim[1], nbins[0]), edges_range(ylim[0], ylim[1], nbins[1]))
+
+
+def histogram_pix(d, edges: Range):
+    """histogram_pix(d, edges::AbstractRange) (:502): fractional 1-based index."""
+    return (np.asarray(d, dtype=np.float64) - edges.first) / edges.step + 1.0
+
+
+def histogram_data(x, edges: Range):
+    """histogram_data(x, edges::AbstractRange) (:518)."""
+    return (np.asarray(x, dtype=np.float64) - 1.0) * edges.step + edges.first
+
+
+@dataclass
+class Histogram:
+    """The StatsBase.Histogram fields the reference returns: edges, weights
+    (nx x ny, colour fastest), closed=:left, isdensity=false."""
+    edges: tuple
+    weights: np.ndarray
+    closed: str = "left"
+    isdensity: bool = False
+
+
+# --------------------------------------------------------- host preparation
+
+def Martin2016_complete(m, A, m50, rho):
+    """Martin2016_complete (src/utilities.jl:205)."""
+    return A / (1.0 + np.exp((np.asarray(m, dtype=np.float64) - m50) / rho))
+
+
+def exp_photerr(m, a, b, c, d):
+    """exp_photerr (src/utilities.jl:218): a^(b (m - c)) + d."""
+    return np.power(a, b * (np.asarray(m, dtype=np.float64) - c)) + d
+
+
+def _apply(f, x: np.ndarray) -> np.ndarray:
+    """Evaluate a user callable elementwise (vectorised when it accepts arrays)."""
+    try:
+        y = f(x)
+        y = np.asarray(y, dtype=np.float64)
+        if y.shape == x.shape:
+            return y
+        if y.ndim == 0:
+            return np.full(x.shape, float(y))
+    except Exception:
+        pass
+    return np.array([float(f(float(v))) for v in x], dtype=np.float64)
+
+
+def mini_spacing(m_ini, colors, mags, dmag, ret_spacing=False):
+    """mini_spacing (src/StarFormationHistories.jl:110-148), vectorised: for
+    every consecutive pair farther apart than dmag in the CMD insert
+    ceil(d/dmag) equal mass steps, then keep first occurrences (unique)."""
+    m_ini = np.asarray(m_ini, dtype=np.float64)
+    colors = np.asarray(colors, dtype=np.float64)
+    mags = np.asarray(mags, dtype=np.float64)
+    if not (m_ini.shape == colors.shape == mags.shape):
+        raise ArgumentError("axes(m_ini) == axes(mags) == axes(colors)")
+    first = m_ini[0]                                  # :117, before sorting
+    idx = np.argsort(m_ini, kind="stable")
+    m, c, y = m_ini[idx], colors[idx], mags[idx]
+    d = np.hypot(np.diff(c), np.diff(y))
+    need = d > dmag
+    npts = np.where(need, np.ceil(d / dmag), 1.0).astype(np.int64)
+    seg = np.repeat(np.arange(d.size), npts)
+    j = (np.arange(seg.size) - np.repeat(np.cumsum(npts) - npts, npts) + 1).astype(np.float64)
+    mass_step = (m[1:] - m[:-1]) / npts
+    vals = np.where(need[seg], m[seg] + mass_step[seg] * j, m[seg + 1])
+    new = np.concatenate([[first], vals])
+    _, first_idx = np.unique(new, return_index=True)  # :141 unique keeps first occurrences
+    new = new[np.sort(first_idx)]
+    if not ret_spacing:
+        return new
+    return new, np.diff(new)
+
+
+def interpolate_mini(m_ini, mags, new_mini):
+    """interpolate_mini (:77-81): sort_ingested then Gridded(Linear())
+    interpolation; raises for duplicate knots or out-of-bounds queries
+    (the reference's BoundsError)."""
+    m_ini = np.asarray(m_ini, dtype=np.float64)
+    mags = np.asarray(mags, dtype=np.float64)
+    order = np.argsort(m_ini, kind="stable")
+    k, yv = m_ini[order], mags[order]
+    if np.any(np.diff(k) <= 0):
+        raise ArgumentError("duplicate isochrone masses are not supported")
+    x = np.asarray(new_mini, dtype=np.float64)
+    if x.size and (x.min() < k[0] or x.max() > k[-1]):
+        raise ArgumentError("BoundsError: interpolation outside the isochrone's mass range")
+    i = np.clip(np.searchsorted(k, x, side="right") - 1, 0, k.size - 2)
+    t = (x - k[i]) / (k[i + 1] - k[i])
+    return yv[i] * (1.0 - t) + yv[i + 1] * t
+
+
+_RTOL = math.sqrt(np.finfo(np.float64).eps)
+
+
+def midpoints(v, ranges: bool = True) -> np.ndarray:
+    """midpoints(v, ranges=true) (:417-431); a Range input keeps its step (:436)."""
+    if isinstance(v, Range):
+        return Range(v.first + v.step / 2, v.step, v.length - 1)
+    v = np.asarray(v, dtype=np.float64)
+    d = np.diff(v)
+    if ranges and d.size and np.all((d == d[0]) | (np.abs(d - d[0]) <= _RTOL * np.maximum(abs(d[0]), np.abs(d)))):
+        start, stop, n = v[0] + d[0] / 2, v[-1] - d[0] / 2, v.size - 1
+        if n == 1:
+            return np.array([start])
+        return start + np.arange(n) * ((stop - start) / (n - 1))
+    return v[:-1] + d / 2
+
+
+def calculate_weights(mini, completeness, imf, normalize_value, mean_mass, spacing=None):
+    """calculate_weights (:765-776): trapezoid of the IMF pdf across each mass
+    step times the completeness of the LEFT point, times normalize/mean_mass."""
+    mini = np.asarray(mini, dtype=np.float64)
+    completeness = np.asarray(completeness, dtype=np.float64)
+    if spacing is None:
+        spacing = np.diff(mini)
+    if mini.size != completeness.size or np.asarray(spacing).size != mini.size - 1:
+        raise ArgumentError("length(mini) == length(completeness) and length(spacing) == length(mini)-1")
+    pdf = _apply(imf, mini)
+    w = np.asarray(spacing) * (pdf[:-1] + pdf[1:]) / 2
+    return w * (completeness[:-1] * normalize_value / mean_mass)
+
+
+def imf_mean(imf) -> float:
+    """mean(imf) (:166-167): the IMF's own mean if it has one."""
+    if hasattr(imf, "mean"):
+        return float(imf.mean())
+    raise ArgumentError("mean_mass must be given for an IMF callable without .mean()")
+
+
+@dataclass
+class SmoothPoints:
+    """What partial_cmd_smooth hands to bin_cmd_smooth (:886-888)."""
+    colors: np.ndarray
+    mags: np.ndarray
+    color_err: np.ndarray
+    mag_err: np.ndarray
+    weights: np.ndarray
+    cov_mult: int
+
+
+def partial_cmd_smooth_points(m_ini, mags, mag_err_funcs, y_index, color_indices, imf,
+                              completeness_funcs=None, bias_funcs=None, *, dmod=0.0,
+                              normalize_value=1.0, mean_mass=None, edges) -> SmoothPoints:
+    """Host half of partial_cmd_smooth (:839-888), NoBinaries."""
+    nf = len(mags)
+    if completeness_funcs is None:
+        completeness_funcs = [lambda m: np.ones_like(np.asarray(m, dtype=np.float64))] * nf
+    if bias_funcs is None:
+        bias_funcs = [lambda m: np.zeros_like(np.asarray(m, dtype=np.float64))] * nf
+    if len(color_indices) != 2:
+        raise ArgumentError("length(color_indices) == 2")
+    if not (nf == len(mag_err_funcs) == len(completeness_funcs) == len(bias_funcs)):
+        raise ArgumentError("length(mags) == length(mag_err_funcs) == length(completeness_funcs) == length(bias_funcs)")
+    for i in set([y_index, *color_indices]):
+        if not (0 <= i < nf):
+            raise ArgumentError("y_index / color_indices out of range")
+    if mean_mass is None:
+        mean_mass = imf_mean(imf)
+    ex, ey = edges
+    c0, c1 = color_indices
+    mags = [np.asarray(m, dtype=np.float64) for m in mags]
+    ymags = mags[y_index]
+    colors = mags[c0] - mags[c1]                                            # :849
+    dmag = min(ex.step, ey.step)                                             # :851
+    new_mini, new_spacing = mini_spacing(m_ini, colors, ymags, dmag, True)   # :852
+    new_iso = [interpolate_mini(m_ini, m, new_mini) + dmod for m in mags]   # :854
+    mag_err = [_apply(mag_err_funcs[i], new_iso[i]) for i in range(nf)]     # :855
+    if y_index in color_indices:                                             # :856
+        x_c_idx = c1 if c0 == y_index else c0
+        color_err = mag_err[x_c_idx]
+        compl = _apply(completeness_funcs[c0], new_iso[c0]) * _apply(completeness_funcs[c1], new_iso[c1])
+        weights = calculate_weights(new_mini, compl, imf, normalize_value, mean_mass, new_spacing)
+        cov_mult = -1 if y_index == c0 else 1                                # :868
+    else:
+        color_err = np.sqrt(mag_err[c0] ** 2 + mag_err[c1] ** 2)             # :871
+        compl = (_apply(completeness_funcs[c0], new_iso[c0]) * _apply(completeness_funcs[c1], new_iso[c1]) *
+                 _apply(completeness_funcs[y_index], new_iso[y_index]))
+        weights = calculate_weights(new_mini, compl, imf, normalize_value, mean_mass, new_spacing)
+        cov_mult = 0
+    bias = [new_iso[i] + _apply(bias_funcs[i], new_iso[i]) for i in range(nf)]   # :884
+    bcol = bias[c0] - bias[c1]                                               # :885
+    return SmoothPoints(midpoints(bcol), midpoints(bias[y_index]), midpoints(color_err),
+                        midpoints(mag_err[y_index]), np.asarray(weights, dtype=np.float64), cov_mult)
+
+
+# ------------------------------------------------------------- GPU deposition
+
+def bin_cmd_smooth(colors, mags, color_err, mag_err, cov_mult: int = 0, *, weights=None, edges=None,
+                   xlim=None, ylim=None, nbins=None, xwidth=None, ywidth=None,
+                   ctx: Context | None = None) -> Histogram:
+    """bin_cmd_smooth (src/StarFormationHistories.jl:574-621) on the GPU."""
+    if cov_mult not in (-1, 0, 1):
+        raise ArgumentError("cov_mult in (-1, 0, 1)")
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (colors, mags, color_err, mag_err)]
+    if weights is None:
+        weights = np.ones_like(arrs[0])
+    w = np.ascontiguousarray(weights, dtype=np.float64).ravel()
+    if not all(a.size == w.size for a in arrs):
+        raise ArgumentError("axes(colors) == axes(mags) == axes(color_err) == axes(mag_err) == axes(weights)")
+    if edges is None:
+        xlim = xlim if xlim is not None else (arrs[0].min(), arrs[0].max())
+        ylim = ylim if ylim is not None else (arrs[1].min(), arrs[1].max())
+    ex, ey = calculate_edges(edges, xlim, ylim, nbins, xwidth, ywidth)
+    nx, ny = ex.length - 1, ey.length - 1
+    out = np.empty(nx * ny, dtype=np.float64)
+    ctx = ctx or default_context()
+    ctx.check(ctx.lib.sfh_bin_cmd_smooth(ctx.handle, arrs[0].ctypes.data, arrs[1].ctypes.data,
+                                         arrs[2].ctypes.data, arrs[3].ctypes.data, w.ctypes.data, w.size,
+                                         int(cov_mult), ex.as_c(), ey.as_c(), out.ctypes.data))
+    return Histogram((ex, ey), out.reshape((nx, ny), order="F"))
+
+
+def partial_cmd_smooth(m_ini, mags, mag_err_funcs, y_index, color_indices, imf,
+                       completeness_funcs=None, bias_funcs=None, *, dmod=0.0, normalize_value=1.0,
+                       binary_model=None, mean_mass=None, edges=None, xlim=None, ylim=None, nbins=None,
+                       xwidth=None, ywidth=None, ctx: Context | None = None) -> Histogram:
+    """partial_cmd_smooth (src/StarFormationHistories.jl:829-911), NoBinaries:
+    host preparation + GPU deposition."""
+    if binary_model is not None and getattr(binary_model, "fraction", 0) != 0:
+        raise ArgumentError("binary models are not implemented on this path")
+    edges = calculate_edges(edges, xlim, ylim, nbins, xwidth, ywidth)
+    sp = partial_cmd_smooth_points(m_ini, mags, mag_err_funcs, y_index, color_indices, imf,
+                                   completeness_funcs, bias_funcs, dmod=dmod,
+                                   normalize_value=normalize_value, mean_mass=mean_mass, edges=edges)
+    return bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, sp.cov_mult, weights=sp.weights,
+                          edges=edges, ctx=ctx)
+
+
+class PointBatch:
+    """Concatenated kernel points of many templates, resident on the device
+    (torch tensors), ready for ``deposit_batch``."""
+
+    def __init__(self, points_list, edges, cov_mult=None, cmult_list=None, device="cuda"):
+        import torch
+        self.edges = edges
+        self.ntemplates = len(points_list)
+        counts = [p.colors.size for p in points_list]
+        self.offsets_host = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        cat = lambda attr: np.concatenate([getattr(p, attr) for p in points_list]).astype(np.float64)
+        self.cov_mult = float(points_list[0].cov_mult if cov_mult is None else cov_mult)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
+        self.colors, self.mags = t(cat("colors")), t(cat("mags"))
+        self.color_err, self.mag_err, self.weights = t(cat("color_err")), t(cat("mag_err")), t(cat("weights"))
+        self.cmult = t(np.concatenate(cmult_list).astype(np.float64)) if cmult_list is not None else None
+        self.offsets = t(self.offsets_host)
+
+    @property
+    def npoints(self):
+        return int(self.offsets_host[-1])
+
+
+def deposit_batch(batch: PointBatch, out=None, ctx: Context | None = None, return_pairs=False):
+    """Deposit every template of ``batch`` in one launch
+    (sfh_bin_cmd_smooth_batch_device).  Returns an FP32 device tensor
+    (nbins x ntemplates column-major, i.e. torch shape (ntemplates, nbins))."""
+    import torch
+    ctx = ctx or default_context()
+    ex, ey = batch.edges
+    nbins = (ex.length - 1) * (ey.length - 1)
+    if out is None:
+        out = torch.empty((batch.ntemplates, nbins), dtype=torch.float32, device=batch.colors.device)
+    pairs = C.c_int64()
+    ctx.check(ctx.lib.sfh_bin_cmd_smooth_batch_device(
+        ctx.handle, batch.colors.data_ptr(), batch.mags.data_ptr(), batch.color_err.data_ptr(),
+        batch.mag_err.data_ptr(), batch.weights.data_ptr(),
+        batch.cmult.data_ptr() if batch.cmult is not None else None, batch.offsets.data_ptr(),
+        batch.ntemplates, batch.cov_mult, ex.as_c(), ey.as_c(), out.data_ptr(), nbins,
+        C.byref(pairs) if return_pairs else None))
+    if return_pairs:
+        return out, pairs.value
+    return out
+
+
+def templates_from_device(out, nbins: int, ctx: Context | None = None) -> HIPTemplates:
+    """Wrap deposited device templates as the likelihood path's HIPTemplates."""
+    return HIPTemplates(ctx=ctx, device_ptr=out.data_ptr(), shape=(nbins, out.shape[0]))
