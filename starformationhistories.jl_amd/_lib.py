@@ -88,6 +88,14 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
+    # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64
+    # (SONAME libamdhip64.so.7) and loads it under the name libamdhip64.so.
+    # Loading torch first makes this library's libamdhip64.so.7 dependency
+    # resolve to that same runtime instead of a second copy from /opt/rocm.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"libsfh_hip.so not found at {LIB_PATH}; build it with __graft_entry__.build() "
@@ -105,7 +113,7 @@ def check(rc: int, ctx=None):
     if rc == SFH_OK:
         return
     msg = ""
-    if ctx is not None and _lib is not None:
+    if _lib is not None:
         raw = _lib.sfh_last_error(ctx)
         msg = raw.decode() if raw else ""
     if rc == SFH_ERR_ARGUMENT:

@@ -416,25 +416,46 @@ extern "C" {
 
 int sfh_version(void) { return 10000; }
 
+// Errors raised before a context exists (sfh_last_error(NULL) reports them).
+static thread_local std::string g_create_error;
+
+static int create_fail(int code, const char* what, hipError_t e)
+{
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(e));
+    g_create_error = buf;
+    return code;
+}
+
 int sfh_ctx_create(int device, void* stream, sfh_ctx** out)
 {
     if (!out) return SFH_ERR_ARGUMENT;
     *out = nullptr;
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SFH_ERR_DEVICE;
-    if (device < 0 || device >= ndev) return SFH_ERR_ARGUMENT;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess) return create_fail(SFH_ERR_DEVICE, "hipGetDeviceCount", e);
+    if (ndev <= 0) {
+        g_create_error = "no HIP device visible";
+        return SFH_ERR_DEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        g_create_error = "device index out of range";
+        return SFH_ERR_ARGUMENT;
+    }
     sfh_ctx* c = new sfh_ctx();
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess) {
+    e = hipSetDevice(device);
+    if (e != hipSuccess) {
         delete c;
-        return SFH_ERR_DEVICE;
+        return create_fail(SFH_ERR_DEVICE, "hipSetDevice", e);
     }
     if (stream) {
         c->stream = (hipStream_t)stream;
     } else {
-        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
             delete c;
-            return SFH_ERR_DEVICE;
+            return create_fail(SFH_ERR_DEVICE, "hipStreamCreateWithFlags", e);
         }
         c->own_stream = true;
     }
@@ -460,7 +481,7 @@ int sfh_ctx_destroy(sfh_ctx* ctx)
     return SFH_OK;
 }
 
-const char* sfh_last_error(const sfh_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+const char* sfh_last_error(const sfh_ctx* ctx) { return ctx ? ctx->last_error.c_str() : g_create_error.c_str(); }
 
 int sfh_ctx_synchronize(sfh_ctx* ctx)
 {
