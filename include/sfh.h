@@ -62,8 +62,8 @@ typedef struct {
 /* Batched precision ladder (sfh_fg_batched*). */
 #define SFH_PREC_FP32    0  /* FP32 templates x FP32 coefficients, FP32 MFMA */
 #define SFH_PREC_BF16X2  1  /* split-BF16 hi+lo on both operands (3 products), FP32 accumulation */
-#define SFH_PREC_BF16X3  2  /* 3-piece templates x 4-piece coefficients (9 products), FP64
-                               promotion of the leading product every few K-steps */
+#define SFH_PREC_F64     2  /* composite on the FP64 matrix cores (IEEE FP64 accumulation),
+                               gradient split-BF16 hi+lo: log-likelihood at FP64 accuracy */
 
 /* ---------------------------------------------------------------- context */
 

@@ -19,9 +19,9 @@ SFH_ERR_DOMAIN = -4
 
 SFH_PREC_FP32 = 0
 SFH_PREC_BF16X2 = 1
-SFH_PREC_BF16X3 = 2
+SFH_PREC_F64 = 2
 
-PRECISIONS = {"fp32": SFH_PREC_FP32, "bf16x2": SFH_PREC_BF16X2, "bf16x3": SFH_PREC_BF16X3}
+PRECISIONS = {"fp32": SFH_PREC_FP32, "bf16x2": SFH_PREC_BF16X2, "f64": SFH_PREC_F64}
 
 
 class ArgumentError(ValueError):

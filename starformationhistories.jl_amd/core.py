@@ -342,7 +342,7 @@ def fg_device(models: HIPTemplates, data: HIPData, d_coeffs, d_negLL=None, d_G=N
                                                   _ptr(d_G), _ptr(d_composite)))
 
 
-def fg_batched(X, models: HIPTemplates, data, precision: str = "bf16x3", want_G: bool = True,
+def fg_batched(X, models: HIPTemplates, data, precision: str = "f64", want_G: bool = True,
                eps: float = _f64eps):
     """B independent fg! evaluations (columns of X, ntemplates x B) -- the
     HMC / ensemble call pattern (src/fitting/hmc_sample.jl:24-37,
@@ -362,7 +362,7 @@ def fg_batched(X, models: HIPTemplates, data, precision: str = "bf16x3", want_G:
 
 
 def fg_batched_device(models: HIPTemplates, data: HIPData, d_X, B: int, d_negLL, d_G=None,
-                      precision: str = "bf16x3", eps: float = _f64eps):
+                      precision: str = "f64", eps: float = _f64eps):
     """Asynchronous batched fg! on device buffers (X: T x B column-major FP64)."""
     models.ctx.check(models.ctx.lib.sfh_fg_batched_device(models.ctx.handle, models.handle, data.handle,
                                                           _ptr(d_X), int(B), _lib.PRECISIONS[precision], eps,

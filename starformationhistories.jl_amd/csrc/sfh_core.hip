@@ -585,10 +585,8 @@ int sfh_templates_destroy(sfh_templates* t)
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    release_batched(t);
     hipFree(t->d_models);
-    if (t->d_a1) hipFree(t->d_a1);
-    if (t->d_a2) hipFree(t->d_a2);
-    if (t->d_scale_exp) hipFree(t->d_scale_exp);
     delete t;
     return SFH_OK;
 }

@@ -54,14 +54,7 @@ struct sfh_ctx {
 struct sfh_templates {
     sfh_ctx* ctx = nullptr;
     int64_t nbins = 0, ntemplates = 0;
-    float* d_models = nullptr;          // nbins x ntemplates, column-major
-    // Batched-path layouts, built lazily on first batched call.
-    bool batched_ready = false;
-    int64_t nb_pad = 0, nt_pad = 0;     // padded bins / templates
-    float* d_a1 = nullptr;              // composite A operand, fragment-tiled FP32
-    uint16_t* d_a2 = nullptr;           // gradient A operand, fragment-tiled BF16 pieces
-    int a2_pieces = 0;
-    int32_t* d_scale_exp = nullptr;     // per-template power-of-two exponent
+    float* d_models = nullptr;          // ld x ntemplates, column-major, ld = nbins rounded up to 256
 };
 
 struct sfh_data {
@@ -112,6 +105,9 @@ struct TimedLaunch {
 };
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Frees the batched-path operand layouts built for a template set.
+void release_batched(const sfh_templates* t);
 
 // Launchers implemented per translation unit.
 int launch_composite(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_coeffs,

@@ -1,0 +1,80 @@
+"""GPU parity of the batched fg! (many walkers per launch, MFMA) against the
+FP64 oracle, for each precision of the ladder.  Tolerances are the documented
+bounds of each form (FP64 final reduction in all of them):
+  f64    : LL rel-err <= 1e-9        gradient norm-rel-err <= 1e-4
+  bf16x2 : LL rel-err <= 5e-5        gradient norm-rel-err <= 1e-3
+  fp32   : LL rel-err <= 1e-5        gradient norm-rel-err <= 1e-4
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BOUNDS = {"f64": (1e-9, 1e-4), "bf16x2": (5e-5, 1e-3), "fp32": (1e-5, 1e-4)}
+
+
+def _problem(N, T, B, seed, total=1e6, sigma_dex=0.3):
+    from sfh_amd import synth
+    models = synth.synthetic_templates(N, T, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    x_true = rng.lognormal(0.0, 1.0, T)
+    x_true *= total / (models.astype(np.float64) @ x_true).sum()
+    data = synth.poisson_data(models.astype(np.float64) @ x_true, seed=seed + 2)
+    X = synth.walkers(x_true, B, sigma_dex, seed + 3)
+    return models, data, X
+
+
+def _errors(negll, G, rnl, rG):
+    ll_err = np.max(np.abs(negll - rnl) / np.abs(rnl))
+    g_err = np.max(np.linalg.norm(G - rG, axis=0) / np.linalg.norm(rG, axis=0))
+    return ll_err, g_err
+
+
+@pytest.mark.parametrize("prec", ["f64", "bf16x2", "fp32"])
+@pytest.mark.parametrize("N,T,B", [(7500, 50, 300), (1000, 17, 1), (60000, 700, 128)])
+def test_batched_against_oracle(sfh, O, gpu_ctx, prec, N, T, B):
+    models, data, X = _problem(N, T, B, seed=N + T + B)
+    tpl = sfh.HIPTemplates(models)
+    hd = sfh.HIPData(data)
+    negll, G = sfh.fg_batched(X, tpl, hd, precision=prec)
+    rnl, rG = O.fg_batched(models, data, X, nthreads=8)
+    ll_err, g_err = _errors(negll, G, rnl, rG)
+    print(f"{prec} N={N} T={T} B={B}: LL rel-err {ll_err:.2e}, grad norm-rel-err {g_err:.2e}")
+    lb, gb = BOUNDS[prec]
+    assert ll_err <= lb
+    assert g_err <= gb
+
+
+def test_batched_without_gradient_and_device(sfh, O, gpu_ctx):
+    import torch
+    models, data, X = _problem(5000, 40, 200, seed=9)
+    tpl = sfh.HIPTemplates(models)
+    hd = sfh.HIPData(data)
+    negll, G = sfh.fg_batched(X, tpl, hd, precision="f64", want_G=False)
+    assert G is None
+    dX = torch.from_numpy(np.asfortranarray(X).ravel(order="F")).cuda()
+    dn = torch.empty(200, dtype=torch.float64, device="cuda")
+    dG = torch.empty(40 * 200, dtype=torch.float64, device="cuda")
+    sfh.fg_batched_device(tpl, hd, dX, 200, dn, dG, precision="f64")
+    gpu_ctx.synchronize()
+    assert np.array_equal(dn.cpu().numpy(), negll)
+    rnl, rG = O.fg_batched(models, data, X, nthreads=8)
+    assert np.max(np.abs(negll - rnl) / np.abs(rnl)) <= 1e-9
+
+
+def test_batched_single_walker_matches_plain(sfh, gpu_ctx):
+    models, data, X = _problem(20000, 64, 1, seed=4)
+    tpl = sfh.HIPTemplates(models)
+    negll, G = sfh.fg_batched(X, tpl, data, precision="f64")
+    Gp = np.empty(64)
+    nl = sfh.fg_(True, Gp, X[:, 0], tpl, data)
+    assert abs(negll[0] - nl) <= 1e-9 * abs(nl)
+    assert np.linalg.norm(G[:, 0] - Gp) <= 1e-4 * np.linalg.norm(Gp)
+
+
+def test_batched_argument_errors(sfh, gpu_ctx):
+    tpl = sfh.HIPTemplates(np.ones((100, 4), dtype=np.float32))
+    with pytest.raises(sfh.ArgumentError):
+        sfh.fg_batched(np.ones((5, 3)), tpl, np.ones(100))
+    with pytest.raises(sfh.ArgumentError):
+        sfh.fg_batched(np.ones((4, 3)), tpl, np.ones(99))
