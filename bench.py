@@ -1,0 +1,245 @@
+#!/usr/bin/env python
+"""Benchmark of the template-mixture likelihood on one MI355X.
+
+Workload (one "step"): one batched fg! (log-likelihood + gradient) of 4096
+coefficient vectors (HMC / ensemble walkers, log-normal 0.3 dex around the
+truth) against 700 templates x 60,000 bins (200 x 300 Hess grid) and a
+Poisson-drawn observed diagram with ~1e6 stars.  The templates are deposited
+on the GPU first (700 synthetic isochrones of 2000 points, covariant kernel
+with colour-magnitude correlation 0.6), which is also timed (templates/s).
+
+Prints ONE JSON line.  `value` is walker evaluations per second of the
+FP64-MFMA form (log-likelihood at FP64 accuracy); the split-BF16 and FP32
+forms, the single-walker ("plain") path, template deposition, errors against
+the FP64 oracle and the oracle timed on the host cores are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6, "bf16": 2500.0, "fp32": 157.3, "hbm": 8000.0}   # TFLOP/s, GB/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--walkers", type=int, default=4096)
+    ap.add_argument("--precision", default="f64", choices=["f64", "bf16x2", "fp32"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class Timer:
+    """HIP events on the stream the library launches on (torch's current stream)."""
+
+    def __init__(self, torch):
+        self.torch = torch
+
+    def __enter__(self):
+        self.s = self.torch.cuda.Event(enable_timing=True)
+        self.e = self.torch.cuda.Event(enable_timing=True)
+        self.torch.cuda.synchronize()
+        self.s.record()
+        return self
+
+    def __exit__(self, *a):
+        self.e.record()
+        self.torch.cuda.synchronize()
+        self.ms = self.s.elapsed_time(self.e)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
+              "described in DESIGN.md but not run.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import __graft_entry__ as ge
+    sfh = ge.load_package()
+    from sfh_amd import synth
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.init()
+    # A dedicated (default-priority) stream shared by torch and the library, so
+    # torch's events bracket exactly the library's kernels.  (torch's default
+    # stream has handle 0, which the library would replace by its own stream.)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    ctx = sfh.Context(0, stream=stream.cuda_stream)
+    T_, B = 700, args.walkers
+    res = {}
+
+    # ---------------------------------------------------- templates (deposition)
+    edges = synth.cfg1_edges()
+    nbins = (edges[0].length - 1) * (edges[1].length - 1)
+    specs = synth.cfg1_specs()
+    t0 = time.time()
+    pc = [synth.template_points(s, 2000, edges, 120.0, correlation=0.6) for s in specs]
+    prep_s = time.time() - t0
+    batch = sfh.PointBatch([p for p, _ in pc], edges, cmult_list=[c for _, c in pc])
+    log(f"host prep of {len(specs)} templates: {prep_s:.2f} s, {batch.npoints} kernel points")
+    out, pairs = sfh.deposit_batch(batch, ctx=ctx, return_pairs=True)
+    ctx.set_timing(True)
+    ctx.reset_stats()
+    nrep = 3
+    with Timer(torch) as tm:
+        for _ in range(nrep):
+            sfh.deposit_batch(batch, out=out, ctx=ctx)
+    dep_ms = tm.ms / nrep
+    kms, kn = ctx.kernel_stats("deposit_cov")
+    dep_kernel_ms = kms / max(kn, 1)
+    ctx.set_timing(False)
+    dep_bytes = nbins * T_ * 4 + batch.npoints * 6 * 8
+    res["templates"] = {
+        "templates_per_s": T_ / (dep_ms * 1e-3), "ms_per_700": dep_ms, "kernel_ms": dep_kernel_ms,
+        "pairs": pairs, "pairs_per_s": pairs / (dep_kernel_ms * 1e-3),
+        "hbm_GBps_algorithmic": dep_bytes / (dep_kernel_ms * 1e-3) / 1e9, "host_prep_s": prep_s,
+        "note": "VALU/transcendental bound: ~3 exp + 3 polynomial (or erf) evaluations per (point, bin) pair"}
+    log(f"deposition: {dep_ms:.2f} ms / 700 templates, {pairs:.3e} pairs")
+
+    tpl = sfh.templates_from_device(out, nbins, ctx=ctx)
+    models = np.asfortranarray(out.cpu().numpy().T)        # nbins x 700 FP32 (host copy for the oracle)
+
+    # ------------------------------------------------------------- data, walkers
+    rng = np.random.default_rng(20261202)
+    x_true = rng.lognormal(0.0, 1.0, T_)
+    comp = np.empty(nbins)
+    sfh.composite_(comp, x_true, tpl)
+    x_true *= 1e6 / comp.sum()
+    sfh.composite_(comp, x_true, tpl)
+    data = synth.poisson_data(comp, seed=7)
+    hd = sfh.HIPData(data, ctx=ctx)
+    X = synth.walkers(x_true, B, 0.3, seed=11)
+    dX = torch.from_numpy(np.asfortranarray(X).ravel(order="F")).cuda()
+    dnl = torch.empty(B, dtype=torch.float64, device="cuda")
+    dG = torch.empty(T_ * B, dtype=torch.float64, device="cuda")
+
+    def batched(prec, steps, warmup):
+        for _ in range(warmup):
+            sfh.fg_batched_device(tpl, hd, dX, B, dnl, dG, precision=prec)
+        ctx.set_timing(True)
+        ctx.reset_stats()
+        with Timer(torch) as tm:
+            for _ in range(steps):
+                sfh.fg_batched_device(tpl, hd, dX, B, dnl, dG, precision=prec)
+        ctx.synchronize()
+        stats = {k: ctx.kernel_stats(k) for k in ("batched_prep_x", "batched_composite_ll",
+                                                    "batched_gradient", "batched_finish")}
+        ctx.set_timing(False)
+        return tm.ms / steps, stats
+
+    # oracle on a sample of walkers (errors of every form)
+    O = ge.load_oracle()
+    ncpu = os.cpu_count() or 1
+    nsample = 8
+    rnl, rG = O.fg_batched(models, data, X[:, :nsample], nthreads=ncpu)
+
+    def errors():
+        nl = dnl[:nsample].cpu().numpy()
+        G = dG.view(B, T_)[:nsample].cpu().numpy().T
+        return {"ll_rel_err_max": float(np.max(np.abs(nl - rnl) / np.abs(rnl))),
+                "grad_norm_rel_err_max": float(np.max(np.linalg.norm(G - rG, axis=0) / np.linalg.norm(rG, axis=0)))}
+
+    ms, stats = batched(args.precision, args.steps, args.warmup)
+    err = errors()
+    value = B / (ms * 1e-3)
+    comp_ms = stats["batched_composite_ll"][0] / max(stats["batched_composite_ll"][1], 1)
+    flops = 2.0 * T_ * nbins * B
+    peak_key = {"f64": "fp64", "bf16x2": "bf16", "fp32": "fp32"}[args.precision]
+    achieved = flops / (comp_ms * 1e-3) / 1e12
+    log(f"batched {args.precision}: {ms:.3f} ms/step, {value:.4g} evals/s, composite kernel {comp_ms:.3f} ms "
+        f"({achieved:.1f} TFLOP/s), LL rel-err {err['ll_rel_err_max']:.2e}")
+    kernels = {k: (v[0] / max(v[1], 1)) for k, v in stats.items()}
+    roofline = {"bound": "mfma", "kernel": "k_gemm_f64_ll" if args.precision == "f64" else "k_gemm<LL>",
+                "achieved": achieved, "peak": PEAK[peak_key], "unit": "TFLOP/s",
+                "frac": achieved / PEAK[peak_key], "traffic": None,
+                "algorithmic": f"2*T*N*B = {flops:.3e} FLOP per launch (composite), {PEAK[peak_key]} TFLOP/s "
+                               f"dense {peak_key} peak"}
+
+    extras = {}
+    if not args.no_extras:
+        for prec in ("bf16x2", "fp32", "f64"):
+            if prec == args.precision:
+                continue
+            m2, st2 = batched(prec, max(3, args.steps // 4), 1)
+            extras[f"batched_{prec}"] = {"evals_per_s": B / (m2 * 1e-3), "ms_per_step": m2,
+                                         "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in st2.items()},
+                                         **errors()}
+            log(f"batched {prec}: {m2:.3f} ms/step, {B / (m2 * 1e-3):.4g} evals/s, {extras[f'batched_{prec}']}")
+        # the plain (single coefficient vector) path
+        d1 = torch.from_numpy(X[:, 0].copy()).cuda()
+        n1 = torch.empty(1, dtype=torch.float64, device="cuda")
+        g1 = torch.empty(T_, dtype=torch.float64, device="cuda")
+        for _ in range(5):
+            sfh.fg_device(tpl, hd, d1, n1, g1)
+        nplain = 200
+        ctx.set_timing(True)
+        ctx.reset_stats()
+        with Timer(torch) as tm:
+            for _ in range(nplain):
+                sfh.fg_device(tpl, hd, d1, n1, g1)
+        pk = {k: ctx.kernel_stats(k) for k in ("composite_partial", "composite_finish", "grad_partial", "fg_final")}
+        ctx.set_timing(False)
+        pms = tm.ms / nplain
+        plain_bytes = 2 * T_ * nbins * 4 + nbins * 12
+        cpk = pk["composite_partial"][0] / max(pk["composite_partial"][1], 1)
+        gpk = pk["grad_partial"][0] / max(pk["grad_partial"][1], 1)
+        extras["plain"] = {
+            "evals_per_s": 1e3 / pms, "ms_per_eval": pms,
+            "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in pk.items()},
+            "hbm_GBps_algorithmic": plain_bytes / (pms * 1e-3) / 1e9,
+            "composite_kernel_GBps": T_ * nbins * 4 / (cpk * 1e-3) / 1e9,
+            "gradient_kernel_GBps": T_ * nbins * 4 / (gpk * 1e-3) / 1e9,
+            "ll_rel_err": float(abs(n1.item() - rnl[0]) / abs(rnl[0]))}
+        extras["batched_over_plain"] = value / extras["plain"]["evals_per_s"]
+        log(f"plain: {pms:.4f} ms/eval ({1e3 / pms:.4g} evals/s); batched/plain = {extras['batched_over_plain']:.1f}x")
+
+    cpu = None
+    if not args.no_cpu:
+        # oracle fg! over walkers, all host cores, bounded to ~cpu_seconds
+        t0 = time.time()
+        O.fg_batched(models, data, X[:, :2], nthreads=ncpu)
+        per = (time.time() - t0) / 2
+        n = int(max(2, min(B, args.cpu_seconds / max(per, 1e-6))))
+        t0 = time.time()
+        O.fg_batched(models, data, X[:, :n], nthreads=ncpu)
+        dt = time.time() - t0
+        cpu = {"value": n / dt, "unit": "evals/s", "cores": ncpu, "kind": "port",
+               "sample": f"{n} of the {B} walkers, same 700 x 60,000 problem: the C FP64 restatement of "
+                         f"fg! (OpenMP over bins and templates), {dt:.1f} s"}
+        log(f"cpu oracle: {n / dt:.3g} evals/s on {ncpu} threads")
+
+    line = {
+        "metric": "loglike+grad evals/s (700 tpl×60k bins) batched vs plain; templates/s; HBM GB/s",
+        "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"f64": "f64", "bf16x2": "bf16", "fp32": "f32"}[args.precision],
+        "data": "synthetic (seeded isochrones deposited on the GPU; Poisson data; log-normal walkers)",
+        "config": {"workload": f"batched fg! of {B} walkers, 700 templates x 60,000 bins "
+                               f"({args.precision}: composite on the matrix cores, FP64 log-likelihood)",
+                   "templates": T_, "bins": nbins, "walkers": B, "precision": args.precision},
+        "roofline": roofline, "cpu_baseline": cpu, "errors": err, "kernel_ms": kernels,
+        "templates": res["templates"], **extras,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
