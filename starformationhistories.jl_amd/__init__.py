@@ -15,6 +15,9 @@ from ._lib import ArgumentError, DeviceError, InexactError  # noqa: E402
 from .core import (Context, HIPData, HIPTemplates, composite_, default_context, fg_,  # noqa: E402
                    fg_batched, fg_batched_device, fg_device, grad_loglikelihood,
                    grad_loglikelihood_, loglikelihood, stack_models)
+from .fitting import (HMCModel, LogTransformFTResult, MCMCModel, construct_x0,  # noqa: E402
+                      fit_templates, fit_templates_fast, fit_templates_lbfgsb, mcmc_sample,
+                      renormalize_x0)
 from .templates import (Histogram, PointBatch, Range, bin_cmd_smooth, calculate_edges,  # noqa: E402
                         calculate_weights, deposit_batch, edges_range, exp_photerr,
                         histogram_data, histogram_pix, interpolate_mini, Martin2016_complete,
@@ -28,5 +31,7 @@ __all__ = [
     "PointBatch", "Range", "bin_cmd_smooth", "calculate_edges", "calculate_weights",
     "deposit_batch", "edges_range", "exp_photerr", "histogram_data", "histogram_pix",
     "interpolate_mini", "Martin2016_complete", "midpoints", "mini_spacing", "partial_cmd_smooth",
-    "partial_cmd_smooth_points", "templates_from_device",
+    "partial_cmd_smooth_points", "templates_from_device", "HMCModel", "LogTransformFTResult",
+    "MCMCModel", "construct_x0", "fit_templates", "fit_templates_fast", "fit_templates_lbfgsb",
+    "mcmc_sample", "renormalize_x0",
 ]
