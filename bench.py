@@ -39,7 +39,22 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (calibrates HBM counters)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch
+    calibrate the counters read for the path's kernels."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 4, "dispatches": 3}), flush=True)
 
 
 def log(msg):
@@ -67,6 +82,9 @@ class Timer:
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
               "described in DESIGN.md but not run.", file=sys.stderr)
@@ -167,9 +185,16 @@ def main():
     log(f"batched {args.precision}: {ms:.3f} ms/step, {value:.4g} evals/s, composite kernel {comp_ms:.3f} ms "
         f"({achieved:.1f} TFLOP/s), LL rel-err {err['ll_rel_err_max']:.2e}")
     kernels = {k: (v[0] / max(v[1], 1)) for k, v in stats.items()}
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if args.precision == "f64" and os.path.exists(tfile):
+        # calibrated FETCH_SIZE + WRITE_SIZE of this kernel from the committed rocprofv3 --pmc runs
+        t = json.load(open(tfile)).get("k_gemm_f64_ll")
+        if t:
+            traffic = t["read_bytes"] + t["write_bytes"]
     roofline = {"bound": "mfma", "kernel": "k_gemm_f64_ll" if args.precision == "f64" else "k_gemm<LL>",
                 "achieved": achieved, "peak": PEAK[peak_key], "unit": "TFLOP/s",
-                "frac": achieved / PEAK[peak_key], "traffic": None,
+                "frac": achieved / PEAK[peak_key], "traffic": traffic,
                 "algorithmic": f"2*T*N*B = {flops:.3e} FLOP per launch (composite), {PEAK[peak_key]} TFLOP/s "
                                f"dense {peak_key} peak"}
 
