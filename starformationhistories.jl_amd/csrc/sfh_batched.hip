@@ -254,6 +254,51 @@ __global__ __launch_bounds__(256) void k_prep_x(const double* __restrict__ X, in
     }
 }
 
+
+// ------------------------------------------------- LDS ring filled by DMA
+// gfx950's global_load_lds_dwordx4 writes 16 bytes per lane straight into LDS
+// at (wave-uniform base + 16 * lane), so a 1 KiB fragment chunk is one
+// instruction per wave and needs no staging registers.  Completion is tracked
+// with vmcnt; a bare s_barrier (no fence, which would force vmcnt(0)) then
+// publishes the stage to the other waves.
+#define SFH_LDS(p) ((__attribute__((address_space(3))) void*)(p))
+
+__device__ inline void dma_chunk(const u32x4* g_chunk, u32x4* lds_chunk, int lane)
+{
+    __builtin_amdgcn_global_load_lds(g_chunk + lane, SFH_LDS(lds_chunk), 16, 0, 0);
+}
+
+// wait until at most n vector-memory operations of this wave are outstanding
+__device__ inline void wait_vmcnt(int n)
+{
+    switch (n) {
+    case 0: __builtin_amdgcn_s_waitcnt(0xF70); break;
+    case 4: __builtin_amdgcn_s_waitcnt(0xF74); break;
+    case 6: __builtin_amdgcn_s_waitcnt(0xF76); break;
+    case 8: __builtin_amdgcn_s_waitcnt(0xF78); break;
+    case 12: __builtin_amdgcn_s_waitcnt(0xF7C); break;
+    default: __builtin_amdgcn_s_waitcnt(0xF70); break;
+    }
+}
+
+__device__ inline void ring_barrier()
+{
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
+// Barrier for register-staged pipelines: this wave's LDS writes complete
+// (lgkmcnt(0)), global loads still in flight stay in flight (no fence, which
+// would wait for vmcnt(0)).
+__device__ inline void stage_barrier()
+{
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
 // ------------------------------------------------------------------ the GEMM
 
 // hi+lo product table (piece of A, piece of B), smallest first so the FP32
@@ -315,18 +360,20 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm(const uint4* __restrict__ A_, c
 {
     constexpr int PA = pieces_a(M);
     constexpr int PB = pieces_b(M);
-    constexpr int A_CHUNKS = kRB * PA * 64;     // uint4 per stage
-    constexpr int B_CHUNKS = kCB * PB * 64;
-    constexpr int A_PER_T = A_CHUNKS / kWG;
-    constexpr int B_PER_T = B_CHUNKS / kWG;
+    constexpr int CA = kRB * PA;                // A chunks (1 KiB) per K-block
+    constexpr int CB = kCB * PB;                // B chunks per K-block
+    constexpr int CH = CA + CB;                 // chunks per stage
+    constexpr int LPW = CH / 4;                 // DMA instructions per wave per stage
+    // ring depth: the split-K gradient streams R from HBM and wants a deep ring;
+    // the composite GEMMs hit L2 and prefer the occupancy of a shallow one
+    constexpr int S = (M == GRAD_BF16 || M == GRAD_FP32) ? 4 : 2;
     constexpr int KP = (M == GRAD_BF16 || M == GRAD_FP32) ? 8 : 0;
-    static_assert(A_CHUNKS % kWG == 0 && B_CHUNKS % kWG == 0, "stage size");
+    static_assert(CH % 4 == 0, "stage size");
 
-    __shared__ u32x4 sA[2][A_CHUNKS];
-    __shared__ u32x4 sB[2][B_CHUNKS];
+    __shared__ u32x4 ring[S][CH * 64];
+    __shared__ double sLL[kWR][kBN];
     const u32x4* A = reinterpret_cast<const u32x4*>(A_);
     const u32x4* Bm = reinterpret_cast<const u32x4*>(B_);
-    __shared__ double sLL[kWR][kBN];
 
     int rt, ct;
     const int64_t split = blockIdx.y;
@@ -351,45 +398,37 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm(const uint4* __restrict__ A_, c
             }
         }
 
-    auto a_src = [&](int64_t kb) { return A + ((kb * nrb_total + (int64_t)rt * kRB) * PA) * 64; };
-    auto b_src = [&](int64_t kb) { return Bm + ((kb * ncb_total + (int64_t)ct * kCB) * PB) * 64; };
-
-    u32x4 ra[A_PER_T], rb[B_PER_T];
-    if (kb0 < kb1) {
-        const u32x4* pa = a_src(kb0);
-        const u32x4* pb = b_src(kb0);
+    // stage kb: chunk c < CA is A rowblock-piece c of this tile, else B
+    auto issue = [&](int64_t kb) {
+        u32x4* st = ring[(kb - kb0) % S];
 #pragma unroll
-        for (int i = 0; i < A_PER_T; ++i) sA[0][tid + i * kWG] = pa[tid + i * kWG];
-#pragma unroll
-        for (int i = 0; i < B_PER_T; ++i) sB[0][tid + i * kWG] = pb[tid + i * kWG];
-    }
-    __syncthreads();
+        for (int i = 0; i < LPW; ++i) {
+            const int c = wave * LPW + i;
+            const u32x4* g = c < CA ? A + ((kb * nrb_total + (int64_t)rt * kRB) * PA + c) * 64
+                                    : Bm + ((kb * ncb_total + (int64_t)ct * kCB) * PB + (c - CA)) * 64;
+            dma_chunk(g, st + c * 64, lane);
+        }
+    };
+    for (int64_t k = kb0; k < kb0 + S - 1 && k < kb1; ++k) issue(k);
 
-    int cur = 0;
     int since = 0;
     for (int64_t kb = kb0; kb < kb1; ++kb) {
+        const int64_t later = (kb1 - 1 - kb) < (S - 2) ? (kb1 - 1 - kb) : (S - 2);
+        wait_vmcnt((int)later * LPW);
+        ring_barrier();
+        if (kb + S - 1 < kb1) issue(kb + S - 1);
+        const u32x4* st = ring[(kb - kb0) % S];
         const bool more = kb + 1 < kb1;
-        {
-            // prefetch the next K-block (the last iteration re-reads its own,
-            // which keeps the registers unconditionally defined)
-            const int64_t kn = more ? kb + 1 : kb;
-            const u32x4* pa = a_src(kn);
-            const u32x4* pb = b_src(kn);
-#pragma unroll
-            for (int i = 0; i < A_PER_T; ++i) ra[i] = pa[tid + i * kWG];
-#pragma unroll
-            for (int i = 0; i < B_PER_T; ++i) rb[i] = pb[tid + i * kWG];
-        }
         // fragments of this wave
         u32x4 fa[kTR][PA], fb[kTC][PB];
 #pragma unroll
         for (int a = 0; a < kTR; ++a)
 #pragma unroll
-            for (int p = 0; p < PA; ++p) fa[a][p] = sA[cur][((wr * kTR + a) * PA + p) * 64 + lane];
+            for (int p = 0; p < PA; ++p) fa[a][p] = st[((wr * kTR + a) * PA + p) * 64 + lane];
 #pragma unroll
         for (int b = 0; b < kTC; ++b)
 #pragma unroll
-            for (int p = 0; p < PB; ++p) fb[b][p] = sB[cur][((wc * kTC + b) * PB + p) * 64 + lane];
+            for (int p = 0; p < PB; ++p) fb[b][p] = st[(CA + (wc * kTC + b) * PB + p) * 64 + lane];
 
         if constexpr (is_f32(M)) {
 #pragma unroll
@@ -431,15 +470,6 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm(const uint4* __restrict__ A_, c
                         }
             }
         }
-        if (more) {
-            const int nxt = cur ^ 1;
-#pragma unroll
-            for (int i = 0; i < A_PER_T; ++i) sA[nxt][tid + i * kWG] = ra[i];
-#pragma unroll
-            for (int i = 0; i < B_PER_T; ++i) sB[nxt][tid + i * kWG] = rb[i];
-            cur = nxt;
-        }
-        __syncthreads();
     }
     if constexpr (KP == 0) {
 #pragma unroll
@@ -541,8 +571,6 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm(const uint4* __restrict__ A_, c
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
-constexpr int kXS = kBN + 2;   // FP64 coefficient row stride in LDS (pad: no bank conflicts)
-
 // Coefficients X (T x B, column-major FP64) -> [kblock][col tile][16 k][128 walkers] FP64.
 __global__ __launch_bounds__(256) void k_prep_x64(const double* __restrict__ X, int64_t T, int64_t B,
                                                   int64_t nct, int64_t nkb, double* __restrict__ out)
@@ -568,27 +596,28 @@ __device__ inline int a_elem(int r, int k)
 // k_gemm<LL>.  Workgroup tile 128 bins x 128 walkers, 4 waves of 64 x 64, each
 // wave 4 x 4 v_mfma_f64_16x16x4f64 tiles (lane l: A[row l%16][k l/16],
 // B[k l/16][col l%16], D[row l/16 + 4i][col l%16] in accumulator i).  A (FP32 fragment tiles)
-// and X (FP64) K-blocks are double-buffered through LDS; the R tile is
-// re-laid out through LDS into the gradient GEMM's fragment tiles.
+// and X (FP64) K-blocks go global -> registers (two K-blocks ahead, two staging
+// sets) -> one of two LDS stages; the R tile is re-laid out through LDS into the
+// gradient GEMM's fragment tiles.
 __global__ __launch_bounds__(kWG, 1) void k_gemm_f64_ll(const uint4* __restrict__ A_, const double* __restrict__ X,
                                                         int64_t nrb_total, int64_t nct_total, int64_t nkb,
                                                         Swizzle sw, EpiLL el)
 {
-    constexpr int A_U4 = kRB * 2 * 64;          // 512 uint4 = 8 KiB per K-block
-    constexpr int X_D = 16 * kBN;               // 2048 doubles = 16 KiB per K-block
-    constexpr int A_PER_T = A_U4 / kWG;         // 2
-    constexpr int X_PER_T = X_D / 2 / kWG;      // 4 (as pairs of doubles)
-    // 64 KiB: two stages of A (16 KiB) + X (2 x 16.6 KiB) during the K loop,
-    // the 128 x 128 FP32 R tile afterwards.
-    __shared__ __attribute__((aligned(16))) unsigned char smem[65536];
+    constexpr int CA = kRB * 2;                 // 8 KiB: FP32 A tile of one K-block (two halves per rowblock)
+    constexpr int CX = 16;                      // 16 KiB: one 128-walker FP64 row per k
+    constexpr int STAGE_U4 = (CA + CX) * 64;    // 1536 16-byte words per K-block
+    constexpr int NPER = STAGE_U4 / kWG;        // 6 per thread
+    constexpr int NPA = CA * 64 / kWG;          // the first 2 of them from the A tile
+    static_assert(STAGE_U4 % kWG == 0 && (CA * 64) % kWG == 0, "stage split");
+    // two LDS stages (48 KiB) during the K loop; the 128 x 128 FP32 R tile (64 KiB) afterwards
+    __shared__ u32x4 lds[4096];
     __shared__ double sLL[kWR][kBN];
-    float* sAf = reinterpret_cast<float*>(smem);                       // [2][A_U4*4]
-    double* sX = reinterpret_cast<double*>(smem + 2 * A_U4 * 16);      // [2][16*kXS]
-    float* sR = reinterpret_cast<float*>(smem);                        // [128 bins][kBN]
+    float* sR = reinterpret_cast<float*>(lds);
 
     int rt, ct;
     if (!swizzle(sw, blockIdx.x, &rt, &ct)) return;
     const u32x4* A = reinterpret_cast<const u32x4*>(A_);
+    const u32x4* Xv = reinterpret_cast<const u32x4*>(X);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave / kWC, wc = wave % kWC;
     const int lr = lane & 15, lk = lane >> 4;
@@ -599,45 +628,23 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm_f64_ll(const uint4* __restrict_
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-    auto a_src = [&](int64_t kb) { return A + ((kb * nrb_total + (int64_t)rt * kRB) * 2) * 64; };
-    auto x_src = [&](int64_t kb) { return reinterpret_cast<const f64x2*>(X + (kb * nct_total + ct) * (int64_t)X_D); };
-
-    u32x4 ra[A_PER_T];
-    f64x2 rx[X_PER_T];
-    {
-        const u32x4* pa = a_src(0);
-        const f64x2* px = x_src(0);
-#pragma unroll
-        for (int i = 0; i < A_PER_T; ++i) ra[i] = pa[tid + i * kWG];
-#pragma unroll
-        for (int i = 0; i < X_PER_T; ++i) rx[i] = px[tid + i * kWG];
-    }
-#define SFH_STAGE_F64(buf)                                                                  \
-    do {                                                                                    \
-        u32x4* dA_ = reinterpret_cast<u32x4*>(sAf + (buf) * A_U4 * 4);                      \
-        _Pragma("unroll") for (int i = 0; i < A_PER_T; ++i) dA_[tid + i * kWG] = ra[i];     \
-        _Pragma("unroll") for (int i = 0; i < X_PER_T; ++i) {                               \
-            const int u_ = tid + i * kWG;                                                   \
-            const int k_ = (2 * u_) / kBN, c_ = (2 * u_) % kBN;                             \
-            *reinterpret_cast<f64x2*>(sX + (buf) * 16 * kXS + k_ * kXS + c_) = rx[i];       \
-        }                                                                                   \
+    // element u of a stage: the A tile (contiguous) then the X rows (contiguous)
+    const int64_t a_row0 = (int64_t)rt * kRB * 2 * 64;
+    const int64_t a_stride = nrb_total * 2 * 64, x_stride = nct_total * CX * 64;
+    const u32x4* x0 = Xv + (int64_t)ct * CX * 64;
+#define SFH_GLOAD(kb, R)                                                                         \
+    do {                                                                                         \
+        const u32x4* pa_ = A + (kb) * a_stride + a_row0;                                         \
+        const u32x4* px_ = x0 + (kb) * x_stride;                                                 \
+        _Pragma("unroll") for (int i_ = 0; i_ < NPER; ++i_)                                      \
+            R[i_] = i_ < NPA ? pa_[tid + i_ * kWG] : px_[tid + (i_ - NPA) * kWG];                \
     } while (0)
-    SFH_STAGE_F64(0);
-    __syncthreads();
-    int cur = 0;
-    for (int64_t kb = 0; kb < nkb; ++kb) {
-        const bool more = kb + 1 < nkb;
-        {
-            const int64_t kn = more ? kb + 1 : kb;
-            const u32x4* pa = a_src(kn);
-            const f64x2* px = x_src(kn);
-#pragma unroll
-            for (int i = 0; i < A_PER_T; ++i) ra[i] = pa[tid + i * kWG];
-#pragma unroll
-            for (int i = 0; i < X_PER_T; ++i) rx[i] = px[tid + i * kWG];
-        }
-        const float* cA = sAf + cur * A_U4 * 4;
-        const double* cX = sX + cur * 16 * kXS;
+#define SFH_LSTORE(st, R)                                                                        \
+    _Pragma("unroll") for (int i_ = 0; i_ < NPER; ++i_) lds[(st) * STAGE_U4 + tid + i_ * kWG] = R[i_]
+
+    auto compute = [&](int st) {
+        const float* cA = reinterpret_cast<const float*>(lds + st * STAGE_U4);
+        const double* cX = reinterpret_cast<const double*>(lds + st * STAGE_U4 + CA * 64);
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
             const int k = 4 * ks + lk;
@@ -645,20 +652,46 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm_f64_ll(const uint4* __restrict_
 #pragma unroll
             for (int a = 0; a < 4; ++a) fa[a] = (double)cA[a_elem(wr * 64 + a * 16 + lr, k)];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) fb[b] = cX[k * kXS + wc * 64 + b * 16 + lr];
+            for (int b = 0; b < 4; ++b) fb[b] = cX[k * kBN + wc * 64 + b * 16 + lr];
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
         }
-        if (more) {
-            SFH_STAGE_F64(cur ^ 1);
-            cur ^= 1;
-        }
-        __syncthreads();
+    };
+
+    // staging set A holds K-block kb+1, set B holds kb+2 (loads two steps ahead;
+    // past the end the last K-block is re-read, keeping the registers defined)
+    u32x4 ra[NPER], rb[NPER];
+    const int64_t last = nkb - 1;
+    SFH_GLOAD(0, ra);
+    SFH_LSTORE(0, ra);
+    SFH_GLOAD(1 < last ? 1 : last, ra);
+    // every load into ra issues before any into rb, as in the loop, so the
+    // waits the compiler places at the loop's stores count only ra's loads
+    __builtin_amdgcn_sched_barrier(0);
+    SFH_GLOAD(2 < last ? 2 : last, rb);
+    stage_barrier();
+    // no conditional stores inside the loop: a skipped store would leave its
+    // staging registers with a load in flight on one path, and the wait the
+    // compiler then places before the next load into them drains every load
+    __builtin_assume(nkb >= 2);                 // the host pads the template axis to two K-blocks or more
+    int64_t kb = 0;
+    for (; kb + 1 < nkb; kb += 2) {
+        compute(0);
+        SFH_LSTORE(1, ra);                      // K-block kb+1
+        stage_barrier();
+        SFH_GLOAD(kb + 3 < last ? kb + 3 : last, ra);
+        compute(1);
+        SFH_LSTORE(0, rb);                      // K-block kb+2 (the last one again when kb+2 == nkb)
+        stage_barrier();
+        SFH_GLOAD(kb + 4 < last ? kb + 4 : last, rb);
     }
-#undef SFH_STAGE_F64
+    if (kb < nkb) compute(0);                   // odd nkb: stage 0 holds K-block nkb-1
+#undef SFH_GLOAD
+#undef SFH_LSTORE
+    __syncthreads();    // every wave is done with the stages before they become the R tile
 
     // epilogue: LL terms (FP64) and R into LDS
     double part[4] = {0.0, 0.0, 0.0, 0.0};
@@ -772,7 +805,7 @@ static int get_operands(sfh_ctx* ctx, const sfh_templates* tpl, int mode, Batche
     const int64_t ld = cdiv(N, 256) * 256;
     if (!o.ex) {
         o.npad = cdiv(N, kBM) * kBM;
-        o.tpad1 = cdiv(T, 16) * 16;
+        o.tpad1 = std::max<int64_t>(cdiv(T, 16), 2) * 16;   // >= 2 K-blocks: the FP64 composite's loop runs in pairs
         o.tpad2 = cdiv(T, kBM) * kBM;
         SFH_HIP_TRY(ctx, hipMalloc(&o.ex, (size_t)T * sizeof(int32_t)));
         SFH_HIP_TRY(ctx, hipMalloc(&o.ex0, (size_t)T * sizeof(int32_t)));
