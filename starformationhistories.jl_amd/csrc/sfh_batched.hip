@@ -117,6 +117,71 @@ __device__ inline uint4 pack8(const uint16_t* h)
 // bin offset inside a 16-bin K-block for fragment slot j of lane half h
 __device__ inline int gperm(int h, int j) { return j < 4 ? 4 * h + j : 8 + 4 * h + (j - 4); }
 
+// ------------------------------------------------- FP64 log-likelihood term
+//
+// The batched epilogues evaluate one log and one quotient per (bin, walker),
+// 2.5e8 of each per 4096-walker launch at 60,000 bins, so they use short FP64
+// sequences instead of the library's extra-precise log and IEEE division.
+
+// q = n / c to within one rounding: hardware reciprocal, two Newton steps and
+// one residual correction.  Valid for c in [2^-1000, 2^1000]; outside that
+// the IEEE division is used.
+__device__ inline double quot(double n, double c)
+{
+    if (!(c >= 0x1p-1000 && c <= 0x1p1000)) return n / c;
+    double r = __builtin_amdgcn_rcp(c);
+    r = fma(fma(-c, r, 1.0), r, r);
+    r = fma(fma(-c, r, 1.0), r, r);
+    const double q = n * r;
+    return fma(r, fma(-c, q, n), q);
+}
+
+// Natural log of a finite x > 0 (any other x: unspecified finite or NaN
+// result), relative error a few units in the last place.  x = 2^e m with
+// m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1), |s| <= 0.1716,
+// by its Taylor series 2s (1 + z/3 + z^2/5 + ... + z^9/19), z = s^2 (the first
+// omitted term is below 2.5e-17 relative); e ln 2 in two parts.
+__device__ inline double log_pos(double x)
+{
+    double m = __builtin_amdgcn_frexp_mant(x);          // [0.5, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);
+    if (m < 0.70710678118654752440) {
+        m += m;
+        e -= 1;
+    }
+    const double f = m - 1.0;                           // exact
+    const double d = m + 1.0;
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(fma(-d, r, 1.0), r, r);
+    r = fma(fma(-d, r, 1.0), r, r);
+    double s = f * r;
+    s = fma(r, fma(-d, s, f), s);
+    const double z = s * s;
+    double p = 1.0 / 19.0;
+    p = fma(p, z, 1.0 / 17.0);
+    p = fma(p, z, 1.0 / 15.0);
+    p = fma(p, z, 1.0 / 13.0);
+    p = fma(p, z, 1.0 / 11.0);
+    p = fma(p, z, 1.0 / 9.0);
+    p = fma(p, z, 1.0 / 7.0);
+    p = fma(p, z, 1.0 / 5.0);
+    p = fma(p, z, 1.0 / 3.0);
+    const double t = s + s;
+    const double lm = fma(t * z, p, t);
+    const double de = (double)e;
+    return fma(de, 6.93147180369123816490e-01, fma(de, 1.90821492927058770002e-10, lm));
+}
+
+// One bin's contribution to logL, n - c - n log(n / c) (-c for n <= 0), and
+// R = 1 - n / c, for a composite already clamped to c >= eps
+// (src/fitting/fitting_base.jl:84-96, 274-280).
+__device__ inline void ll_term(double n, double c, double& ll, float& r)
+{
+    const double q = quot(n, c);
+    ll += (n > 0.0) ? (n - c - n * log_pos(q)) : -c;
+    r = (float)(1.0 - q);
+}
+
 // ------------------------------------------------------------ layout builders
 
 // Per-template power-of-two scale: e_t = exponent of max_i |C[i,t]| (0 if the
@@ -501,10 +566,7 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm(const uint4* __restrict__ A_, c
                     float r = 0.f;
                     if (bin < el.nbins && wok) {
                         const double m = acc64[a][b][i];
-                        const double c = m > el.eps ? m : el.eps;
-                        const double n = el.data[bin];
-                        part[b] += (n > 0.0) ? (n - c - n * log(n / c)) : -c;
-                        r = (float)(1.0 - n / c);
+                        ll_term(el.data[bin], m > el.eps ? m : el.eps, part[b], r);
                     }
                     rv[i] = r;
                 }
@@ -708,10 +770,7 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm_f64_ll(const uint4* __restrict_
                 float r = 0.f;
                 if (bin < el.nbins && w < el.B) {
                     const double m = acc[a][b][i];
-                    const double c = m > el.eps ? m : el.eps;
-                    const double n = el.data[bin];
-                    part[b] += (n > 0.0) ? (n - c - n * log(n / c)) : -c;
-                    r = (float)(1.0 - n / c);
+                    ll_term(el.data[bin], m > el.eps ? m : el.eps, part[b], r);
                 }
                 sR[rl * kBN + cl] = r;
             }

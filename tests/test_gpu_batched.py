@@ -45,6 +45,33 @@ def test_batched_against_oracle(sfh, O, gpu_ctx, prec, N, T, B):
     assert g_err <= gb
 
 
+@pytest.mark.parametrize("prec", ["f64", "bf16x2", "fp32"])
+def test_batched_extreme_values(sfh, O, gpu_ctx, prec):
+    """Empty bins, counts up to 1e7, an all-zero template, composites clamped
+    at eps (all-zero and 1e-30 walkers) and huge coefficients: the
+    log-likelihood epilogue's quotient and log over their whole range."""
+    from sfh_amd import synth
+    N, T, B = 3000, 20, 64
+    models = synth.synthetic_templates(N, T, seed=21)
+    models[:, 3] = 0.0
+    rng = np.random.default_rng(22)
+    data = np.where(rng.uniform(size=N) < 0.3, 0.0, 10 ** rng.uniform(-1, 7, N))
+    X = 10 ** rng.uniform(0, 6, (T, B))
+    X[:, 0] = 0.0
+    X[:, 1] = 1e-30
+    X[:, 2] = 1e10
+    X[:5, 3] = 0.0
+    tpl = sfh.HIPTemplates(models)
+    negll, G = sfh.fg_batched(X, tpl, data, precision=prec)
+    rnl, rG = O.fg_batched(models, data, X, nthreads=8)
+    ll_err, g_err = _errors(negll, G, rnl, rG)
+    print(f"{prec} extreme values: LL rel-err {ll_err:.2e}, grad norm-rel-err {g_err:.2e}")
+    lb, gb = BOUNDS[prec]
+    assert np.all(np.isfinite(negll)) and np.all(np.isfinite(G))
+    assert ll_err <= lb
+    assert g_err <= gb
+
+
 def test_batched_without_gradient_and_device(sfh, O, gpu_ctx):
     import torch
     models, data, X = _problem(5000, 40, 200, seed=9)
