@@ -158,6 +158,90 @@ int sfh_fg_batched_device(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data
                           const double* d_X, int64_t B, int precision, double eps,
                           double* d_negLL, double* d_G);
 
+/* ------------------------------- P2: many independent fits of uneven size (new) */
+
+/* A group of independent fits (fit f: templates[f], data[f]; any mix of
+ * template and bin counts, each fit at most 1024 templates), evaluated in one
+ * persistent launch per fg! of every fit and optimised on the device.  The
+ * template and data handles stay owned by the caller and must outlive the
+ * group.  Vectors of all fits are concatenated: fit f's ntemplates values
+ * start at offset sum_{g<f} ntemplates(g); its ntemplates^2 matrix at
+ * sum_{g<f} ntemplates(g)^2. */
+typedef struct sfh_multifit sfh_multifit;
+
+int sfh_multifit_create(sfh_ctx* ctx, int64_t nfits, const sfh_templates* const* templates,
+                        const sfh_data* const* data, sfh_multifit** out);
+int sfh_multifit_destroy(sfh_multifit* mf);
+
+/* Work decomposition of the grouped fg! launch: about item_elems template
+ * elements per work item (whole 32-bin blocks of one fit), workgroups of
+ * 64*waves threads (waves in {4, 8, 16}) and wg_per_cu persistent workgroups
+ * per compute unit. */
+int sfh_multifit_get_shape(const sfh_multifit* mf, int64_t* item_elems, int* waves, int* wg_per_cu);
+int sfh_multifit_set_shape(sfh_multifit* mf, int64_t item_elems, int waves, int wg_per_cu);
+/* Times every candidate shape on this group's fits (reps launches each, at
+ * x = ones) and keeps the fastest; returns it (any pointer may be NULL). */
+int sfh_multifit_autotune(sfh_multifit* mf, int reps, int64_t* item_elems, int* waves, int* wg_per_cu);
+
+/* fg!(true, G, coeffs_f, models_f, data_f, composite) of src/fitting/solvers.jl:20-38
+ * for every fit f in one launch: negLL[f] = -logL_f, G (concatenated, may be
+ * NULL) = -∇logL_f.  X: concatenated coefficient vectors. */
+int sfh_multifit_fg(sfh_multifit* mf, const double* X, double eps, double* negLL, double* G);
+int sfh_multifit_fg_device(sfh_multifit* mf, const double* d_X, double eps, double* d_negLL,
+                           double* d_G);
+
+#define SFH_FIT_LOG 0    /* fit_templates: log(coeffs), MAP (Jacobian -sum log x) then MLE */
+#define SFH_FIT_SQRT 1   /* fit_templates_fast: coeffs = theta^2, MLE only */
+#define SFH_OPT_BFGS 0   /* dense inverse-Hessian BFGS (Optim.BFGS; inverse Hessian returned) */
+#define SFH_OPT_LBFGS 1  /* limited-memory BFGS */
+
+#define SFH_FIT_RUNNING 0
+#define SFH_FIT_CONVERGED 1      /* max |gradient| <= g_abstol */
+#define SFH_FIT_MAX_ITERATIONS 2
+#define SFH_FIT_LINESEARCH_FAILED 3
+
+typedef struct {
+    int transform;           /* SFH_FIT_LOG or SFH_FIT_SQRT */
+    int method;              /* SFH_OPT_BFGS or SFH_OPT_LBFGS */
+    int memory;              /* L-BFGS pairs kept (1..32) */
+    int steps_per_graph;     /* optimiser steps per captured HIP graph (1..256) */
+    double g_abstol;         /* Optim.Options g_abstol (default 1e-8), on the transformed variables */
+    int64_t max_iterations;  /* per stage (Optim.Options iterations, default 1000) */
+    double eps;              /* the likelihood's clamp, eps(Float64) by default */
+} sfh_fit_options;
+
+/* Fills *opt with the defaults above (LOG, BFGS, memory 10, 16 steps per graph). */
+int sfh_fit_options_default(sfh_fit_options* opt);
+
+/* Per-fit results, concatenated as described above; any pointer may be NULL.
+ * Stage 0 is the MAP stage of SFH_FIT_LOG, stage 1 the MLE (the only stage of
+ * SFH_FIT_SQRT).  fmin, iterations, evaluations and status hold 2*nfits
+ * values, [stage * nfits + f]. */
+typedef struct {
+    double* mu_map;          /* exp(theta) at the MAP optimum (LOG) */
+    double* mu_mle;          /* coefficients at the MLE optimum */
+    double* invH_map;        /* BFGS inverse Hessian in the transformed variables, MAP stage */
+    double* invH_mle;        /* ... MLE stage */
+    double* fmin;            /* objective at the optimum: -logL - sum(theta) (MAP), -logL (MLE) */
+    int64_t* iterations;
+    int64_t* evaluations;    /* fg! evaluations */
+    int32_t* status;         /* SFH_FIT_* */
+    int64_t steps;           /* out: optimiser steps run (one grouped fg! each) */
+} sfh_fit_results;
+
+/* fit_templates (src/fitting/solvers.jl:172-221) with SFH_FIT_LOG, or
+ * fit_templates_fast (:248-275) with SFH_FIT_SQRT, for every fit of the group
+ * at once: x0 renormalised as renormalize_x0 does (src/fitting/utilities.jl:104-115;
+ * x0 NULL means ones), then quasi-Newton iterations whose line search accepts
+ * a step on LineSearches.HagerZhang's Wolfe / approximate-Wolfe conditions
+ * (delta 0.1, sigma 0.9, epsilon 1e-6; bracketing with safeguarded cubic
+ * steps) from a first trial step of length min(1, |d|)
+ * (LineSearches.InitialStatic(1.0, true)), every fit's fg! evaluated in one
+ * launch per step, steps captured in HIP graphs and replayed until every fit
+ * has stopped. */
+int sfh_multifit_fit(sfh_multifit* mf, const double* x0, const sfh_fit_options* opt,
+                     sfh_fit_results* res);
+
 /* --------------------------------------------------- P1: template deposition */
 
 /* bin_cmd_smooth(colors, mags, color_err, mag_err, cov_mult; weights, edges)

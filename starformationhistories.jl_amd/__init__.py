@@ -18,6 +18,7 @@ from .core import (Context, HIPData, HIPTemplates, composite_, default_context, 
 from .fitting import (HMCModel, LogTransformFTResult, MCMCModel, construct_x0,  # noqa: E402
                       fit_templates, fit_templates_fast, fit_templates_lbfgsb, mcmc_sample,
                       renormalize_x0)
+from .multifit import MultiFit, StageInfo  # noqa: E402
 from .templates import (Histogram, PointBatch, Range, bin_cmd_smooth, calculate_edges,  # noqa: E402
                         calculate_weights, deposit_batch, edges_range, exp_photerr,
                         histogram_data, histogram_pix, interpolate_mini, Martin2016_complete,
@@ -33,5 +34,5 @@ __all__ = [
     "interpolate_mini", "Martin2016_complete", "midpoints", "mini_spacing", "partial_cmd_smooth",
     "partial_cmd_smooth_points", "templates_from_device", "HMCModel", "LogTransformFTResult",
     "MCMCModel", "construct_x0", "fit_templates", "fit_templates_fast", "fit_templates_lbfgsb",
-    "mcmc_sample", "renormalize_x0",
+    "mcmc_sample", "renormalize_x0", "MultiFit", "StageInfo",
 ]

@@ -40,6 +40,23 @@ class sfh_range(C.Structure):
     _fields_ = [("first", C.c_double), ("step", C.c_double), ("length", C.c_int64)]
 
 
+class sfh_fit_options(C.Structure):
+    _fields_ = [("transform", C.c_int), ("method", C.c_int), ("memory", C.c_int), ("steps_per_graph", C.c_int),
+                ("g_abstol", C.c_double), ("max_iterations", C.c_int64), ("eps", C.c_double)]
+
+
+class sfh_fit_results(C.Structure):
+    _fields_ = [("mu_map", C.c_void_p), ("mu_mle", C.c_void_p), ("invH_map", C.c_void_p),
+                ("invH_mle", C.c_void_p), ("fmin", C.c_void_p), ("iterations", C.c_void_p),
+                ("evaluations", C.c_void_p), ("status", C.c_void_p), ("steps", C.c_int64)]
+
+
+SFH_FIT_LOG = 0
+SFH_FIT_SQRT = 1
+SFH_OPT_BFGS = 0
+SFH_OPT_LBFGS = 1
+FIT_STATUS = {0: "running", 1: "converged", 2: "max_iterations", 3: "linesearch_failed"}
+
 _vp = C.c_void_p
 _i64 = C.c_int64
 _dbl = C.c_double
@@ -75,6 +92,15 @@ SIGNATURES = [
     ("sfh_fg_device", _int, [_vp, _vp, _vp, _vp, _int, _dbl, _vp, _vp, _vp]),
     ("sfh_fg_batched", _int, [_vp, _vp, _vp, _vp, _i64, _int, _dbl, _vp, _vp]),
     ("sfh_fg_batched_device", _int, [_vp, _vp, _vp, _vp, _i64, _int, _dbl, _vp, _vp]),
+    ("sfh_multifit_create", _int, [_vp, _i64, _vp, _vp, C.POINTER(_vp)]),
+    ("sfh_multifit_destroy", _int, [_vp]),
+    ("sfh_multifit_get_shape", _int, [_vp, _p64, C.POINTER(_int), C.POINTER(_int)]),
+    ("sfh_multifit_set_shape", _int, [_vp, _i64, _int, _int]),
+    ("sfh_multifit_autotune", _int, [_vp, _int, _p64, C.POINTER(_int), C.POINTER(_int)]),
+    ("sfh_multifit_fg", _int, [_vp, _vp, _dbl, _vp, _vp]),
+    ("sfh_multifit_fg_device", _int, [_vp, _vp, _dbl, _vp, _vp]),
+    ("sfh_fit_options_default", _int, [C.POINTER(sfh_fit_options)]),
+    ("sfh_multifit_fit", _int, [_vp, _vp, C.POINTER(sfh_fit_options), C.POINTER(sfh_fit_results)]),
     ("sfh_bin_cmd_smooth", _int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _int, sfh_range, sfh_range, _vp]),
     ("sfh_bin_cmd_smooth_batch_device", _int,
      [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, sfh_range, sfh_range, _vp, _i64, _p64]),

@@ -463,9 +463,8 @@ int sfh_ctx_create(int device, void* stream, sfh_ctx** out)
     return SFH_OK;
 }
 
-int sfh_ctx_destroy(sfh_ctx* ctx)
+static void ctx_teardown(sfh_ctx* ctx)
 {
-    if (!ctx) return SFH_OK;
     {
         std::lock_guard<std::recursive_mutex> g(ctx->mu);
         hipSetDevice(ctx->device);
@@ -478,6 +477,45 @@ int sfh_ctx_destroy(sfh_ctx* ctx)
         if (ctx->own_stream) hipStreamDestroy(ctx->stream);
     }
     delete ctx;
+}
+
+}  // extern "C"
+
+namespace sfh {
+
+void ctx_retain(sfh_ctx* ctx)
+{
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    ++ctx->children;
+}
+
+void ctx_release(sfh_ctx* ctx)
+{
+    bool last = false;
+    {
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        last = --ctx->children == 0 && ctx->closed;
+    }
+    if (last) ctx_teardown(ctx);
+}
+
+}  // namespace sfh
+
+extern "C" {
+
+// Handles may outlive the call that closes their context (e.g. garbage
+// collection order at interpreter exit): the teardown then waits for the
+// last of them.
+int sfh_ctx_destroy(sfh_ctx* ctx)
+{
+    if (!ctx) return SFH_OK;
+    {
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        if (ctx->closed) return SFH_OK;
+        ctx->closed = true;
+        if (ctx->children > 0) return SFH_OK;
+    }
+    ctx_teardown(ctx);
     return SFH_OK;
 }
 
@@ -562,6 +600,7 @@ static int templates_make(sfh_ctx* ctx, const float* src, bool src_device, int64
         delete t;
         return rc;
     }
+    ctx_retain(ctx);
     *out = t;
     return SFH_OK;
 }
@@ -582,12 +621,15 @@ int sfh_templates_destroy(sfh_templates* t)
 {
     if (!t) return SFH_OK;
     sfh_ctx* ctx = t->ctx;
-    std::lock_guard<std::recursive_mutex> g(ctx->mu);
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    release_batched(t);
-    hipFree(t->d_models);
-    delete t;
+    {
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        hipSetDevice(ctx->device);
+        hipStreamSynchronize(ctx->stream);
+        release_batched(t);
+        hipFree(t->d_models);
+        delete t;
+    }
+    ctx_release(ctx);
     return SFH_OK;
 }
 
@@ -623,6 +665,7 @@ static int data_make(sfh_ctx* ctx, const double* src, bool dev, int64_t nbins, s
         delete d;
         return set_error(ctx, SFH_ERR_DEVICE, "data upload: %s", hipGetErrorString(e));
     }
+    ctx_retain(ctx);
     *out = d;
     return SFH_OK;
 }
@@ -641,11 +684,14 @@ int sfh_data_destroy(sfh_data* d)
 {
     if (!d) return SFH_OK;
     sfh_ctx* ctx = d->ctx;
-    std::lock_guard<std::recursive_mutex> g(ctx->mu);
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    hipFree(d->d_data);
-    delete d;
+    {
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        hipSetDevice(ctx->device);
+        hipStreamSynchronize(ctx->stream);
+        hipFree(d->d_data);
+        delete d;
+    }
+    ctx_release(ctx);
     return SFH_OK;
 }
 

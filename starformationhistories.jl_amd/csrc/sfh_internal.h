@@ -49,6 +49,10 @@ struct sfh_ctx {
     // small pinned host staging buffer
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    // handles (templates, data, fit groups) created on this context and not
+    // yet destroyed; sfh_ctx_destroy defers the teardown until they are gone
+    int64_t children = 0;
+    bool closed = false;
 };
 
 struct sfh_templates {
@@ -92,6 +96,11 @@ int pinned(sfh_ctx* ctx, size_t bytes, void** out);
 void timing_begin(sfh_ctx* ctx, const char* name, hipEvent_t* start_out);
 void timing_end(sfh_ctx* ctx, const char* name, hipEvent_t start);
 void resolve_timing(sfh_ctx* ctx);
+
+// Handle bookkeeping: a handle created on ctx calls ctx_retain; its destroy
+// calls ctx_release last, outside any lock on ctx (it may free ctx).
+void ctx_retain(sfh_ctx* ctx);
+void ctx_release(sfh_ctx* ctx);
 
 // Check the last launch error.
 int check_launch(sfh_ctx* ctx, const char* name);
