@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--fits", type=int, default=256, help="independent fits in the uneven multi-fit run")
+    ap.add_argument("--no-multifit", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates HBM counters)")
     return ap.parse_args()
@@ -78,6 +80,171 @@ class Timer:
         self.e.record()
         self.torch.cuda.synchronize()
         self.ms = self.s.elapsed_time(self.e)
+
+
+def uneven_fits(torch, nfits, seed=20261203):
+    """Seeded uneven fits: ntemplates ~ U[50, 700], nbins ~ U[5k, 60k] (a
+    colour x magnitude grid of aspect 2:3), distance modulus ~ U[24, 30], star
+    total log-uniform in [1e4, 1e6].  Template j of a fit is a main-sequence-like
+    ridge (position set by its age / metallicity index and shifted with the
+    distance modulus, width by a photometric-error scale) with a luminosity
+    function falling to faint magnitudes, unit sum, FP32, generated on the GPU.
+    Data: Poisson draws of the composite at log-normal coefficients."""
+    rng = np.random.default_rng(seed)
+    fits = []
+    for f in range(nfits):
+        T = int(rng.integers(50, 701))
+        N = int(rng.integers(5000, 60001))
+        nx = max(8, int(round(np.sqrt(N * 2.0 / 3.0))))
+        ny = max(8, N // nx)
+        N = nx * ny
+        dmod = rng.uniform(24.0, 30.0)
+        total = 10 ** rng.uniform(4.0, 6.0)
+        g = torch.Generator(device="cuda").manual_seed(int(seed + f))
+        na = max(1, int(round(np.sqrt(T * 2))))
+        j = torch.arange(T, device="cuda", dtype=torch.float32)
+        age = (j % na) / max(na - 1, 1)
+        met = torch.floor(j / na) / max((T + na - 1) // na - 1, 1)
+        u = torch.linspace(0.0, 1.0, nx, device="cuda")[None, :, None]
+        v = torch.linspace(0.0, 1.0, ny, device="cuda")[None, None, :]
+        c0 = (0.15 + 0.5 * age + 0.08 * (dmod - 24.0) / 6.0)[:, None, None]
+        c1 = (0.5 + 0.3 * met)[:, None, None]
+        w = (0.01 + 0.03 * torch.rand(T, generator=g, device="cuda"))[:, None, None]
+        ridge = c0 + c1 * (u - 0.3) ** 2
+        C = torch.exp(-0.5 * ((v - ridge) / w) ** 2) * torch.exp(-2.5 * v) + 1e-7
+        C = (C / C.sum(dim=(1, 2), keepdim=True)).reshape(T, N).contiguous()
+        x = torch.from_numpy(rng.lognormal(0.0, 1.0, T)).cuda()
+        comp = x @ C.double()
+        comp *= total / comp.sum()
+        data = torch.poisson(comp, generator=g)
+        fits.append((C, data, T, N))
+    return fits
+
+
+def cpu_fit_templates(O, models, data, nthreads):
+    """fit_templates (src/fitting/solvers.jl:172-221) on the host: SciPy BFGS
+    (inf-norm gradient tolerance 1e-8, as Optim's g_abstol) around the C FP64
+    oracle's fg!, MAP then MLE in log-coefficients.  Returns MAP/MLE (mu, fmin)
+    and the fg! evaluation count."""
+    from scipy.optimize import minimize
+    T = models.shape[1]
+    x0 = np.ones(T)
+    x0 = x0 * (data.sum() / (models.astype(np.float64) @ x0).sum())
+    nev = [0]
+
+    def obj(th, mle):
+        nev[0] += 1
+        x = np.exp(th)
+        f, G, _ = O.fg(models, data, x, nthreads=nthreads)
+        return (f, G * x) if mle else (f - th.sum(), G * x - 1.0)
+
+    opts = {"gtol": 1e-8, "maxiter": 10000, "norm": np.inf}
+    r1 = minimize(obj, np.log(x0), jac=True, method="BFGS", args=(False,), options=opts)
+    r2 = minimize(obj, r1.x, jac=True, method="BFGS", args=(True,), options=opts)
+    return {"map": (np.exp(r1.x), float(r1.fun)), "mle": (np.exp(r2.x), float(r2.fun)), "evals": nev[0]}
+
+
+def multifit_bench(sfh, torch, ctx, O, args, ncpu):
+    """Uneven batched fits: every fit's fg! in one persistent launch, the
+    fit_templates iterations (BFGS, MAP then MLE) on the device in captured
+    HIP graphs, launch shape from the autotuner; convergence of a sample of
+    fits checked against the same fits done on the host."""
+    from sfh_amd.multifit import MultiFit
+    t0 = time.time()
+    fits = uneven_fits(torch, args.fits)
+    tpls = [sfh.HIPTemplates(device_ptr=C.data_ptr(), shape=(N, T), ctx=ctx) for C, _, T, N in fits]
+    datas = [sfh.HIPData(device_ptr=d.data_ptr(), nbins=N, ctx=ctx) for _, d, _, N in fits]
+    ctx.synchronize()
+    sizes = [(T, N) for _, _, T, N in fits]
+    # host copies of the smallest fits, for the host-side sample
+    order = [int(f) for f in np.argsort([T * N for T, N in sizes])[:16]]
+    host = {f: (fits[f][0].cpu().numpy().T, fits[f][1].cpu().numpy()) for f in order}
+    del fits
+    torch.cuda.empty_cache()
+    mf = MultiFit(tpls, datas, ctx=ctx)
+    gen_s = time.time() - t0
+    elems = float(sum(T * N for T, N in sizes))
+    t0 = time.time()
+    shape = mf.autotune(reps=3)
+    tune_s = time.time() - t0
+    log(f"multifit: {len(sizes)} fits, {elems:.3e} template elements ({elems * 4 / 1e9:.1f} GB), "
+        f"generated in {gen_s:.1f} s; autotuned shape {shape} in {tune_s:.1f} s")
+    # one grouped fg! of every fit: kernel time -> HBM GB/s
+    X = [np.ones(T) for T, _ in sizes]
+    mf.fg(X)
+    ctx.set_timing(True)
+    ctx.reset_stats()
+    for _ in range(5):
+        mf.fg(X)
+    kms, kn = ctx.kernel_stats("multifit_fg")
+    rms, rn = ctx.kernel_stats("multifit_reduce")
+    ctx.set_timing(False)
+    fg_ms = kms / max(kn, 1)
+    fg_GBps = elems * 4 / (fg_ms * 1e-3) / 1e9
+    log(f"multifit grouped fg!: {fg_ms:.3f} ms per launch = {fg_GBps:.0f} GB/s of templates")
+    # the fits
+    ctx.set_timing(True)
+    ctx.reset_stats()
+    t0 = time.time()
+    res = mf.fit_templates()
+    fit_s = time.time() - t0
+    gms, gn = ctx.kernel_stats("multifit_graph")
+    ctx.set_timing(False)
+    st = {}
+    evals = 0
+    work = 0.0
+    for f, r in enumerate(res):
+        for key in ("map", "mle"):
+            st[r[key].result.status] = st.get(r[key].result.status, 0) + 1
+            evals += r[key].result.evaluations
+            work += r[key].result.evaluations * sizes[f][0] * sizes[f][1]
+    iters = np.array([r["map"].result.iterations + r["mle"].result.iterations for r in res])
+    log(f"multifit fit_templates: {fit_s:.2f} s for {len(res)} fits ({len(res) / fit_s:.1f} fits/s), "
+        f"{mf.steps} steps, {evals} fg! evaluations, statuses {st}")
+    out = {"fits": len(res), "templates_range": [min(t for t, _ in sizes), max(t for t, _ in sizes)],
+           "bins_range": [min(n for _, n in sizes), max(n for _, n in sizes)],
+           "template_GB": elems * 4 / 1e9, "autotuned_shape": {"item_elems": shape[0], "waves": shape[1],
+                                                              "wg_per_cu": shape[2], "tune_s": tune_s},
+           "grouped_fg_kernel_ms": fg_ms, "grouped_fg_GBps": fg_GBps, "grouped_fg_hbm_frac": fg_GBps / PEAK["hbm"],
+           "reduce_kernel_ms": rms / max(rn, 1),
+           "fit_seconds": fit_s, "fits_per_s": len(res) / fit_s, "optimizer_steps": mf.steps,
+           "graph_ms_total": gms, "fg_evaluations": evals, "status_counts": st,
+           "iterations_median": float(np.median(iters)), "iterations_max": int(iters.max())}
+    if not args.no_cpu:
+        # host fit_templates on the smallest fits, bounded to about cpu_seconds
+        sample, cpu_s, cpu_work = [], 0.0, 0.0
+        for f in order:
+            if cpu_s > args.cpu_seconds:
+                break
+            models, data = host[f]
+            t1 = time.time()
+            ref = cpu_fit_templates(O, np.asfortranarray(models, dtype=np.float32), data, ncpu)
+            dt = time.time() - t1
+            cpu_s += dt
+            cpu_work += ref["evals"] * sizes[f][0] * sizes[f][1]
+            row = {"fit": int(f), "templates": sizes[f][0], "bins": sizes[f][1], "cpu_s": dt}
+            for key in ("map", "mle"):
+                mu_ref, f_ref = ref[key]
+                mu = res[f][key].mu
+                big = mu_ref > 1e-3 * mu_ref.max()
+                row[f"{key}_mu_rel_err"] = float(np.max(np.abs(mu[big] - mu_ref[big]) / mu_ref[big]))
+                row[f"{key}_fmin_rel_diff"] = float((res[f][key].result.minimum - f_ref) / abs(f_ref))
+                row[f"{key}_status"] = res[f][key].result.status
+            sample.append(row)
+        # the host's time per (template element x fg! evaluation), projected onto the whole group
+        proj = cpu_s / cpu_work * work
+        out["cpu_sample"] = sample
+        out["cpu_projected_seconds_all_fits"] = proj
+        out["gpu_over_cpu_projected"] = proj / fit_s
+        log(f"multifit host sample: {len(sample)} fits in {cpu_s:.1f} s on {ncpu} threads; "
+            f"projected host time for all fits {proj:.0f} s ({proj / fit_s:.0f}x the GPU)")
+    del host
+    mf.close()
+    for t in tpls:
+        t.close()
+    for d in datas:
+        d.close()
+    return out
 
 
 def main():
@@ -250,6 +417,9 @@ def main():
                "sample": f"{n} of the {B} walkers, same 700 x 60,000 problem: the C FP64 restatement of "
                          f"fg! (OpenMP over bins and templates), {dt:.1f} s"}
         log(f"cpu oracle: {n / dt:.3g} evals/s on {ncpu} threads")
+
+    if not args.no_extras and not args.no_multifit:
+        extras["multifit"] = multifit_bench(sfh, torch, ctx, O, args, ncpu)
 
     line = {
         "metric": "loglike+grad evals/s (700 tpl×60k bins) batched vs plain; templates/s; HBM GB/s",

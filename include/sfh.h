@@ -164,7 +164,8 @@ int sfh_fg_batched_device(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data
  * template and bin counts, each fit at most 1024 templates), evaluated in one
  * persistent launch per fg! of every fit and optimised on the device.  The
  * template and data handles stay owned by the caller and must outlive the
- * group.  Vectors of all fits are concatenated: fit f's ntemplates values
+ * group; the group keeps its own copy of the templates in a blocked layout
+ * (32-bin blocks contiguous over all templates), as much device memory again.  Vectors of all fits are concatenated: fit f's ntemplates values
  * start at offset sum_{g<f} ntemplates(g); its ntemplates^2 matrix at
  * sum_{g<f} ntemplates(g)^2. */
 typedef struct sfh_multifit sfh_multifit;

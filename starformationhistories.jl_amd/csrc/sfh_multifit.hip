@@ -3,10 +3,13 @@
 // and the quasi-Newton iterations of fit_templates / fit_templates_fast
 // (src/fitting/solvers.jl:172-275) run on the device for all fits at once.
 //
-// Grouped fg! (k_multi_fg).  A work item is a run of 32-bin blocks of one
+// Grouped fg! (k_multi_fg).  The group keeps a blocked copy of every fit's
+// templates, [32-bin block][template][32 bins], so one block of one fit is a
+// single contiguous T x 128-byte run.  A work item is a run of blocks of one
 // fit.  For a block, each of the 64*NW threads loads one float4 (4 bins) of
-// ceil(T/(8 NW)) template columns -- 8 threads cover a column's 128-byte
-// segment -- and keeps them in registers through the whole block:
+// ceil(T/(8 NW)) templates -- 8 threads cover a template's 128 bytes -- and
+// keeps them in registers through the whole block (the next block's loads
+// in flight meanwhile):
 //   composite  m_i  = sum_t C[i,t] x_t      FP64 FMAs, reduced over the 8 NW column slices
 //   LL, R      on 32 threads: n - c - n log(n/c), 1 - n/c with c = max(m, eps)
 //   gradient   g_t += sum_i C[i,t] R_i      FP64 FMAs into per-thread registers
@@ -39,11 +42,19 @@ constexpr int kWG = 256;
 constexpr int kSB = 32;                  // bins per block: one 128-byte segment of a column
 constexpr int kQ = kSB / 4;              // float4 quads per block: thread (slice s, quad q) = 8 s + q
 constexpr int kMaxT = 1024;
-constexpr int kMaxBins = 1 << 21;   // keeps column offsets within 32 bits
+
+// Pointers read from a FitDesc in memory are generic (flat) to the compiler;
+// flat loads also count against the LDS counter, so every LDS access would
+// wait for every load in flight.  Loads go through global-space pointers.
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) float gfloat;
+typedef const __attribute__((address_space(1))) f4 gfloat4;
+typedef const __attribute__((address_space(1))) double gdouble;
 constexpr int kMaxMem = 32;
 
 struct FitDesc {
-    const float* C;      // ld x T templates, column-major
+    const float* C;      // ld x T templates, column-major (the caller's)
+    const float* Cb;     // the group's blocked copy: [32-bin block][T][32 bins]
     const double* n;     // data
     int64_t ld, N, T;
     int64_t xoff;        // offset of the fit's T-vectors
@@ -60,90 +71,122 @@ struct Item {
 
 // ------------------------------------------------------------ grouped fg!
 
+// One block of one work item on registers c (already loaded): composite,
+// LL and R on 32 threads, gradient accumulation into g.
+template <int K, int NW>
+__device__ __forceinline__ void fg_block(const FitDesc& fd, int64_t b0, double eps, int want_G, const double* sx,
+                                         double (*sm)[kSB], double* sr, f4 (&c)[K], double nv, double (&g)[K],
+                                         double& ll)
+{
+    constexpr int SL = 8 * NW;
+    const int tid = threadIdx.x, q = tid & (kQ - 1), s = tid >> 3, lane = tid & 63, wave = tid >> 6;
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double xv = sx[s + SL * k];
+        m0 = fma((double)c[k].x, xv, m0);
+        m1 = fma((double)c[k].y, xv, m1);
+        m2 = fma((double)c[k].z, xv, m2);
+        m3 = fma((double)c[k].w, xv, m3);
+    }
+    // sum over the 8 slices of this wave (lane bits 3..5), then over waves in order
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1) {
+        m0 += __shfl_xor(m0, off, 64);
+        m1 += __shfl_xor(m1, off, 64);
+        m2 += __shfl_xor(m2, off, 64);
+        m3 += __shfl_xor(m3, off, 64);
+    }
+    if (lane < kQ) {
+        sm[wave][4 * q + 0] = m0;
+        sm[wave][4 * q + 1] = m1;
+        sm[wave][4 * q + 2] = m2;
+        sm[wave][4 * q + 3] = m3;
+    }
+    __syncthreads();
+    if (tid < kSB) {
+        double m = sm[0][tid];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) m += sm[w][tid];
+        const int64_t bin = b0 + tid;
+        double r = 0.0;
+        if (bin < fd.N) {
+            const double cl = m > eps ? m : eps;
+            const double n = nv;
+            ll += (n > 0.0) ? (n - cl - n * log(n / cl)) : -cl;
+            r = 1.0 - n / cl;
+        }
+        sr[tid] = r;
+    }
+    __syncthreads();
+    if (want_G) {
+        const double r0 = sr[4 * q], r1 = sr[4 * q + 1], r2 = sr[4 * q + 2], r3 = sr[4 * q + 3];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            // convert again rather than keep the FP64 copies of the block
+            // live across the barrier (twice the registers)
+            __asm__ volatile("" : "+v"(c[k]));
+            g[k] = fma((double)c[k].x, r0, g[k]);
+            g[k] = fma((double)c[k].y, r1, g[k]);
+            g[k] = fma((double)c[k].z, r2, g[k]);
+            g[k] = fma((double)c[k].w, r3, g[k]);
+        }
+    }
+}
+
 // One work item with K columns per thread (K >= ceil(T / (8 NW))).  Every
 // load is unconditional -- columns past the last are clamped to it and meet
-// x = 0 in the composite; their gradient sums are not written -- so all K
-// loads of a block are in flight together.
+// x = 0 in the composite; their gradient sums are not written; block indices
+// past the item's last are clamped to it -- and two register sets alternate,
+// so the loads of the next block are in flight while a block is processed and
+// each wait covers only the older set.
 template <int K, int NW>
 __device__ __forceinline__ void fg_item(const FitDesc& fd, const Item& item, int64_t it, double eps, int want_G,
                                         const double* sx, double (*sm)[kSB], double* sr, double* __restrict__ ll_part,
                                         double* __restrict__ g_part)
 {
     constexpr int SL = 8 * NW;    // column slices
-    const int tid = threadIdx.x, q = tid & (kQ - 1), s = tid >> 3, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, q = tid & (kQ - 1), s = tid >> 3, wave = tid >> 6;
     const int T = (int)fd.T;
-    // element offsets of this thread's columns (< 2^31: T <= 1024, ld < 2^21)
+    // element offsets of this thread's columns inside a block of the blocked copy
     int32_t col[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int t = s + SL * k;
-        col[k] = (int32_t)((t < T ? t : T - 1) * fd.ld) + 4 * q;
+        col[k] = (t < T ? t : T - 1) * kSB + 4 * q;
     }
+    const int64_t bstride = (int64_t)T * kSB;
+    gfloat* blk0 = (gfloat*)fd.Cb + (item.bin0 / kSB) * bstride;
+    gdouble* nd = (gdouble*)fd.n;
+    const int last = item.nblk - 1;
     double g[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) g[k] = 0.0;
     double ll = 0.0;    // threads 0..31: their bin of every block
 
-    for (int b = 0; b < item.nblk; ++b) {
-        const int64_t b0 = item.bin0 + (int64_t)b * kSB;
-        if (b0 >= fd.N) break;
-        float4 c[K];
-        const float* blk = fd.C + b0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) c[k] = *reinterpret_cast<const float4*>(blk + col[k]);
-        double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const double xv = sx[s + SL * k];
-            m0 = fma((double)c[k].x, xv, m0);
-            m1 = fma((double)c[k].y, xv, m1);
-            m2 = fma((double)c[k].z, xv, m2);
-            m3 = fma((double)c[k].w, xv, m3);
-        }
-        // sum over the 8 slices of this wave (lane bits 3..5), then over waves in order
-#pragma unroll
-        for (int off = 8; off < 64; off <<= 1) {
-            m0 += __shfl_xor(m0, off, 64);
-            m1 += __shfl_xor(m1, off, 64);
-            m2 += __shfl_xor(m2, off, 64);
-            m3 += __shfl_xor(m3, off, 64);
-        }
-        if (lane < kQ) {
-            sm[wave][4 * q + 0] = m0;
-            sm[wave][4 * q + 1] = m1;
-            sm[wave][4 * q + 2] = m2;
-            sm[wave][4 * q + 3] = m3;
-        }
-        __syncthreads();
-        if (tid < kSB) {
-            double m = sm[0][tid];
-#pragma unroll
-            for (int w = 1; w < NW; ++w) m += sm[w][tid];
-            const int64_t bin = b0 + tid;
-            double r = 0.0;
-            if (bin < fd.N) {
-                const double cl = m > eps ? m : eps;
-                const double n = fd.n[bin];
-                ll += (n > 0.0) ? (n - cl - n * log(n / cl)) : -cl;
-                r = 1.0 - n / cl;
-            }
-            sr[tid] = r;
-        }
-        __syncthreads();
-        if (want_G) {
-            const double r0 = sr[4 * q], r1 = sr[4 * q + 1], r2 = sr[4 * q + 2], r3 = sr[4 * q + 3];
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                // convert again rather than keep the FP64 copies of the block
-                // live across the barrier (twice the registers)
-                __asm__ volatile("" : "+v"(c[k].x), "+v"(c[k].y), "+v"(c[k].z), "+v"(c[k].w));
-                g[k] = fma((double)c[k].x, r0, g[k]);
-                g[k] = fma((double)c[k].y, r1, g[k]);
-                g[k] = fma((double)c[k].z, r2, g[k]);
-                g[k] = fma((double)c[k].w, r3, g[k]);
-            }
-        }
+    // a register set: the block's template values and its data counts (every
+    // thread loads one count, index clamped into the fit, so the whole set
+    // retires together and no later load has to be waited for on its own)
+    f4 ca[K], cb[K];
+    double na, nb;
+    const int64_t nlast = fd.N - 1;
+#define SFH_LOAD(R, NV, b)                                                                   \
+    do {                                                                                     \
+        const int bb_ = (b) < last ? (b) : last;                                             \
+        gfloat* p_ = blk0 + (int64_t)bb_ * bstride;                                          \
+        _Pragma("unroll") for (int k = 0; k < K; ++k) R[k] = *(gfloat4*)(p_ + col[k]);       \
+        const int64_t i_ = item.bin0 + (int64_t)bb_ * kSB + (tid & (kSB - 1));               \
+        NV = nd[i_ < nlast ? i_ : nlast];                                                    \
+    } while (0)
+    SFH_LOAD(ca, na, 0);
+    for (int b = 0; b <= last; b += 2) {
+        SFH_LOAD(cb, nb, b + 1);
+        fg_block<K, NW>(fd, item.bin0 + (int64_t)b * kSB, eps, want_G, sx, sm, sr, ca, na, g, ll);
+        SFH_LOAD(ca, na, b + 2);
+        if (b + 1 <= last)
+            fg_block<K, NW>(fd, item.bin0 + (int64_t)(b + 1) * kSB, eps, want_G, sx, sm, sr, cb, nb, g, ll);
     }
+#undef SFH_LOAD
     if (wave == 0) {
 #pragma unroll
         for (int off = 16; off > 0; off >>= 1) ll += __shfl_xor(ll, off, 64);
@@ -185,7 +228,7 @@ __device__ __forceinline__ void fg_dispatch(int kmax, const FitDesc& fd, const I
 // smallest column-count variant that covers its own fit.
 // Occupancy floor per variant: without it the compiler widens every template
 // value to FP64 ahead of the FMAs and doubles the register footprint.
-constexpr int fg_waves_per_eu(int kmax) { return kmax <= 8 ? 4 : kmax <= 16 ? 2 : 1; }
+constexpr int fg_waves_per_eu(int kmax) { return kmax <= 4 ? 4 : kmax <= 8 ? 3 : kmax <= 16 ? 2 : 1; }
 
 template <int NW, int KMAX>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(fg_waves_per_eu(KMAX))))
@@ -205,7 +248,7 @@ void k_multi_fg(const FitDesc* __restrict__ fits, const Item* __restrict__ items
         const int T = (int)fd.T;
         const int kmax = (T + SL - 1) / SL;
         __syncthreads();    // the previous item is done with sx
-        for (int t = threadIdx.x; t < KMAX * SL; t += NT) sx[t] = t < T ? X[fd.xoff + t] : 0.0;
+        for (int t = threadIdx.x; t < KMAX * SL; t += NT) sx[t] = t < T ? ((gdouble*)X)[fd.xoff + t] : 0.0;
         __syncthreads();
         fg_dispatch<NW, KMAX>(kmax, fd, item, it, eps, want_G, sx, sm, sr, ll_part, g_part);
     }
@@ -705,6 +748,18 @@ __global__ __launch_bounds__(kWG) void k_opt_count(const OptState* __restrict__ 
     if (threadIdx.x == 0) *out = (int64_t)c;
 }
 
+// Blocked copy of one fit: Cb[(b*T + t)*32 + i] = C[t*ld + 32 b + i] (bins
+// past N are the source's zero padding, 32 b + 31 < ld).
+__global__ __launch_bounds__(kWG) void k_block_copy(const float* __restrict__ C, int64_t ld, int64_t T, int64_t nblk,
+                                                    float* __restrict__ Cb)
+{
+    const int64_t n = nblk * T * kQ;
+    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) {
+        const int64_t q = i % kQ, t = (i / kQ) % T, b = i / (kQ * T);
+        reinterpret_cast<float4*>(Cb)[i] = *reinterpret_cast<const float4*>(C + t * ld + b * kSB + 4 * q);
+    }
+}
+
 // Per-template column sums (FP64) for renormalize_x0, and per-fit data sums.
 __global__ __launch_bounds__(kWG) void k_colsum(const FitDesc* __restrict__ fits, const int32_t* __restrict__ col_fit,
                                                 double* __restrict__ colsum)
@@ -765,6 +820,7 @@ struct sfh_multifit {
     int wg_per_cu = 2;
     std::vector<FitDesc> h_fits;
     FitDesc* d_fits = nullptr;
+    float* d_blocked = nullptr;         // blocked template copies of all fits
     Item* d_items = nullptr;
     int64_t nitems = 0, items_cap = 0;
     double* d_llp = nullptr;
@@ -790,7 +846,7 @@ namespace {
 
 void free_all(sfh_multifit* mf)
 {
-    void* ptrs[] = {mf->d_fits, mf->d_items, mf->d_llp, mf->d_gp, mf->d_colsum_tmp, mf->d_nsum, mf->d_st, mf->d_V,
+    void* ptrs[] = {mf->d_fits, mf->d_blocked, mf->d_items, mf->d_llp, mf->d_gp, mf->d_colsum_tmp, mf->d_nsum, mf->d_st, mf->d_V,
                     mf->d_X, mf->d_active, mf->d_count, mf->d_H, mf->d_Hmap, mf->d_S, mf->d_Y, mf->d_rho};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -895,16 +951,18 @@ int sfh_multifit_create(sfh_ctx* ctx, int64_t nfits, const sfh_templates* const*
     mf->nfits = nfits;
     mf->h_fits.resize(nfits);
     std::vector<int32_t> col_fit;
+    std::vector<int64_t> boff;
+    int64_t blocked_elems = 0;
     for (int64_t f = 0; f < nfits; ++f) {
         const sfh_templates* t = templates[f];
         const sfh_data* d = data[f];
         if (!t || !d || t->ctx != ctx || d->ctx != ctx || t->nbins != d->nbins || t->ntemplates < 1 ||
-            t->ntemplates > kMaxT || t->nbins > kMaxBins) {
+            t->ntemplates > kMaxT) {
             delete mf;
             return set_error(ctx, SFH_ERR_ARGUMENT,
                              "sfh_multifit_create: fit %lld needs templates and data of one context with "
-                             "axes(models,1) == axes(data,1), 1..%d templates and at most %d bins",
-                             (long long)f, kMaxT, kMaxBins);
+                             "axes(models,1) == axes(data,1) and 1..%d templates",
+                             (long long)f, kMaxT);
         }
         FitDesc& fd = mf->h_fits[f];
         fd.C = t->d_models;
@@ -918,6 +976,8 @@ int sfh_multifit_create(sfh_ctx* ctx, int64_t nfits, const sfh_templates* const*
         mf->sumT2 += fd.T * fd.T;
         mf->maxT = std::max(mf->maxT, fd.T);
         for (int64_t j = 0; j < fd.T; ++j) col_fit.push_back((int32_t)f);
+        boff.push_back(blocked_elems);
+        blocked_elems += cdiv(fd.N, kSB) * kSB * fd.T;
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) {
@@ -928,6 +988,20 @@ int sfh_multifit_create(sfh_ctx* ctx, int64_t nfits, const sfh_templates* const*
     int rc = SFH_OK;
     int32_t* d_colfit = nullptr;
     do {
+        if (hipMalloc(&mf->d_blocked, blocked_elems * sizeof(float)) != hipSuccess) {
+            rc = set_error(ctx, SFH_ERR_ALLOC, "sfh_multifit_create: blocked template copy (%lld floats) failed",
+                           (long long)blocked_elems);
+            break;
+        }
+        for (int64_t f = 0; f < nfits; ++f) {
+            FitDesc& fd = mf->h_fits[f];
+            fd.Cb = mf->d_blocked + boff[f];
+            const int64_t nblk = cdiv(fd.N, kSB);
+            const int64_t n = nblk * fd.T * kQ;
+            hipLaunchKernelGGL(k_block_copy, dim3((unsigned)std::min<int64_t>(cdiv(n, kWG), 65536)), dim3(kWG), 0,
+                               ctx->stream, fd.C, fd.ld, fd.T, nblk, mf->d_blocked + boff[f]);
+        }
+        if ((rc = check_launch(ctx, "k_block_copy"))) break;
         if (hipMalloc(&mf->d_fits, nfits * sizeof(FitDesc)) != hipSuccess ||
             hipMalloc(&mf->d_V, (size_t)NVEC * mf->sumT * sizeof(double)) != hipSuccess ||
             hipMalloc(&mf->d_nsum, nfits * sizeof(double)) != hipSuccess ||
