@@ -245,6 +245,32 @@ int sfh_multifit_fit(sfh_multifit* mf, const double* x0, const sfh_fit_options* 
 
 /* --------------------------------------------------- P1: template deposition */
 
+/* A function tabulated on a uniform grid: values[k] = f(first + k*step),
+ * 0 <= k < length; read back by linear interpolation, clamped at the ends. */
+typedef struct {
+    double first;
+    double step;
+    int64_t length;
+    const double* values;
+} sfh_table;
+
+/* The pair histogram of binary_hess(::RandomBinaryPairs, m_ini, mags, ...)
+ * -- src/StarFormationHistories.jl:627-682, for y_index in color_indices.
+ * For the npoints-point (downsampled) isochrone, every pair of intervals
+ * i < j: magnitudes flux2mag(mag2flux(mag[i]) + mag2flux(mag[j])) in the two
+ * colour filters (a = first(color_indices), b = last), weight
+ * (m_{i+1}-m_i)(m_{j+1}-m_j)(imf_i+imf_{i+1})(imf_j+imf_{j+1}) * prefac
+ * times compl_a(mag_a) * compl_b(mag_b) (prefac = normalize_value /
+ * mean_mass / 4 / trapz(m_ini, imf)), measured magnitudes mag + bias(mag),
+ * binned (left-closed) at colour a - b and y = a (y_is_a) or b.  The
+ * completeness and bias callables arrive tabulated (sfh_table) over the
+ * pairs' magnitude range.  out: the FP64 nx x ny histogram (colour fastest),
+ * accumulated deterministically. */
+int sfh_binary_pairs_hist(sfh_ctx* ctx, int64_t npoints, const double* mini, const double* imf_pdf,
+                          const double* mag_a, const double* mag_b, int y_is_a, double prefac,
+                          const sfh_table* compl_a, const sfh_table* compl_b, const sfh_table* bias_a,
+                          const sfh_table* bias_b, sfh_range xedges, sfh_range yedges, double* out);
+
 /* bin_cmd_smooth(colors, mags, color_err, mag_err, cov_mult; weights, edges)
  * -- src/StarFormationHistories.jl:574-621.  cov_mult in {-1,0,1} as the
  * reference checks (:579): 0 deposits GaussianPSFAsymmetric in pixel space

@@ -339,13 +339,16 @@ def calculate_weights(mini, completeness, imf, normalize_value, mean_mass, spaci
 
 @dataclass
 class SmoothPoints:
-    """The arrays partial_cmd_smooth hands to bin_cmd_smooth (:886-888)."""
+    """The arrays partial_cmd_smooth hands to bin_cmd_smooth (:886-888), and
+    the resampled isochrone its binary branch uses."""
     colors: np.ndarray
     mags: np.ndarray
     color_err: np.ndarray
     mag_err: np.ndarray
     weights: np.ndarray
     cov_mult: int
+    new_mini: list = None
+    new_iso: list = None
 
 
 def partial_cmd_smooth_points(m_ini, mags, mag_err_funcs, y_index, color_indices, imf,
@@ -384,17 +387,139 @@ def partial_cmd_smooth_points(m_ini, mags, mag_err_funcs, y_index, color_indices
     bcol = [a - b for a, b in zip(bias[c0], bias[c1])]                         # :885
     return SmoothPoints(np.array(midpoints(bcol)), np.array(midpoints(bias[y_index])),
                         np.array(midpoints(color_err)), np.array(midpoints(mag_err[y_index])),
-                        np.array(weights), cov_mult)
+                        np.array(weights), cov_mult, new_mini, new_iso)
 
 
 def partial_cmd_smooth(m_ini, mags, mag_err_funcs, y_index, color_indices, imf,
                        completeness_funcs=None, bias_funcs=None, *, dmod=0.0,
-                       normalize_value=1.0, mean_mass, ex: Range, ey: Range, nthreads=1):
-    """partial_cmd_smooth with NoBinaries (src/StarFormationHistories.jl:829-893).
-    Returns (weights matrix (nx, ny), pair count)."""
+                       normalize_value=1.0, mean_mass, ex: Range, ey: Range, nthreads=1,
+                       binary_fraction=0.0):
+    """partial_cmd_smooth (src/StarFormationHistories.jl:829-911) with
+    NoBinaries (binary_fraction 0) or RandomBinaryPairs(binary_fraction).
+    Returns (weights matrix (nx, ny), pair count of the single-star part)."""
     sp = partial_cmd_smooth_points(m_ini, mags, mag_err_funcs, y_index, color_indices, imf,
                                    completeness_funcs, bias_funcs, dmod=dmod,
                                    normalize_value=normalize_value, mean_mass=mean_mass,
                                    ex=ex, ey=ey)
-    return bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, sp.cov_mult,
-                          sp.weights, ex, ey, nthreads=nthreads)
+    single, pairs = bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, sp.cov_mult,
+                                   sp.weights, ex, ey, nthreads=nthreads)
+    if binary_fraction == 0:
+        return single, pairs
+    ds = 4                                                                     # :898
+    binary = binary_hess(sp.new_mini[::ds], [m[::ds] for m in sp.new_iso], mag_err_funcs, y_index,
+                         color_indices, imf, completeness_funcs, bias_funcs, ex=ex, ey=ey,
+                         normalize_value=normalize_value, mean_mass=mean_mass, nthreads=nthreads)
+    b = binary_fraction
+    bmf = 2 * b / (1 + b)                                                      # simulate.jl:143,187
+    bmf *= trapz(sp.new_mini, [imf(m) for m in sp.new_mini])                   # :907
+    return single * (1 - bmf) + bmf * binary, pairs
+
+
+# --------------------------------------------------------------------------
+# Binary systems (src/StarFormationHistories.jl:627-706)
+# --------------------------------------------------------------------------
+
+def mag2flux(m, zpt=0.0):
+    """mag2flux (src/utilities.jl:57)."""
+    return 10.0 ** (4 * (zpt - m) / 10)
+
+
+def flux2mag(f, zpt=0.0):
+    """flux2mag (src/utilities.jl:67)."""
+    return zpt - 5 * math.log10(f) / 2
+
+
+def trapz(x, y):
+    """Trapezoidal rule over samples (Trapz.trapz)."""
+    return sum((x[i + 1] - x[i]) * (y[i] + y[i + 1]) for i in range(len(x) - 1)) / 2
+
+
+def _bin_left(v, e: Range):
+    """Left-closed bin of v over the edges of e (StatsBase: edges[k] <= v < edges[k+1]), or -1."""
+    vals = range_values(e)
+    k = bisect.bisect_right(list(vals), v) - 1
+    return k if 0 <= k < e.len - 1 else -1
+
+
+def binary_pairs_hist_loops(m_ini, imf_vec, mag_a, mag_b, y_is_a, prefac, compl, bias, ex: Range, ey: Range):
+    """The pair histogram of binary_hess (:645-682), one pair at a time (small
+    inputs): returns (nx, ny) weights."""
+    nx, ny = ex.len - 1, ey.len - 1
+    out = np.zeros((nx, ny))
+    dm = [m_ini[i + 1] - m_ini[i] for i in range(len(m_ini) - 1)]
+    for i in range(len(dm)):
+        for j in range(i + 1, len(dm)):
+            ma = flux2mag(mag2flux(mag_a[i]) + mag2flux(mag_a[j]))
+            mb = flux2mag(mag2flux(mag_b[i]) + mag2flux(mag_b[j]))
+            w = dm[i] * dm[j] * (imf_vec[i] + imf_vec[i + 1]) * (imf_vec[j] + imf_vec[j + 1]) * prefac
+            w *= compl[0](ma) * compl[1](mb)
+            pa, pb = ma + bias[0](ma), mb + bias[1](mb)
+            kx, ky = _bin_left(pa - pb, ex), _bin_left(pa if y_is_a else pb, ey)
+            if kx >= 0 and ky >= 0:
+                out[kx, ky] += w
+    return out
+
+
+def binary_pairs_hist(m_ini, imf_vec, mag_a, mag_b, y_is_a, prefac, compl, bias, ex: Range, ey: Range):
+    """The same, vectorised over the secondary of each primary (numpy, FP64;
+    the callables are applied one value at a time)."""
+    m_ini = np.asarray(m_ini, dtype=np.float64)
+    imf_vec = np.asarray(imf_vec, dtype=np.float64)
+    mag_a = np.asarray(mag_a, dtype=np.float64)
+    mag_b = np.asarray(mag_b, dtype=np.float64)
+    nx, ny = ex.len - 1, ey.len - 1
+    ev_x, ev_y = range_values(ex), range_values(ey)
+    out = np.zeros(nx * ny)
+    n = m_ini.size - 1
+    dm = m_ini[1:] - m_ini[:-1]
+    isum = imf_vec[:-1] + imf_vec[1:]
+    fa, fb = 10.0 ** (4 * (0.0 - mag_a[:n]) / 10), 10.0 ** (4 * (0.0 - mag_b[:n]) / 10)
+    ca, cb = np.vectorize(compl[0], otypes=[float]), np.vectorize(compl[1], otypes=[float])
+    ba, bb = np.vectorize(bias[0], otypes=[float]), np.vectorize(bias[1], otypes=[float])
+    for i in range(n - 1):
+        j = np.arange(i + 1, n)
+        ma = 0.0 - 5 * np.log10(fa[i] + fa[j]) / 2
+        mb = 0.0 - 5 * np.log10(fb[i] + fb[j]) / 2
+        w = dm[i] * dm[j] * isum[i] * isum[j] * prefac
+        w = w * (ca(ma) * cb(mb))
+        pa, pb = ma + ba(ma), mb + bb(mb)
+        kx = np.searchsorted(ev_x, pa - pb, side="right") - 1
+        ky = np.searchsorted(ev_y, pa if y_is_a else pb, side="right") - 1
+        ok = (kx >= 0) & (kx < nx) & (ky >= 0) & (ky < ny)
+        np.add.at(out, kx[ok] + nx * ky[ok], w[ok])
+    return out.reshape(nx, ny, order="F")
+
+
+def binary_hess(m_ini, mags, mag_err_funcs, y_index, color_indices, imf, completeness_funcs=None,
+                bias_funcs=None, *, ex: Range, ey: Range, normalize_value=1.0, mean_mass, nthreads=1):
+    """binary_hess(::RandomBinaryPairs, ...) (:627-706), y_index in
+    color_indices: the pair histogram, then bin_cmd_smooth of its non-empty
+    bins at their centres.  Returns the (nx, ny) weights."""
+    nf = len(mags)
+    if completeness_funcs is None:
+        completeness_funcs = [lambda m: 1.0] * nf
+    if bias_funcs is None:
+        bias_funcs = [lambda m: 0.0] * nf
+    c0, c1 = color_indices
+    assert y_index in color_indices
+    imf_vec = [imf(m) for m in m_ini]                                          # :637
+    prefac = normalize_value / mean_mass / 4 / trapz(m_ini, imf_vec)           # :642-643
+    hist = binary_pairs_hist(m_ini, imf_vec, mags[c0], mags[c1], y_index == c0, prefac,
+                             [completeness_funcs[c0], completeness_funcs[c1]],
+                             [bias_funcs[c0], bias_funcs[c1]], ex, ey)
+    nx, ny = ex.len - 1, ey.len - 1
+    hw = hist.ravel(order="F")
+    good = np.flatnonzero(hw != 0)                                             # :683
+    cx = [ex.first + k * ex.step + ex.step / 2 for k in range(nx)]             # :685-686
+    cy = [ey.first + k * ey.step + ey.step / 2 for k in range(ny)]
+    h_colors = np.array([cx[g % nx] for g in good])
+    h_mags = np.array([cy[g // nx] for g in good])
+    ymag_err = np.array([mag_err_funcs[y_index](cy[g // nx]) for g in good])
+    xmags = h_mags - h_colors if y_index == c0 else h_mags + h_colors           # :693-697
+    x_c_idx = c1 if c0 == y_index else c0
+    color_err = np.array([mag_err_funcs[x_c_idx](v) for v in xmags])
+    cov_mult = -1 if y_index == c0 else 1
+    if good.size == 0:
+        return np.zeros((nx, ny))
+    w, _ = bin_cmd_smooth(h_colors, h_mags, color_err, ymag_err, cov_mult, hw[good], ex, ey, nthreads=nthreads)
+    return w

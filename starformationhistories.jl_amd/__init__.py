@@ -19,7 +19,9 @@ from .fitting import (HMCModel, LogTransformFTResult, MCMCModel, construct_x0,  
                       fit_templates, fit_templates_fast, fit_templates_lbfgsb, mcmc_sample,
                       renormalize_x0)
 from .multifit import MultiFit, StageInfo  # noqa: E402
-from .templates import (Histogram, PointBatch, Range, bin_cmd_smooth, calculate_edges,  # noqa: E402
+from .templates import (Histogram, NoBinaries, PointBatch, RandomBinaryPairs, Range,  # noqa: E402
+                        binary_hess, binary_mass_fraction, binary_number_fraction, binary_pairs_hist,
+                        binary_system_fraction, bin_cmd_smooth, calculate_edges, flux2mag, mag2flux, trapz,
                         calculate_weights, deposit_batch, edges_range, exp_photerr,
                         histogram_data, histogram_pix, interpolate_mini, Martin2016_complete,
                         midpoints, mini_spacing, partial_cmd_smooth, partial_cmd_smooth_points,
@@ -34,5 +36,7 @@ __all__ = [
     "interpolate_mini", "Martin2016_complete", "midpoints", "mini_spacing", "partial_cmd_smooth",
     "partial_cmd_smooth_points", "templates_from_device", "HMCModel", "LogTransformFTResult",
     "MCMCModel", "construct_x0", "fit_templates", "fit_templates_fast", "fit_templates_lbfgsb",
-    "mcmc_sample", "renormalize_x0", "MultiFit", "StageInfo",
+    "mcmc_sample", "renormalize_x0", "MultiFit", "StageInfo", "NoBinaries", "RandomBinaryPairs",
+    "binary_hess", "binary_mass_fraction", "binary_number_fraction", "binary_pairs_hist",
+    "binary_system_fraction", "flux2mag", "mag2flux", "trapz",
 ]

@@ -40,6 +40,10 @@ class sfh_range(C.Structure):
     _fields_ = [("first", C.c_double), ("step", C.c_double), ("length", C.c_int64)]
 
 
+class sfh_table(C.Structure):
+    _fields_ = [("first", C.c_double), ("step", C.c_double), ("length", C.c_int64), ("values", C.c_void_p)]
+
+
 class sfh_fit_options(C.Structure):
     _fields_ = [("transform", C.c_int), ("method", C.c_int), ("memory", C.c_int), ("steps_per_graph", C.c_int),
                 ("g_abstol", C.c_double), ("max_iterations", C.c_int64), ("eps", C.c_double)]
@@ -101,6 +105,9 @@ SIGNATURES = [
     ("sfh_multifit_fg_device", _int, [_vp, _vp, _dbl, _vp, _vp]),
     ("sfh_fit_options_default", _int, [C.POINTER(sfh_fit_options)]),
     ("sfh_multifit_fit", _int, [_vp, _vp, C.POINTER(sfh_fit_options), C.POINTER(sfh_fit_results)]),
+    ("sfh_binary_pairs_hist", _int,
+     [_vp, _i64, _vp, _vp, _vp, _vp, _int, _dbl, C.POINTER(sfh_table), C.POINTER(sfh_table),
+      C.POINTER(sfh_table), C.POINTER(sfh_table), sfh_range, sfh_range, _vp]),
     ("sfh_bin_cmd_smooth", _int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _int, sfh_range, sfh_range, _vp]),
     ("sfh_bin_cmd_smooth_batch_device", _int,
      [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, sfh_range, sfh_range, _vp, _i64, _p64]),

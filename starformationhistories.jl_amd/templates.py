@@ -17,6 +17,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from . import _lib
 from ._lib import ArgumentError, sfh_range
 from .core import Context, HIPTemplates, default_context
 
@@ -201,13 +202,16 @@ def imf_mean(imf) -> float:
 
 @dataclass
 class SmoothPoints:
-    """What partial_cmd_smooth hands to bin_cmd_smooth (:886-888)."""
+    """What partial_cmd_smooth hands to bin_cmd_smooth (:886-888), plus the
+    resampled isochrone (new_mini, new_iso_mags) its binary branch uses."""
     colors: np.ndarray
     mags: np.ndarray
     color_err: np.ndarray
     mag_err: np.ndarray
     weights: np.ndarray
     cov_mult: int
+    new_mini: np.ndarray = None
+    new_iso_mags: list = None
 
 
 def partial_cmd_smooth_points(m_ini, mags, mag_err_funcs, y_index, color_indices, imf,
@@ -252,7 +256,8 @@ def partial_cmd_smooth_points(m_ini, mags, mag_err_funcs, y_index, color_indices
     bias = [new_iso[i] + _apply(bias_funcs[i], new_iso[i]) for i in range(nf)]   # :884
     bcol = bias[c0] - bias[c1]                                               # :885
     return SmoothPoints(midpoints(bcol), midpoints(bias[y_index]), midpoints(color_err),
-                        midpoints(mag_err[y_index]), np.asarray(weights, dtype=np.float64), cov_mult)
+                        midpoints(mag_err[y_index]), np.asarray(weights, dtype=np.float64), cov_mult,
+                        new_mini, new_iso)
 
 
 # ------------------------------------------------------------- GPU deposition
@@ -286,16 +291,176 @@ def partial_cmd_smooth(m_ini, mags, mag_err_funcs, y_index, color_indices, imf,
                        completeness_funcs=None, bias_funcs=None, *, dmod=0.0, normalize_value=1.0,
                        binary_model=None, mean_mass=None, edges=None, xlim=None, ylim=None, nbins=None,
                        xwidth=None, ywidth=None, ctx: Context | None = None) -> Histogram:
-    """partial_cmd_smooth (src/StarFormationHistories.jl:829-911), NoBinaries:
-    host preparation + GPU deposition."""
-    if binary_model is not None and getattr(binary_model, "fraction", 0) != 0:
-        raise ArgumentError("binary models are not implemented on this path")
+    """partial_cmd_smooth (src/StarFormationHistories.jl:829-911): host
+    preparation + GPU deposition; with a binary model of non-zero fraction the
+    single-star template is mixed with binary_hess of the isochrone
+    downsampled 4x (:895-910)."""
+    binary_model = NoBinaries() if binary_model is None else binary_model
     edges = calculate_edges(edges, xlim, ylim, nbins, xwidth, ywidth)
+    if mean_mass is None:
+        mean_mass = imf_mean(imf)
     sp = partial_cmd_smooth_points(m_ini, mags, mag_err_funcs, y_index, color_indices, imf,
                                    completeness_funcs, bias_funcs, dmod=dmod,
                                    normalize_value=normalize_value, mean_mass=mean_mass, edges=edges)
-    return bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, sp.cov_mult, weights=sp.weights,
-                          edges=edges, ctx=ctx)
+    single = bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, sp.cov_mult, weights=sp.weights,
+                            edges=edges, ctx=ctx)
+    bfrac = binary_system_fraction(binary_model)
+    if not bfrac <= 1:
+        raise ArgumentError("binary_system_fraction(binary_model) <= 1")
+    if bfrac == 0:
+        return single
+    ds = 4                                                                   # :898
+    binary = binary_hess(binary_model, sp.new_mini[::ds], [m[::ds] for m in sp.new_iso_mags], mag_err_funcs,
+                         y_index, color_indices, imf, completeness_funcs, bias_funcs, edges,
+                         normalize_value=normalize_value, mean_mass=mean_mass, ctx=ctx)
+    bmf = binary_mass_fraction(binary_model, imf)
+    bmf *= trapz(sp.new_mini, _apply(imf, sp.new_mini))                      # :907
+    return Histogram(edges, single.weights * (1 - bmf) + bmf * binary.weights)
+
+
+# ------------------------------------------------------------------ binaries
+
+class NoBinaries:
+    """NoBinaries (src/simulate.jl:150-157)."""
+    fraction = 0.0
+
+
+class RandomBinaryPairs:
+    """RandomBinaryPairs(fraction) (src/simulate.jl:159-170): binaries as
+    random pairs drawn from the single-star IMF, 0 <= fraction <= 1."""
+
+    def __init__(self, fraction: float):
+        if not (0.0 <= fraction <= 1.0):
+            raise ArgumentError("(fraction >= 0) && (fraction <= 1)")
+        self.fraction = float(fraction)
+
+
+def binary_system_fraction(model) -> float:
+    """src/simulate.jl:139: the model's fraction (0 for NoBinaries)."""
+    return float(model.fraction)
+
+
+def binary_number_fraction(model) -> float:
+    """src/simulate.jl:143: 2b / (1 + b)."""
+    b = binary_system_fraction(model)
+    return 2 * b / (1 + b)
+
+
+def binary_mass_fraction(model, imf) -> float:
+    """src/simulate.jl:157,187: 0 for NoBinaries, the number fraction of
+    stars in pairs for RandomBinaryPairs."""
+    if isinstance(model, NoBinaries):
+        return 0.0
+    if isinstance(model, RandomBinaryPairs):
+        return binary_number_fraction(model)
+    raise NotImplementedError(f"binary_mass_fraction for {type(model).__name__}")
+
+
+def mag2flux(m, zpt=0.0):
+    """mag2flux (src/utilities.jl:57): exp10(4 (zpt - m) / 10)."""
+    return np.power(10.0, 4.0 * (zpt - np.asarray(m, dtype=np.float64)) / 10.0)
+
+
+def flux2mag(f, zpt=0.0):
+    """flux2mag (src/utilities.jl:67): zpt - 5 log10(f) / 2."""
+    return zpt - 5.0 * np.log10(np.asarray(f, dtype=np.float64)) / 2.0
+
+
+def trapz(x, y) -> float:
+    """Trapezoidal integral of samples y(x) (Trapz.trapz)."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    return float(np.sum((x[1:] - x[:-1]) * (y[1:] + y[:-1])) / 2)
+
+
+_TABLE_POINTS = 16385
+
+
+def _table(f, lo, hi):
+    """f tabulated on a uniform grid over [lo, hi] (an sfh_table) and the
+    array that backs it."""
+    grid = np.linspace(lo, hi, _TABLE_POINTS)
+    vals = np.ascontiguousarray(_apply(f, grid))
+    return _lib.sfh_table(float(grid[0]), float(grid[1] - grid[0]), vals.size, vals.ctypes.data), vals
+
+
+def binary_pairs_hist(m_ini, imf_pdf, mag_a, mag_b, y_is_a, prefac, completeness_funcs, bias_funcs, edges,
+                      ctx: Context | None = None) -> np.ndarray:
+    """The pair histogram of binary_hess (:627-682) on the GPU
+    (sfh_binary_pairs_hist): (nx, ny) FP64.  completeness_funcs / bias_funcs:
+    the callables of colour filters a and b, tabulated here (16385 points over
+    the pairs' magnitude range, linear interpolation)."""
+    ctx = ctx or default_context()
+    m_ini = np.ascontiguousarray(m_ini, dtype=np.float64)
+    imf_pdf = np.ascontiguousarray(imf_pdf, dtype=np.float64)
+    mag_a = np.ascontiguousarray(mag_a, dtype=np.float64)
+    mag_b = np.ascontiguousarray(mag_b, dtype=np.float64)
+    n = m_ini.size
+    if not (imf_pdf.size == mag_a.size == mag_b.size == n):
+        raise ArgumentError("all(length(i) == length(m_ini) for i in mags)")
+    ex, ey = edges
+    out = np.zeros((ex.length - 1) * (ey.length - 1))
+    if n < 3:
+        return out.reshape(ex.length - 1, ey.length - 1, order="F")
+    tabs, keep = [], []
+    for mags, f_c, f_b in ((mag_a, completeness_funcs[0], bias_funcs[0]), (mag_b, completeness_funcs[1], bias_funcs[1])):
+        # a pair is at most 2.5 log10(2) brighter than its brighter member
+        lo, hi = float(mags[:-1].min()) - 0.76, float(mags[:-1].max()) + 1e-9
+        for f in (f_c, f_b):
+            t, v = _table(f, lo, hi)
+            tabs.append(t)
+            keep.append(v)
+    ctx.check(ctx.lib.sfh_binary_pairs_hist(ctx.handle, n, m_ini.ctypes.data, imf_pdf.ctypes.data,
+                                            mag_a.ctypes.data, mag_b.ctypes.data, 1 if y_is_a else 0,
+                                            float(prefac), C.byref(tabs[0]), C.byref(tabs[2]), C.byref(tabs[1]),
+                                            C.byref(tabs[3]), ex.as_c(), ey.as_c(), out.ctypes.data))
+    return out.reshape(ex.length - 1, ey.length - 1, order="F")
+
+
+def binary_hess(model, m_ini, mags, mag_err_funcs, y_index, color_indices, imf, completeness_funcs=None,
+                bias_funcs=None, edges=None, *, normalize_value=1.0, mean_mass=None,
+                ctx: Context | None = None) -> Histogram:
+    """binary_hess(::RandomBinaryPairs, ...) (src/StarFormationHistories.jl:627-706):
+    the pair histogram on the GPU, then bin_cmd_smooth of its non-empty bins
+    at their centres (GPU)."""
+    if not isinstance(model, RandomBinaryPairs):
+        raise NotImplementedError(f"binary_hess(model::{type(model).__name__}, ...) not implemented.")
+    nf = len(mags)
+    if completeness_funcs is None:
+        completeness_funcs = [lambda m: np.ones_like(np.asarray(m, dtype=np.float64))] * nf
+    if bias_funcs is None:
+        bias_funcs = [lambda m: np.zeros_like(np.asarray(m, dtype=np.float64))] * nf
+    m_ini = np.asarray(m_ini, dtype=np.float64)
+    mags = [np.asarray(m, dtype=np.float64) for m in mags]
+    if not all(m.size == m_ini.size for m in mags):
+        raise ArgumentError("all(length(i) == length(m_ini) for i in mags)")
+    if mean_mass is None:
+        mean_mass = imf_mean(imf)
+    c0, c1 = color_indices
+    if y_index not in color_indices:                                         # :699-701
+        raise NotImplementedError("`binary_hess(model::RandomBinaryPairs, ...)` for "
+                                  "`!(y_index in color_indices)` not yet implemented.")
+    ex, ey = edges
+    imf_vec = _apply(imf, m_ini)                                             # :637
+    prefac = normalize_value / mean_mass / 4 / trapz(m_ini, imf_vec)          # :642-643
+    hist = binary_pairs_hist(m_ini, imf_vec, mags[c0], mags[c1], y_index == c0, prefac,
+                             [completeness_funcs[c0], completeness_funcs[c1]], [bias_funcs[c0], bias_funcs[c1]],
+                             edges, ctx=ctx)
+    hw = hist.ravel(order="F")                                               # vec(hist.weights)
+    good = np.flatnonzero(hw != 0)                                           # :683
+    nx, ny = ex.length - 1, ey.length - 1
+    cx = ex.first + np.arange(nx) * ex.step + ex.step / 2                    # :685-686
+    cy = ey.first + np.arange(ny) * ey.step + ey.step / 2
+    h_colors = np.tile(cx, ny)[good]
+    h_mags = np.repeat(cy, nx)[good]
+    ymag_err = np.repeat(_apply(mag_err_funcs[y_index], cy), nx)[good]
+    xmags = h_mags - h_colors if y_index == c0 else h_mags + h_colors          # :693-697
+    x_c_idx = c1 if c0 == y_index else c0
+    color_err = _apply(mag_err_funcs[x_c_idx], xmags)
+    cov_mult = -1 if y_index == c0 else 1
+    if good.size == 0:
+        return Histogram(edges, np.zeros((nx, ny)))
+    return bin_cmd_smooth(h_colors, h_mags, color_err, ymag_err, cov_mult, weights=hw[good], edges=edges, ctx=ctx)
 
 
 class PointBatch:
