@@ -19,6 +19,7 @@ from .fitting import (HMCModel, LogTransformFTResult, MCMCModel, construct_x0,  
                       fit_templates, fit_templates_fast, fit_templates_lbfgsb, mcmc_sample,
                       renormalize_x0)
 from .multifit import MultiFit, StageInfo  # noqa: E402
+from . import hierarchical  # noqa: E402
 from .templates import (Histogram, NoBinaries, PointBatch, RandomBinaryPairs, Range,  # noqa: E402
                         binary_hess, binary_mass_fraction, binary_number_fraction, binary_pairs_hist,
                         binary_system_fraction, bin_cmd_smooth, calculate_edges, flux2mag, mag2flux, trapz,
@@ -38,5 +39,5 @@ __all__ = [
     "MCMCModel", "construct_x0", "fit_templates", "fit_templates_fast", "fit_templates_lbfgsb",
     "mcmc_sample", "renormalize_x0", "MultiFit", "StageInfo", "NoBinaries", "RandomBinaryPairs",
     "binary_hess", "binary_mass_fraction", "binary_number_fraction", "binary_pairs_hist",
-    "binary_system_fraction", "flux2mag", "mag2flux", "trapz",
+    "binary_system_fraction", "flux2mag", "mag2flux", "trapz", "hierarchical",
 ]
