@@ -297,6 +297,53 @@ int sfh_bin_cmd_smooth_batch_device(sfh_ctx* ctx, const double* d_colors, const 
                                     double cov_mult, sfh_range xedges, sfh_range yedges,
                                     float* d_out, int64_t ld_out, int64_t* pairs_out);
 
+/* ------------------------------- learned template surrogate primitives (new) */
+
+/* No reference counterpart: the device operations the template surrogate
+ * (a network from (logAge, [M/H], distance modulus, error scale) to a
+ * log-template) is trained and evaluated with.  All pointers are device
+ * pointers; every call is asynchronous on the context's stream except
+ * sfh_nn_mse, which returns the loss.  dtype 0 = BF16 (stored as uint16),
+ * 1 = FP32.  Matrices are row-major. */
+
+/* C (M x N, FP32, ldc) = alpha * A B^T + beta * C, A: M x K (lda), B: N x K
+ * (ldb), K a multiple of 16 (zero padded), lda and ldb multiples of 8; on
+ * the matrix cores with FP32 accumulation. */
+int sfh_nn_gemm_nt(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* B, int64_t ldb, float* C,
+                   int64_t ldc, int64_t M, int64_t N, int64_t K, float alpha, float beta);
+/* Y (rows x ldy, dtype) = act(X + bias) with zero padding past cols; pre
+ * (rows x cols FP32, may be NULL) keeps X + bias.  act 0 = identity, 1 = SiLU. */
+int sfh_nn_bias_act(sfh_ctx* ctx, int dtype, const float* X, int64_t ldx, const float* bias, int64_t rows,
+                    int64_t cols, int act, float* pre, void* Y, int64_t ldy);
+/* dX = dY * act'(pre) elementwise (dX may alias dY). */
+int sfh_nn_act_bwd(sfh_ctx* ctx, const float* dY, const float* pre, int64_t n, int act, float* dX);
+/* out (ocols x ldo, dtype) = act(in)^T (in: rows x cols FP32, ldi), zero padded. */
+int sfh_nn_transpose(sfh_ctx* ctx, int dtype, const float* in, int64_t ldi, int64_t rows, int64_t cols, void* out,
+                     int64_t ldo, int64_t ocols, int act);
+/* out[c] = sum_r X[r, c] (+ beta * out[c]), fixed summation order. */
+int sfh_nn_colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols, float beta, float* out);
+/* Stride-2 4x4 transposed convolution (padding 1), the two gathers around
+ * its matrix products: col2im sums the 4 taps landing on each of the
+ * B x 2H x 2W x Co outputs from cols (B*H*W rows, column co*16+ky*4+kx) and
+ * adds bias; im2col builds those rows from an output gradient. */
+int sfh_nn_col2im(sfh_ctx* ctx, const float* cols, int64_t ldc, int64_t B, int64_t H, int64_t W, int64_t Co,
+                  const float* bias, float* out);
+int sfh_nn_im2col(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t H, int64_t W, int64_t Co,
+                  void* dcols, int64_t ldd);
+/* The same transposed convolution with one output channel (B x 2H x 2W) and a
+ * per-pixel bias map; its backward gives din, dw (16 x Ci) and dbmap (any may be NULL). */
+int sfh_nn_conv1_fwd(sfh_ctx* ctx, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci, const float* w,
+                     const float* bmap, float* out);
+int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci,
+                     const float* w, float* din, float* dw, float* dbmap);
+/* Mean squared error over the top-left R x S of each of B outputs of H2 x W2
+ * against target (B x R x S); dout (may be NULL) gets its gradient. */
+int sfh_nn_mse(sfh_ctx* ctx, const float* out, const float* target, int64_t B, int64_t H2, int64_t W2, int64_t R,
+               int64_t S, float* dout, float* loss);
+/* One Adam step (bias-corrected, step >= 1) on FP32 parameters. */
+int sfh_nn_adam(sfh_ctx* ctx, float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                float eps, int64_t step, float weight_decay);
+
 #ifdef __cplusplus
 }
 #endif

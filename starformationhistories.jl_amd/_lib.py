@@ -109,6 +109,18 @@ SIGNATURES = [
      [_vp, _i64, _vp, _vp, _vp, _vp, _int, _dbl, C.POINTER(sfh_table), C.POINTER(sfh_table),
       C.POINTER(sfh_table), C.POINTER(sfh_table), sfh_range, sfh_range, _vp]),
     ("sfh_bin_cmd_smooth", _int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _int, sfh_range, sfh_range, _vp]),
+    ("sfh_nn_gemm_nt", _int, [_vp, _int, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _i64, C.c_float, C.c_float]),
+    ("sfh_nn_bias_act", _int, [_vp, _int, _vp, _i64, _vp, _i64, _i64, _int, _vp, _vp, _i64]),
+    ("sfh_nn_act_bwd", _int, [_vp, _vp, _vp, _i64, _int, _vp]),
+    ("sfh_nn_transpose", _int, [_vp, _int, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _int]),
+    ("sfh_nn_colsum", _int, [_vp, _vp, _i64, _i64, _i64, C.c_float, _vp]),
+    ("sfh_nn_col2im", _int, [_vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp]),
+    ("sfh_nn_im2col", _int, [_vp, _int, _vp, _i64, _i64, _i64, _i64, _vp, _i64]),
+    ("sfh_nn_conv1_fwd", _int, [_vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp]),
+    ("sfh_nn_conv1_bwd", _int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp]),
+    ("sfh_nn_mse", _int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, C.POINTER(C.c_float)]),
+    ("sfh_nn_adam", _int, [_vp, _vp, _vp, _vp, _vp, _i64, C.c_float, C.c_float, C.c_float, C.c_float, _i64,
+                           C.c_float]),
     ("sfh_bin_cmd_smooth_batch_device", _int,
      [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, sfh_range, sfh_range, _vp, _i64, _p64]),
 ]
