@@ -303,7 +303,7 @@ int sfh_bin_cmd_smooth_batch_device(sfh_ctx* ctx, const double* d_colors, const 
  * (a network from (logAge, [M/H], distance modulus, error scale) to a
  * log-template) is trained and evaluated with.  All pointers are device
  * pointers; every call is asynchronous on the context's stream except
- * sfh_nn_mse, which returns the loss.  dtype 0 = BF16 (stored as uint16),
+ * sfh_nn_mse with a host loss pointer.  dtype 0 = BF16 (stored as uint16),
  * 1 = FP32.  Matrices are row-major. */
 
 /* C (M x N, FP32, ldc) = alpha * A B^T + beta * C, A: M x K (lda), B: N x K
@@ -337,9 +337,39 @@ int sfh_nn_conv1_fwd(sfh_ctx* ctx, const float* in, int64_t B, int64_t H, int64_
 int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci,
                      const float* w, float* din, float* dw, float* dbmap);
 /* Mean squared error over the top-left R x S of each of B outputs of H2 x W2
- * against target (B x R x S); dout (may be NULL) gets its gradient. */
-int sfh_nn_mse(sfh_ctx* ctx, const float* out, const float* target, int64_t B, int64_t H2, int64_t W2, int64_t R,
-               int64_t S, float* dout, float* loss);
+ * against target rows (R x S each): output b against row rows[b] (b when rows
+ * is NULL).  dout (may be NULL) gets its gradient.  With loss non-NULL the
+ * call waits and returns the loss; loss_acc (device, may be NULL) has the
+ * loss added to it on the stream, for asynchronous training loops. */
+int sfh_nn_mse(sfh_ctx* ctx, const float* out, const float* target, const int64_t* rows, int64_t B, int64_t H2,
+               int64_t W2, int64_t R, int64_t S, float* dout, float* loss, double* loss_acc);
+/* Training loss of the surrogate over the R x S crop of B outputs of H2 x W2
+ * (log images, target rows chosen by rows as in sfh_nn_mse):
+ *   w_mse MSE + w_kl mean_b KL(p_e || p_s),  s = exp(scale out + offset),
+ *   e = exp(scale target + offset), p = s / sum s,
+ * the KL being the log-likelihood lost per star by fitting with s instead of
+ * e.  dout gets the gradient (zero outside the crop); acc (device, 2 doubles,
+ * may be NULL) has the MSE and the mean KL added.  Asynchronous. */
+int sfh_nn_template_loss(sfh_ctx* ctx, const float* out, const float* target, const int64_t* rows, int64_t B,
+                         int64_t H2, int64_t W2, int64_t R, int64_t S, float scale, float offset, float w_mse,
+                         float w_kl, float* dout, double* acc);
+/* dst[i, :cols] = src[rows[i], :cols] (FP32). */
+int sfh_nn_gather_rows(sfh_ctx* ctx, const float* src, int64_t lds, const int64_t* rows, int64_t n, int64_t cols,
+                       float* dst, int64_t ldd);
+/* Templates (B rows of ldt, bin = ix + nx iy, as the deposition kernels write
+ * them) to training images img[b, ix, iy] = (log(T + eps) - offset) / scale,
+ * and network outputs (B x H2 x W2, top-left nx x ny used) back to templates
+ * T = max(exp(out scale + offset) - eps, 0). */
+int sfh_nn_log_target(sfh_ctx* ctx, const float* T, int64_t ldt, int64_t B, int64_t nx, int64_t ny, float eps,
+                      float offset, float scale, float* img);
+int sfh_nn_log_template(sfh_ctx* ctx, const float* out, int64_t B, int64_t H2, int64_t W2, int64_t nx, int64_t ny,
+                        float eps, float offset, float scale, float* T, int64_t ldt);
+/* Per template k of n (rows of E and S, nbins each): out[3k..3k+2] =
+ * KL(E/sum E || S'/sum S'), sum E, sum S' with S' = max(S, floor_value), in
+ * FP64.  N times the KL is the log-likelihood change of fitting N stars drawn
+ * from E with S' instead of E (amplitude free). */
+int sfh_nn_template_kl(sfh_ctx* ctx, const float* E, int64_t lde, const float* S, int64_t lds, int64_t n,
+                       int64_t nbins, double floor_value, double* out);
 /* One Adam step (bias-corrected, step >= 1) on FP32 parameters. */
 int sfh_nn_adam(sfh_ctx* ctx, float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
                 float eps, int64_t step, float weight_decay);

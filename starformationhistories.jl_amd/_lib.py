@@ -118,7 +118,14 @@ SIGNATURES = [
     ("sfh_nn_im2col", _int, [_vp, _int, _vp, _i64, _i64, _i64, _i64, _vp, _i64]),
     ("sfh_nn_conv1_fwd", _int, [_vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp]),
     ("sfh_nn_conv1_bwd", _int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp]),
-    ("sfh_nn_mse", _int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, C.POINTER(C.c_float)]),
+    ("sfh_nn_mse", _int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, C.POINTER(C.c_float), _vp]),
+    ("sfh_nn_template_loss", _int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, C.c_float, C.c_float,
+                                    C.c_float, C.c_float, _vp, _vp]),
+    ("sfh_nn_gather_rows", _int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64]),
+    ("sfh_nn_log_target", _int, [_vp, _vp, _i64, _i64, _i64, _i64, C.c_float, C.c_float, C.c_float, _vp]),
+    ("sfh_nn_log_template", _int, [_vp, _vp, _i64, _i64, _i64, _i64, _i64, C.c_float, C.c_float, C.c_float, _vp,
+                                   _i64]),
+    ("sfh_nn_template_kl", _int, [_vp, _vp, _i64, _vp, _i64, _i64, _i64, C.c_double, _vp]),
     ("sfh_nn_adam", _int, [_vp, _vp, _vp, _vp, _vp, _i64, C.c_float, C.c_float, C.c_float, C.c_float, _i64,
                            C.c_float]),
     ("sfh_bin_cmd_smooth_batch_device", _int,

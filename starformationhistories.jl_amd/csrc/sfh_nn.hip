@@ -85,11 +85,21 @@ __device__ inline f32x16 mma(const Frag<float>& a, const Frag<float>& b, f32x16 
     return c;
 }
 
+// Split K: workgroup z covers k in [z kchunk, min(K, (z + 1) kchunk)); with
+// more than one slice each writes its raw sums to C + z M N (ldc = N) and
+// k_splitk_reduce applies alpha and beta.
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_gemm_nt(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
                                                  int64_t ldb, float* __restrict__ C, int64_t ldc, int64_t M,
-                                                 int64_t N, int64_t K, float alpha, float beta)
+                                                 int64_t N, int64_t K, float alpha, float beta, int64_t kchunk)
 {
+    {
+        const int64_t kb = (int64_t)blockIdx.z * kchunk;
+        A += kb;
+        B += kb;
+        K = K - kb < kchunk ? K - kb : kchunk;
+        if (gridDim.z > 1) C += (int64_t)blockIdx.z * M * N;
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t m0 = (int64_t)blockIdx.y * 128 + (wave >> 1) * 64;
     const int64_t n0 = (int64_t)blockIdx.x * 128 + (wave & 1) * 64;
@@ -211,25 +221,45 @@ __global__ __launch_bounds__(kWG) void k_transpose(const float* __restrict__ in,
 }
 
 // column sums of a rows x cols matrix: per-chunk partials then a fixed-order sum
+// Column sums in two deterministic passes.  Workgroup (x, y): columns
+// 64 x .. 64 x + 63, rows of chunk y; thread (g, c) = (t / 64, t % 64) adds
+// rows g, g + 4, ..., then the four row groups are added in order.
 __global__ __launch_bounds__(kWG) void k_colsum_part(const float* __restrict__ X, int64_t ld, int64_t rows,
                                                      int64_t cols, int64_t chunk, float* __restrict__ part)
 {
-    const int64_t c = (int64_t)blockIdx.x * kWG + threadIdx.x;
-    if (c >= cols) return;
+    __shared__ float red[4][64];
+    const int g = threadIdx.x >> 6, cl = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * 64 + cl;
     const int64_t r0 = (int64_t)blockIdx.y * chunk, r1 = r0 + chunk < rows ? r0 + chunk : rows;
     float s = 0.f;
-    for (int64_t r = r0; r < r1; ++r) s += X[r * ld + c];
-    part[(int64_t)blockIdx.y * cols + c] = s;
+    if (c < cols) {
+#pragma unroll 8
+        for (int64_t r = r0 + g; r < r1; r += 4) s += X[r * ld + c];
+    }
+    red[g][cl] = s;
+    __syncthreads();
+    if (g == 0 && c < cols) part[(int64_t)blockIdx.y * cols + c] = ((red[0][cl] + red[1][cl]) + red[2][cl]) + red[3][cl];
 }
-
+// out[(c / N) ldo + c % N] = alpha (sum of the partials of column c) + beta out[...]
 __global__ __launch_bounds__(kWG) void k_colsum_final(const float* __restrict__ part, int64_t nparts, int64_t cols,
-                                                      float beta, float* __restrict__ out)
+                                                      float alpha, float beta, float* __restrict__ out, int64_t N,
+                                                      int64_t ldo)
 {
-    const int64_t c = (int64_t)blockIdx.x * kWG + threadIdx.x;
-    if (c >= cols) return;
+    __shared__ float red[4][64];
+    const int g = threadIdx.x >> 6, cl = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * 64 + cl;
     float s = 0.f;
-    for (int64_t p = 0; p < nparts; ++p) s += part[p * cols + c];
-    out[c] = beta == 0.f ? s : s + beta * out[c];
+    if (c < cols) {
+#pragma unroll 8
+        for (int64_t p = g; p < nparts; p += 4) s += part[p * cols + c];
+    }
+    red[g][cl] = s;
+    __syncthreads();
+    if (g == 0 && c < cols) {
+        s = alpha * (((red[0][cl] + red[1][cl]) + red[2][cl]) + red[3][cl]);
+        float* o = out + (c / N) * ldo + c % N;
+        *o = beta == 0.f ? s : s + beta * *o;
+    }
 }
 
 // ---------------------------------------------------------- convolutions
@@ -239,12 +269,13 @@ __global__ __launch_bounds__(kWG) void k_colsum_final(const float* __restrict__ 
 // two ky of parity (Y + 1) % 2.  cols: [B*H*W][ldc], column co*16 + ky*4 + kx.
 __global__ __launch_bounds__(kWG) void k_col2im(const float* __restrict__ cols, int64_t ldc, int64_t B, int64_t H,
                                                 int64_t W, int64_t Co, const float* __restrict__ bias,
-                                                float* __restrict__ out)
+                                                const float* __restrict__ bmap, float* __restrict__ out)
 {
     const int64_t H2 = 2 * H, W2 = 2 * W, n = B * H2 * W2 * Co;
     for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) {
         const int64_t co = i % Co, X = (i / Co) % W2, Y = (i / (Co * W2)) % H2, b = i / (Co * W2 * H2);
         float s = bias ? bias[co] : 0.f;
+        if (bmap) s += bmap[i - b * H2 * W2 * Co];
         const int ky0 = (int)((Y + 1) & 1), kx0 = (int)((X + 1) & 1);
 #pragma unroll
         for (int dy = 0; dy < 2; ++dy) {
@@ -283,83 +314,14 @@ __global__ __launch_bounds__(kWG) void k_im2col(const float* __restrict__ dout, 
     }
 }
 
-// Single-channel output layer (Co = 1) of the same transposed convolution on
-// the vector units: out[b,Y,X] = bmap[Y,X] + sum_taps sum_ci in[b,y,x,ci] w[tap,ci]
-__global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in, int64_t B, int64_t H, int64_t W,
-                                                   int64_t Ci, const float* __restrict__ w,
-                                                   const float* __restrict__ bmap, float* __restrict__ out)
-{
-    const int64_t H2 = 2 * H, W2 = 2 * W, n = B * H2 * W2;
-    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) {
-        const int64_t X = i % W2, Y = (i / W2) % H2, b = i / (W2 * H2);
-        float s = bmap[Y * W2 + X];
-        const int ky0 = (int)((Y + 1) & 1), kx0 = (int)((X + 1) & 1);
-        for (int dy = 0; dy < 2; ++dy) {
-            const int ky = ky0 + 2 * dy;
-            const int64_t y = (Y + 1 - ky) / 2;
-            if (y < 0 || y >= H) continue;
-            for (int dx = 0; dx < 2; ++dx) {
-                const int kx = kx0 + 2 * dx;
-                const int64_t x = (X + 1 - kx) / 2;
-                if (x < 0 || x >= W) continue;
-                const float* pin = in + ((b * H + y) * W + x) * Ci;
-                const float* pw = w + (ky * 4 + kx) * Ci;
-                for (int64_t c = 0; c < Ci; ++c) s += pin[c] * pw[c];
-            }
-        }
-        out[i] = s;
-    }
-}
-
-// din[b,y,x,ci] = sum_taps dout[b,2y-1+ky,2x-1+kx] w[tap,ci]
-__global__ __launch_bounds__(kWG) void k_conv1_bwd_in(const float* __restrict__ dout, int64_t B, int64_t H,
-                                                      int64_t W, int64_t Ci, const float* __restrict__ w,
-                                                      float* __restrict__ din)
-{
-    const int64_t H2 = 2 * H, W2 = 2 * W, n = B * H * W * Ci;
-    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) {
-        const int64_t c = i % Ci, p = i / Ci, x = p % W, y = (p / W) % H, b = p / (W * H);
-        float s = 0.f;
-        for (int ky = 0; ky < 4; ++ky) {
-            const int64_t Y = 2 * y - 1 + ky;
-            if (Y < 0 || Y >= H2) continue;
-            for (int kx = 0; kx < 4; ++kx) {
-                const int64_t X = 2 * x - 1 + kx;
-                if (X < 0 || X >= W2) continue;
-                s += dout[(b * H2 + Y) * W2 + X] * w[(ky * 4 + kx) * Ci + c];
-            }
-        }
-        din[i] = s;
-    }
-}
-
-// dw[tap,ci] partials: workgroup g sums pixels g*chunk .. over its range;
-// thread t handles (tap, ci) = (t / Ci, t % Ci), 16 Ci <= 512 threads worth
-__global__ __launch_bounds__(kWG) void k_conv1_bwd_w(const float* __restrict__ dout, const float* __restrict__ in,
-                                                     int64_t B, int64_t H, int64_t W, int64_t Ci, int64_t chunk,
-                                                     float* __restrict__ part)
-{
-    const int64_t H2 = 2 * H, W2 = 2 * W, P = B * H * W;
-    const int64_t p0 = (int64_t)blockIdx.x * chunk, p1 = p0 + chunk < P ? p0 + chunk : P;
-    for (int64_t t = threadIdx.x; t < 16 * Ci; t += kWG) {
-        const int64_t tap = t / Ci, c = t % Ci, ky = tap / 4, kx = tap % 4;
-        float s = 0.f;
-        for (int64_t p = p0; p < p1; ++p) {
-            const int64_t x = p % W, y = (p / W) % H, b = p / (W * H);
-            const int64_t Y = 2 * y - 1 + ky, X = 2 * x - 1 + kx;
-            if (Y < 0 || Y >= H2 || X < 0 || X >= W2) continue;
-            s += in[p * Ci + c] * dout[(b * H2 + Y) * W2 + X];
-        }
-        part[(int64_t)blockIdx.x * 16 * Ci + t] = s;
-    }
-}
-
 // Masked mean squared error over the top-left R x S of each B x H2 x W2
 // output: loss partials per workgroup and dout = 2 (out - target) / (B R S)
-// inside the crop, 0 outside.
+// inside the crop, 0 outside.  Output b is compared with target row rows[b]
+// (b itself when rows is NULL).
 __global__ __launch_bounds__(kWG) void k_mse(const float* __restrict__ out, const float* __restrict__ target,
-                                             int64_t B, int64_t H2, int64_t W2, int64_t R, int64_t S,
-                                             float* __restrict__ dout, float* __restrict__ part)
+                                             const int64_t* __restrict__ rows, int64_t B, int64_t H2, int64_t W2,
+                                             int64_t R, int64_t S, float* __restrict__ dout,
+                                             float* __restrict__ part)
 {
     __shared__ float red[kWG];
     const int64_t n = B * H2 * W2;
@@ -369,7 +331,8 @@ __global__ __launch_bounds__(kWG) void k_mse(const float* __restrict__ out, cons
         const int64_t X = i % W2, Y = (i / W2) % H2, b = i / (W2 * H2);
         float d = 0.f;
         if (Y < R && X < S) {
-            d = out[i] - target[(b * R + Y) * S + X];
+            const int64_t tb = rows ? rows[b] : b;
+            d = out[i] - target[(tb * R + Y) * S + X];
             acc += d * d;
         }
         if (dout) dout[i] = scale * d;
@@ -381,6 +344,205 @@ __global__ __launch_bounds__(kWG) void k_mse(const float* __restrict__ out, cons
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Template loss: w_mse MSE(out, target) + w_kl mean_b KL(p_e || p_s) over the
+// R x S crop, with s = exp(sc out + off), e = exp(sc target + off) (outputs
+// clamped at 15 inside the exponential) and p = s / sum s.  Pass 1 (grid
+// kTLChunks x B): per chunk of an image, the sums of s, e, e (t - y) and
+// (y - t)^2.  Pass 2: dout = w_mse 2 (y - t) / (B R S) + w_kl sc (p_s - p_e) / B
+// inside the crop, 0 outside.  Pass 3 adds the two losses to acc[0], acc[1].
+constexpr int kTLChunks = 16;
+
+__global__ __launch_bounds__(kWG) void k_tl_sums(const float* __restrict__ out, const float* __restrict__ target,
+                                                 const int64_t* __restrict__ rows, int64_t H2, int64_t W2, int64_t R,
+                                                 int64_t S, float sc, float off, double* __restrict__ part)
+{
+    __shared__ double red[4][kWG];
+    const int64_t b = blockIdx.y, n = R * S;
+    const int64_t tb = rows ? rows[b] : b;
+    const int64_t c0 = n * blockIdx.x / kTLChunks, c1 = n * (blockIdx.x + 1) / kTLChunks;
+    double ss = 0.0, se = 0.0, a = 0.0, q = 0.0;
+    for (int64_t i = c0 + threadIdx.x; i < c1; i += kWG) {
+        const int64_t Y = i / S, X = i % S;
+        const float y = out[(b * H2 + Y) * W2 + X], t = target[tb * n + i];
+        const float sv = expf(fminf(y, 15.f) * sc + off), ev = expf(t * sc + off);
+        ss += sv;
+        se += ev;
+        a += (double)ev * (double)(t - y);
+        q += (double)(y - t) * (double)(y - t);
+    }
+    red[0][threadIdx.x] = ss;
+    red[1][threadIdx.x] = se;
+    red[2][threadIdx.x] = a;
+    red[3][threadIdx.x] = q;
+    __syncthreads();
+    for (int w = kWG / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[(b * kTLChunks + blockIdx.x) * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__device__ inline void tl_totals(const double* __restrict__ part, int64_t b, double& ss, double& se)
+{
+    ss = 0.0;
+    se = 0.0;
+    for (int c = 0; c < kTLChunks; ++c) {
+        ss += part[(b * kTLChunks + c) * 4];
+        se += part[(b * kTLChunks + c) * 4 + 1];
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_tl_grad(const float* __restrict__ out, const float* __restrict__ target,
+                                                 const int64_t* __restrict__ rows, int64_t B, int64_t H2, int64_t W2,
+                                                 int64_t R, int64_t S, float sc, float off, float w_mse, float w_kl,
+                                                 const double* __restrict__ part, float* __restrict__ dout)
+{
+    const int64_t b = blockIdx.y, n2 = H2 * W2;
+    const int64_t tb = rows ? rows[b] : b;
+    double ssd, sed;
+    tl_totals(part, b, ssd, sed);
+    const float inv_ss = (float)(1.0 / ssd), inv_se = (float)(1.0 / sed);
+    const float gm = w_mse * 2.f / (float)(B * R * S), gk = w_kl * sc / (float)B;
+    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n2; i += (int64_t)gridDim.x * kWG) {
+        const int64_t Y = i / W2, X = i % W2;
+        float d = 0.f;
+        if (Y < R && X < S) {
+            const float y = out[b * n2 + i], t = target[tb * R * S + Y * S + X];
+            const float ps = expf(fminf(y, 15.f) * sc + off) * inv_ss, pe = expf(t * sc + off) * inv_se;
+            d = gm * (y - t) + gk * (ps - pe);
+        }
+        dout[b * n2 + i] = d;
+    }
+}
+
+// acc[0] += MSE, acc[1] += mean over images of KL = sc A / Se + log Ss - log Se
+__global__ __launch_bounds__(64) void k_tl_accumulate(const double* __restrict__ part, int64_t B, int64_t R,
+                                                      int64_t S, float sc, double* __restrict__ acc)
+{
+    if (threadIdx.x != 0) return;
+    double q = 0.0, kl = 0.0;
+    for (int64_t b = 0; b < B; ++b) {
+        double ss, se, a = 0.0;
+        tl_totals(part, b, ss, se);
+        for (int c = 0; c < kTLChunks; ++c) {
+            a += part[(b * kTLChunks + c) * 4 + 2];
+            q += part[(b * kTLChunks + c) * 4 + 3];
+        }
+        kl += (double)sc * a / se + log(ss) - log(se);
+    }
+    acc[0] += q / (double)(B * R * S);
+    acc[1] += kl / (double)B;
+}
+
+// acc[0] += (sum of the nb partials, in a fixed order) / denom
+__global__ __launch_bounds__(64) void k_mse_accumulate(const float* __restrict__ part, int nb, double denom,
+                                                       double* __restrict__ acc)
+{
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < nb; ++i) s += (double)part[i];
+    acc[0] += s / denom;
+}
+
+// dst[i, c] = src[rows[i], c] for c < cols
+__global__ __launch_bounds__(kWG) void k_gather_rows(const float* __restrict__ src, int64_t lds,
+                                                     const int64_t* __restrict__ rows, int64_t n, int64_t cols,
+                                                     float* __restrict__ dst, int64_t ldd)
+{
+    for (int64_t t = (int64_t)blockIdx.x * kWG + threadIdx.x; t < n * cols; t += (int64_t)gridDim.x * kWG) {
+        const int64_t i = t / cols, c = t % cols;
+        dst[i * ldd + c] = src[rows[i] * lds + c];
+    }
+}
+
+// Template layout (bin = ix + nx iy, one template per row of ldt) <-> image
+// layout (image b: nx rows of ny, row ix holds the magnitude bins of colour
+// bin ix).  Both directions are 32 x 32 tile transposes through LDS, so reads
+// and writes stay coalesced.
+constexpr int kTile = 32;
+
+// img[b, ix, iy] = (log(T[b, ix + nx iy] + eps) - offset) / scale
+__global__ __launch_bounds__(kWG) void k_log_target(const float* __restrict__ T, int64_t ldt, int64_t nx, int64_t ny,
+                                                    float eps, float offset, float inv_scale, float* __restrict__ img)
+{
+    __shared__ float tile[kTile][kTile + 1];
+    const int64_t b = blockIdx.z;
+    const int64_t ix0 = (int64_t)blockIdx.x * kTile, iy0 = (int64_t)blockIdx.y * kTile;
+    const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;     // 32 x 8
+    const float* Tb = T + b * ldt;
+    for (int r = ty; r < kTile; r += kWG / kTile) {
+        const int64_t ix = ix0 + tx, iy = iy0 + r;
+        if (ix < nx && iy < ny) tile[r][tx] = (logf(Tb[ix + nx * iy] + eps) - offset) * inv_scale;
+    }
+    __syncthreads();
+    float* ib = img + b * nx * ny;
+    for (int r = ty; r < kTile; r += kWG / kTile) {
+        const int64_t ix = ix0 + r, iy = iy0 + tx;
+        if (ix < nx && iy < ny) ib[ix * ny + iy] = tile[tx][r];
+    }
+}
+
+// T[b, ix + nx iy] = max(exp(out[b, ix, iy] scale + offset) - eps, 0) from the
+// top-left nx x ny of each H2 x W2 network output
+__global__ __launch_bounds__(kWG) void k_log_template(const float* __restrict__ out, int64_t H2, int64_t W2,
+                                                      int64_t nx, int64_t ny, float eps, float offset, float scale,
+                                                      float* __restrict__ T, int64_t ldt)
+{
+    __shared__ float tile[kTile][kTile + 1];
+    const int64_t b = blockIdx.z;
+    const int64_t ix0 = (int64_t)blockIdx.x * kTile, iy0 = (int64_t)blockIdx.y * kTile;
+    const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+    const float* ob = out + b * H2 * W2;
+    for (int r = ty; r < kTile; r += kWG / kTile) {
+        const int64_t ix = ix0 + r, iy = iy0 + tx;
+        if (ix < nx && iy < ny) tile[r][tx] = fmaxf(expf(ob[ix * W2 + iy] * scale + offset) - eps, 0.f);
+    }
+    __syncthreads();
+    float* Tb = T + b * ldt;
+    for (int r = ty; r < kTile; r += kWG / kTile) {
+        const int64_t ix = ix0 + tx, iy = iy0 + r;
+        if (ix < nx && iy < ny) Tb[ix + nx * iy] = tile[tx][r];
+    }
+}
+
+// Per template k (one workgroup): with s' = max(S, floor), the sums
+// Se = sum E, Ss = sum s', A = sum_{E > 0} E log(E / s') in FP64 and
+// KL(E / Se || s' / Ss) = A / Se - log Se + log Ss.  out[3k..3k+2] = KL, Se, Ss.
+__global__ __launch_bounds__(kWG) void k_template_kl(const float* __restrict__ E, int64_t lde,
+                                                     const float* __restrict__ S, int64_t lds, int64_t nbins,
+                                                     double floor_, double* __restrict__ out)
+{
+    __shared__ double red[3][kWG];
+    const int64_t k = blockIdx.x;
+    const float* e = E + k * lde;
+    const float* s = S + k * lds;
+    double se = 0.0, ss = 0.0, a = 0.0;
+    for (int64_t i = threadIdx.x; i < nbins; i += kWG) {
+        const double ei = e[i], si = fmax((double)s[i], floor_);
+        se += ei;
+        ss += si;
+        if (ei > 0.0) a += ei * log(ei / si);
+    }
+    red[0][threadIdx.x] = se;
+    red[1][threadIdx.x] = ss;
+    red[2][threadIdx.x] = a;
+    __syncthreads();
+    for (int w = kWG / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        se = red[0][0];
+        ss = red[1][0];
+        a = red[2][0];
+        out[3 * k] = se > 0.0 ? a / se - log(se) + log(ss) : 0.0;
+        out[3 * k + 1] = se;
+        out[3 * k + 2] = ss;
+    }
 }
 
 // Adam step on FP32 master parameters (bias-corrected moments)
@@ -411,6 +573,70 @@ using namespace sfh::nn;
     std::lock_guard<std::recursive_mutex> _g(ctx->mu);     \
     if (int _rc = bind(ctx)) return _rc;
 
+namespace {
+
+int colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols, float alpha, float beta, float* out,
+           int64_t N, int64_t ldo, const char* name);
+
+// C = alpha A B^T + beta C.  Products with few output tiles and a long K (the
+// weight gradients, K = pixels) are split along K so that the launch fills the
+// chip: about 1024 workgroups, slices of at least 256, summed in order.
+int gemm(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* B, int64_t ldb, float* C, int64_t ldc,
+         int64_t M, int64_t N, int64_t K, float alpha, float beta)
+{
+    if (M == 0 || N == 0) return SFH_OK;
+    const int64_t tiles = cdiv(N, 128) * cdiv(M, 128);
+    int64_t splits = 1, kchunk = std::max<int64_t>(K, 16);
+    if (tiles < 512 && K >= 1024) {
+        splits = std::min<int64_t>(std::max<int64_t>(1, 1024 / tiles), K / 256);
+        kchunk = cdiv(cdiv(K, splits), 16) * 16;
+        splits = cdiv(K, kchunk);
+    }
+    float* out = C;
+    int64_t ldo = ldc;
+    if (splits > 1) {
+        void* p = nullptr;
+        if (int rc = scratch(ctx, "nn_splitk", (size_t)splits * M * N * sizeof(float), &p)) return rc;
+        out = (float*)p;
+        ldo = N;
+    }
+    dim3 grid((unsigned)cdiv(N, 128), (unsigned)cdiv(M, 128), (unsigned)splits);
+    // slices hold raw sums; alpha and beta are applied when they are added
+    const float ka = splits > 1 ? 1.f : alpha, kb = splits > 1 ? 0.f : beta;
+    {
+        TimedLaunch tl(ctx, "nn_gemm");
+        if (dtype == 0)
+            hipLaunchKernelGGL(k_gemm_nt<uint16_t>, grid, dim3(kWG), 0, ctx->stream, (const uint16_t*)A, lda,
+                               (const uint16_t*)B, ldb, out, ldo, M, N, K, ka, kb, kchunk);
+        else
+            hipLaunchKernelGGL(k_gemm_nt<float>, grid, dim3(kWG), 0, ctx->stream, (const float*)A, lda,
+                               (const float*)B, ldb, out, ldo, M, N, K, ka, kb, kchunk);
+    }
+    if (int rc = check_launch(ctx, "k_gemm_nt")) return rc;
+    // the slices are the rows of a splits x (M N) matrix
+    if (splits > 1) return colsum(ctx, out, M * N, splits, M * N, alpha, beta, C, N, ldc, "nn_splitk_part");
+    return SFH_OK;
+}
+
+// Column sums of X (rows x cols, ld) into out[(c / N) ldo + c % N] with alpha
+// and beta; N = ldo = cols for a plain vector.
+int colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols, float alpha, float beta, float* out,
+           int64_t N, int64_t ldo, const char* name)
+{
+    if (cols == 0) return SFH_OK;
+    const int64_t nparts = std::min<int64_t>(std::max<int64_t>(1, cdiv(rows, 64)), 1024);
+    const int64_t chunk = std::max<int64_t>(1, cdiv(rows, nparts));
+    void* p = nullptr;
+    if (int rc = scratch(ctx, name, (size_t)nparts * cols * sizeof(float), &p)) return rc;
+    hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)cdiv(cols, 64), (unsigned)nparts), dim3(kWG), 0, ctx->stream, X,
+                       ld, rows, cols, chunk, (float*)p);
+    hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)cdiv(cols, 64)), dim3(kWG), 0, ctx->stream, (const float*)p,
+                       nparts, cols, alpha, beta, out, N, ldo);
+    return check_launch(ctx, "k_colsum");
+}
+
+}  // namespace
+
 extern "C" {
 
 int sfh_nn_gemm_nt(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* B, int64_t ldb, float* C,
@@ -420,16 +646,7 @@ int sfh_nn_gemm_nt(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const vo
     SFH_ARG_CHECK(ctx, (dtype == 0 || dtype == 1) && K % 16 == 0 && lda >= K && ldb >= K && ldc >= N && M >= 0 && N >= 0,
                   "sfh_nn_gemm_nt: dtype 0 (BF16) or 1 (FP32), K a multiple of 16, lda, ldb >= K, ldc >= N");
     SFH_ARG_CHECK(ctx, lda % 8 == 0 && ldb % 8 == 0, "sfh_nn_gemm_nt: lda, ldb multiples of 8");
-    if (M == 0 || N == 0) return SFH_OK;
-    dim3 grid((unsigned)cdiv(N, 128), (unsigned)cdiv(M, 128));
-    TimedLaunch tl(ctx, "nn_gemm");
-    if (dtype == 0)
-        hipLaunchKernelGGL(k_gemm_nt<uint16_t>, grid, dim3(kWG), 0, ctx->stream, (const uint16_t*)A, lda,
-                           (const uint16_t*)B, ldb, C, ldc, M, N, K, alpha, beta);
-    else
-        hipLaunchKernelGGL(k_gemm_nt<float>, grid, dim3(kWG), 0, ctx->stream, (const float*)A, lda, (const float*)B,
-                           ldb, C, ldc, M, N, K, alpha, beta);
-    return check_launch(ctx, "k_gemm_nt");
+    return gemm(ctx, dtype, A, lda, B, ldb, C, ldc, M, N, K, alpha, beta);
 }
 
 int sfh_nn_bias_act(sfh_ctx* ctx, int dtype, const float* X, int64_t ldx, const float* bias, int64_t rows,
@@ -474,14 +691,8 @@ int sfh_nn_transpose(sfh_ctx* ctx, int dtype, const float* in, int64_t ldi, int6
 int sfh_nn_colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols, float beta, float* out)
 {
     SFH_NN_BEGIN(ctx);
-    const int64_t chunk = 256, nparts = std::max<int64_t>(1, cdiv(rows, chunk));
-    void* p = nullptr;
-    if (int rc = scratch(ctx, "nn_colsum", (size_t)nparts * cols * sizeof(float), &p)) return rc;
-    hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)cdiv(cols, kWG), (unsigned)nparts), dim3(kWG), 0, ctx->stream, X,
-                       ld, rows, cols, chunk, (float*)p);
-    hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)cdiv(cols, kWG)), dim3(kWG), 0, ctx->stream, (const float*)p,
-                       nparts, cols, beta, out);
-    return check_launch(ctx, "k_colsum");
+    SFH_ARG_CHECK(ctx, X && out && rows >= 0 && cols >= 0 && ld >= cols, "sfh_nn_colsum: bad arguments");
+    return colsum(ctx, X, ld, rows, cols, 1.f, beta, out, cols, cols, "nn_colsum");
 }
 
 int sfh_nn_col2im(sfh_ctx* ctx, const float* cols, int64_t ldc, int64_t B, int64_t H, int64_t W, int64_t Co,
@@ -490,7 +701,8 @@ int sfh_nn_col2im(sfh_ctx* ctx, const float* cols, int64_t ldc, int64_t B, int64
     SFH_NN_BEGIN(ctx);
     SFH_ARG_CHECK(ctx, ldc >= 16 * Co, "sfh_nn_col2im: ldc >= 16 Co");
     const int64_t n = B * 4 * H * W * Co;
-    hipLaunchKernelGGL(k_col2im, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, cols, ldc, B, H, W, Co, bias, out);
+    hipLaunchKernelGGL(k_col2im, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, cols, ldc, B, H, W, Co, bias,
+                       (const float*)nullptr, out);
     return check_launch(ctx, "k_col2im");
 }
 
@@ -509,61 +721,170 @@ int sfh_nn_im2col(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t
     return check_launch(ctx, "k_im2col");
 }
 
+// The single-channel output layer as the other layers: a matrix product to
+// the 16 taps of each input pixel, then the col2im gather (with the per-pixel
+// bias map); FP32 operands on the matrix cores.
 int sfh_nn_conv1_fwd(sfh_ctx* ctx, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci, const float* w,
                      const float* bmap, float* out)
 {
     SFH_NN_BEGIN(ctx);
-    const int64_t n = B * 4 * H * W;
-    hipLaunchKernelGGL(k_conv1_fwd, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, in, B, H, W, Ci, w, bmap, out);
-    return check_launch(ctx, "k_conv1_fwd");
+    SFH_ARG_CHECK(ctx, in && w && out && B >= 0 && H >= 1 && W >= 1 && Ci >= 16 && Ci % 16 == 0,
+                  "sfh_nn_conv1_fwd: Ci a positive multiple of 16");
+    const int64_t P = B * H * W;
+    if (P == 0) return SFH_OK;
+    void* p = nullptr;
+    if (int rc = scratch(ctx, "nn_conv1_cols", (size_t)P * 16 * sizeof(float), &p)) return rc;
+    float* cols = (float*)p;
+    if (int rc = gemm(ctx, 1, in, Ci, w, Ci, cols, 16, P, 16, Ci, 1.f, 0.f)) return rc;
+    const int64_t n = P * 4;
+    hipLaunchKernelGGL(k_col2im, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, (const float*)cols, (int64_t)16, B, H,
+                       W, (int64_t)1, (const float*)nullptr, bmap, out);
+    return check_launch(ctx, "k_col2im");
 }
 
+// Backward: dcols = im2col(dout) (P x 16); din = dcols w (P x Ci);
+// dw = dcols^T in (16 x Ci, K = pixels, split along K); dbmap = sum over b of dout.
 int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci,
                      const float* w, float* din, float* dw, float* dbmap)
 {
     SFH_NN_BEGIN(ctx);
-    const int64_t P = B * H * W;
-    if (din)
-        hipLaunchKernelGGL(k_conv1_bwd_in, dim3(grid_for(P * Ci)), dim3(kWG), 0, ctx->stream, dout, B, H, W, Ci, w,
-                           din);
-    if (dw) {
-        const int64_t chunk = 1024, nparts = cdiv(P, chunk);
-        void* p = nullptr;
-        if (int rc = scratch(ctx, "nn_conv1", (size_t)nparts * 16 * Ci * sizeof(float), &p)) return rc;
-        hipLaunchKernelGGL(k_conv1_bwd_w, dim3((unsigned)nparts), dim3(kWG), 0, ctx->stream, dout, in, B, H, W, Ci,
-                           chunk, (float*)p);
-        hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)cdiv(16 * Ci, kWG)), dim3(kWG), 0, ctx->stream,
-                           (const float*)p, nparts, 16 * Ci, 0.f, dw);
+    SFH_ARG_CHECK(ctx, dout && B >= 0 && H >= 1 && W >= 1 && Ci >= 16 && Ci % 16 == 0 && (!din || w) && (!dw || in),
+                  "sfh_nn_conv1_bwd: Ci a positive multiple of 16, w with din, in with dw");
+    const int64_t P = B * H * W, Pp = cdiv(P, 16) * 16;
+    if (P == 0) return SFH_OK;
+    void* p = nullptr;
+    if (din || dw) {
+        if (int rc = scratch(ctx, "nn_conv1_cols", (size_t)P * 16 * sizeof(float), &p)) return rc;
+        float* dcols = (float*)p;
+        hipLaunchKernelGGL(k_im2col<float>, dim3(grid_for(P * 16)), dim3(kWG), 0, ctx->stream, dout, B, H, W,
+                           (int64_t)1, dcols, (int64_t)16);
+        if (int rc = check_launch(ctx, "k_im2col")) return rc;
+        if (din) {
+            // din = dcols (P x 16) times w (16 x Ci): B operand w^T (Ci x 16)
+            void* q = nullptr;
+            if (int rc = scratch(ctx, "nn_conv1_wT", (size_t)16 * Ci * sizeof(float), &q)) return rc;
+            hipLaunchKernelGGL(k_transpose<float>, dim3(1, (unsigned)cdiv(Ci, 32)), dim3(kWG), 0, ctx->stream, w, Ci,
+                               (int64_t)16, Ci, (float*)q, (int64_t)16, Ci, 0);
+            if (int rc = gemm(ctx, 1, dcols, 16, q, 16, din, Ci, P, Ci, 16, 1.f, 0.f)) return rc;
+        }
+        if (dw) {
+            // dw = dcols^T (16 x P) times in (P x Ci): operands dcols^T and in^T, K = Pp (zero padded)
+            void* q = nullptr;
+            if (int rc = scratch(ctx, "nn_conv1_T", (size_t)(16 + Ci) * Pp * sizeof(float), &q)) return rc;
+            float* colsT = (float*)q;
+            float* inT = colsT + 16 * Pp;
+            hipLaunchKernelGGL(k_transpose<float>, dim3((unsigned)cdiv(Pp, 32), 1), dim3(kWG), 0, ctx->stream,
+                               (const float*)dcols, (int64_t)16, P, (int64_t)16, colsT, Pp, (int64_t)16, 0);
+            hipLaunchKernelGGL(k_transpose<float>, dim3((unsigned)cdiv(Pp, 32), (unsigned)cdiv(Ci, 32)), dim3(kWG), 0,
+                               ctx->stream, in, Ci, P, Ci, inT, Pp, Ci, 0);
+            if (int rc = check_launch(ctx, "k_transpose")) return rc;
+            if (int rc = gemm(ctx, 1, colsT, Pp, inT, Pp, dw, Ci, 16, Ci, Pp, 1.f, 0.f)) return rc;
+        }
     }
-    if (dbmap) {
-        // dbmap[Y,X] = sum_b dout[b,Y,X]: column sums of a B x (4 H W) matrix
-        const int64_t cols = 4 * H * W;
-        void* p = nullptr;
-        if (int rc = scratch(ctx, "nn_colsum", (size_t)cols * sizeof(float), &p)) return rc;
-        hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)cdiv(cols, kWG), 1), dim3(kWG), 0, ctx->stream, dout, cols,
-                           B, cols, B, (float*)p);
-        hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)cdiv(cols, kWG)), dim3(kWG), 0, ctx->stream,
-                           (const float*)p, (int64_t)1, cols, 0.f, dbmap);
-    }
-    return check_launch(ctx, "k_conv1_bwd");
+    if (dbmap)
+        if (int rc = colsum(ctx, dout, 4 * H * W, B, 4 * H * W, 1.f, 0.f, dbmap, 4 * H * W, 4 * H * W,
+                            "nn_colsum_map"))
+            return rc;
+    return SFH_OK;
 }
 
-int sfh_nn_mse(sfh_ctx* ctx, const float* out, const float* target, int64_t B, int64_t H2, int64_t W2, int64_t R,
-               int64_t S, float* dout, float* loss)
+int sfh_nn_mse(sfh_ctx* ctx, const float* out, const float* target, const int64_t* rows, int64_t B, int64_t H2,
+               int64_t W2, int64_t R, int64_t S, float* dout, float* loss, double* loss_acc)
 {
     SFH_NN_BEGIN(ctx);
-    SFH_ARG_CHECK(ctx, R <= H2 && S <= W2 && loss, "sfh_nn_mse: crop within the output, loss required");
+    SFH_ARG_CHECK(ctx, R <= H2 && S <= W2 && B >= 1, "sfh_nn_mse: crop within the output, B >= 1");
     const int nb = 512;
     void* p = nullptr;
     if (int rc = scratch(ctx, "nn_mse", nb * sizeof(float), &p)) return rc;
-    hipLaunchKernelGGL(k_mse, dim3(nb), dim3(kWG), 0, ctx->stream, out, target, B, H2, W2, R, S, dout, (float*)p);
+    hipLaunchKernelGGL(k_mse, dim3(nb), dim3(kWG), 0, ctx->stream, out, target, rows, B, H2, W2, R, S, dout,
+                       (float*)p);
+    if (int rc = check_launch(ctx, "k_mse")) return rc;
+    const double denom = (double)(B * R * S);
+    if (loss_acc) {
+        hipLaunchKernelGGL(k_mse_accumulate, dim3(1), dim3(64), 0, ctx->stream, (const float*)p, nb, denom, loss_acc);
+        if (int rc = check_launch(ctx, "k_mse_accumulate")) return rc;
+    }
+    if (!loss) return SFH_OK;
     float part[nb];
     SFH_HIP_TRY(ctx, hipMemcpyAsync(part, p, nb * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     SFH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     double s = 0.0;
     for (int i = 0; i < nb; ++i) s += part[i];
-    *loss = (float)(s / (double)(B * R * S));
+    *loss = (float)(s / denom);
     return SFH_OK;
+}
+
+int sfh_nn_template_loss(sfh_ctx* ctx, const float* out, const float* target, const int64_t* rows, int64_t B,
+                         int64_t H2, int64_t W2, int64_t R, int64_t S, float scale, float offset, float w_mse,
+                         float w_kl, float* dout, double* acc)
+{
+    SFH_NN_BEGIN(ctx);
+    SFH_ARG_CHECK(ctx, out && target && dout && B >= 1 && B <= 65535 && R >= 1 && S >= 1 && R <= H2 && S <= W2 &&
+                       scale > 0.f,
+                  "sfh_nn_template_loss: bad arguments");
+    void* p = nullptr;
+    if (int rc = scratch(ctx, "nn_tl", (size_t)B * kTLChunks * 4 * sizeof(double), &p)) return rc;
+    double* part = (double*)p;
+    hipLaunchKernelGGL(k_tl_sums, dim3(kTLChunks, (unsigned)B), dim3(kWG), 0, ctx->stream, out, target, rows, H2, W2,
+                       R, S, scale, offset, part);
+    hipLaunchKernelGGL(k_tl_grad, dim3((unsigned)std::min<int64_t>(cdiv(H2 * W2, kWG), 64), (unsigned)B), dim3(kWG),
+                       0, ctx->stream, out, target, rows, B, H2, W2, R, S, scale, offset, w_mse, w_kl,
+                       (const double*)part, dout);
+    if (acc)
+        hipLaunchKernelGGL(k_tl_accumulate, dim3(1), dim3(64), 0, ctx->stream, (const double*)part, B, R, S, scale,
+                           acc);
+    return check_launch(ctx, "k_tl");
+}
+
+int sfh_nn_gather_rows(sfh_ctx* ctx, const float* src, int64_t lds, const int64_t* rows, int64_t n, int64_t cols,
+                       float* dst, int64_t ldd)
+{
+    SFH_NN_BEGIN(ctx);
+    SFH_ARG_CHECK(ctx, src && rows && dst && n >= 0 && cols >= 0 && lds >= cols && ldd >= cols,
+                  "sfh_nn_gather_rows: bad arguments");
+    if (n == 0 || cols == 0) return SFH_OK;
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(n * cols)), dim3(kWG), 0, ctx->stream, src, lds, rows, n, cols,
+                       dst, ldd);
+    return check_launch(ctx, "k_gather_rows");
+}
+
+int sfh_nn_log_target(sfh_ctx* ctx, const float* T, int64_t ldt, int64_t B, int64_t nx, int64_t ny, float eps,
+                      float offset, float scale, float* img)
+{
+    SFH_NN_BEGIN(ctx);
+    SFH_ARG_CHECK(ctx, T && img && B >= 0 && nx >= 1 && ny >= 1 && ldt >= nx * ny && eps > 0.f && scale > 0.f &&
+                       B <= 65535,
+                  "sfh_nn_log_target: bad arguments");
+    if (B == 0) return SFH_OK;
+    hipLaunchKernelGGL(k_log_target, dim3((unsigned)cdiv(nx, kTile), (unsigned)cdiv(ny, kTile), (unsigned)B),
+                       dim3(kWG), 0, ctx->stream, T, ldt, nx, ny, eps, offset, 1.f / scale, img);
+    return check_launch(ctx, "k_log_target");
+}
+
+int sfh_nn_log_template(sfh_ctx* ctx, const float* out, int64_t B, int64_t H2, int64_t W2, int64_t nx, int64_t ny,
+                        float eps, float offset, float scale, float* T, int64_t ldt)
+{
+    SFH_NN_BEGIN(ctx);
+    SFH_ARG_CHECK(ctx, out && T && B >= 0 && nx >= 1 && ny >= 1 && nx <= H2 && ny <= W2 && ldt >= nx * ny &&
+                       B <= 65535,
+                  "sfh_nn_log_template: bad arguments");
+    if (B == 0) return SFH_OK;
+    hipLaunchKernelGGL(k_log_template, dim3((unsigned)cdiv(nx, kTile), (unsigned)cdiv(ny, kTile), (unsigned)B),
+                       dim3(kWG), 0, ctx->stream, out, H2, W2, nx, ny, eps, offset, scale, T, ldt);
+    return check_launch(ctx, "k_log_template");
+}
+
+int sfh_nn_template_kl(sfh_ctx* ctx, const float* E, int64_t lde, const float* S, int64_t lds, int64_t n,
+                       int64_t nbins, double floor_value, double* out)
+{
+    SFH_NN_BEGIN(ctx);
+    SFH_ARG_CHECK(ctx, E && S && out && n >= 0 && nbins >= 1 && lde >= nbins && lds >= nbins && floor_value > 0.0 &&
+                       n <= 2147483647,
+                  "sfh_nn_template_kl: bad arguments");
+    if (n == 0) return SFH_OK;
+    hipLaunchKernelGGL(k_template_kl, dim3((unsigned)n), dim3(kWG), 0, ctx->stream, E, lde, S, lds, nbins,
+                       floor_value, out);
+    return check_launch(ctx, "k_template_kl");
 }
 
 int sfh_nn_adam(sfh_ctx* ctx, float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,

@@ -10,9 +10,22 @@ Architecture (about 4.3M parameters):
       -> ConvT(32->1) + per-pixel bias map -> 208 x 304, top-left 200 x 300 kept
     ConvT = transposed convolution, 4x4 kernel, stride 2, padding 1.
 
+The network predicts images of y = (ln(T + LOG_EPS) - LOG_OFFSET) / LOG_SCALE
+(T per unit stellar mass, image row = colour bin, column = magnitude bin).
+
+Pipeline: ``sample_parameters`` draws a design over the parameter box,
+``exact_templates`` builds those templates with partial_cmd_smooth (host
+point preparation and the GPU deposition kernel), ``TemplateSurrogate.train``
+fits the network to them and ``fill_templates`` fills a fine
+(logAge, [M/H]) grid at one distance and error scale, checking the network
+against exact templates on a sparse lattice and computing exactly every cell
+whose check fails.  The check is the log-likelihood change of a fit:
+fitting N stars drawn from the exact template with the network's template
+instead (amplitude free) lowers the best log-likelihood by
+dlnL = N KL(exact || network) (``template_delta_lnL``).
+
 No reference counterpart: StarFormationHistories.jl builds every template
-with partial_cmd_smooth; here the network fills fine grids cheaply and
-partial_cmd_smooth remains the fallback (``fill_templates``).  PyTorch tensors
+with partial_cmd_smooth, which remains the fallback here.  PyTorch tensors
 serve only as device memory; all arithmetic runs in libsfh_hip.so.
 """
 from __future__ import annotations
@@ -34,6 +47,14 @@ CROP_H, CROP_W = 200, 300
 
 SILU = 1
 IDENT = 0
+
+# the log-template transform
+LOG_EPS = 1e-12
+LOG_OFFSET = -20.0
+LOG_SCALE = 5.0
+# isochrone wiggles are drawn from this seed for every template, so a template
+# is a function of its four parameters alone
+ISOCHRONE_SEED = 7
 
 
 def _pad16(n):
@@ -180,12 +201,19 @@ class TemplateSurrogate:
         B = x.shape[0]
         self._alloc(B)
         self.B = B
-        self._refresh_weights()
         x16 = np.zeros((B, 16), dtype=np.float32)
         x16[:, :4] = x
         self.x0f[:B * 16].copy_(self.torch.from_numpy(x16.ravel()))
         self.torch.cuda.synchronize()
+        self._forward_device(B)
+        self.ctx.synchronize()
+        return self.out[:B * OUT_H * OUT_W].view(B, OUT_H, OUT_W)
+
+    def _forward_device(self, B):
+        """The network on the scaled inputs already in x0f (B x 16), queued on
+        the context's stream."""
         v = self.views
+        self._refresh_weights()
         self._bias_act(self.x0f, 16, None, B, 16, IDENT, None, self.x0, 16)
         # MLP
         self._gemm(self.x0, 16, self.w["W0"], 16, self.z, HID0, B, HID0, 16)
@@ -211,8 +239,6 @@ class TemplateSurrogate:
                 self._bias_act(pres[li], CH, None, B * H * W, CH, SILU, None, self.a5, CH, dt=1)
         self._c(self.lib.sfh_nn_conv1_fwd(self.ctx.handle, self.a5.data_ptr(), B, H, W, CH, v["w4"].data_ptr(),
                                           v["bmap"].data_ptr(), self.out.data_ptr()))
-        self.ctx.synchronize()
-        return self.out[:B * OUT_H * OUT_W].view(B, OUT_H, OUT_W)
 
     # ---------------------------------------------------------- backward
     def backward(self, target):
@@ -223,8 +249,8 @@ class TemplateSurrogate:
         t = target.contiguous().view(-1)
         self.torch.cuda.synchronize()
         loss = C.c_float()
-        self._c(self.lib.sfh_nn_mse(self.ctx.handle, self.out.data_ptr(), t.data_ptr(), B, OUT_H, OUT_W, CROP_H,
-                                    CROP_W, self.dout.data_ptr(), C.byref(loss)))
+        self._c(self.lib.sfh_nn_mse(self.ctx.handle, self.out.data_ptr(), t.data_ptr(), None, B, OUT_H, OUT_W,
+                                    CROP_H, CROP_W, self.dout.data_ptr(), C.byref(loss), None))
         self._backward_from_dout()
         return float(loss.value)
 
@@ -294,6 +320,95 @@ class TemplateSurrogate:
         self.adam_step(lr)
         return loss
 
+    def images(self, T, nx=CROP_H, ny=CROP_W, out=None):
+        """Training images (n x nx x ny, FP32 on the device) of templates T
+        (n x nbins device tensor, bin = ix + nx iy)."""
+        n = T.shape[0]
+        if out is None:
+            out = self.torch.empty((n, nx, ny), dtype=self.torch.float32, device=self.dev)
+        self.torch.cuda.synchronize()
+        for b0 in range(0, n, 65535):
+            b1 = min(n, b0 + 65535)
+            self._c(self.lib.sfh_nn_log_target(self.ctx.handle, T[b0].data_ptr(), T.stride(0), b1 - b0, nx, ny,
+                                               LOG_EPS, LOG_OFFSET, LOG_SCALE, out[b0].data_ptr()))
+        self.ctx.synchronize()
+        return out
+
+    def train(self, params, images, steps, batch=32, lr=1e-3, lr_min=1e-5, warmup=200, seed=0, log_every=250,
+              weight_decay=0.0, kl_weight=0.0):
+        """Adam on shuffled mini-batches of (params: n x 4 physical, images:
+        n x 200 x 300 device tensor from ``images``) for ``steps`` steps; the
+        learning rate rises linearly over ``warmup`` steps, then follows a
+        cosine from lr to lr_min.  The loss is the MSE of the log image plus
+        kl_weight times the mean KL(exact || network) per star (the
+        log-likelihood a fit loses per star).  Runs without host
+        synchronisation except every ``log_every`` steps, when the means since
+        the last report are read.  Returns [(step, MSE, KL per star), ...]."""
+        torch = self.torch
+        n = images.shape[0]
+        if images.shape[1:] != (CROP_H, CROP_W) or len(params) != n:
+            raise ValueError("images must be n x 200 x 300 with one parameter row each")
+        self._alloc(batch)
+        xs = torch.from_numpy(scale_params(params).astype(np.float32)).to(self.dev)
+        rng = np.random.default_rng(seed)
+        per_epoch = max(1, n // batch)
+        nep = -(-steps // per_epoch)
+        rows = np.concatenate([rng.permutation(n)[:per_epoch * batch] for _ in range(nep)])
+        rows_d = torch.from_numpy(rows.astype(np.int64)).to(self.dev)
+        acc = torch.zeros(2, dtype=torch.float64, device=self.dev)
+        self.x0f.zero_()
+        torch.cuda.synchronize()
+        hist, last = [], 0
+        self.B = batch
+        for k in range(steps):
+            r = rows_d[k * batch:(k + 1) * batch]
+            self._c(self.lib.sfh_nn_gather_rows(self.ctx.handle, xs.data_ptr(), 4, r.data_ptr(), batch, 4,
+                                                self.x0f.data_ptr(), 16))
+            self._forward_device(batch)
+            self._c(self.lib.sfh_nn_template_loss(self.ctx.handle, self.out.data_ptr(), images.data_ptr(),
+                                                  r.data_ptr(), batch, OUT_H, OUT_W, CROP_H, CROP_W, LOG_SCALE,
+                                                  LOG_OFFSET, 1.0, float(kl_weight), self.dout.data_ptr(),
+                                                  acc.data_ptr()))
+            self._backward_from_dout()
+            if k < warmup:
+                rate = lr * (k + 1) / warmup
+            else:
+                u = (k - warmup) / max(1, steps - warmup)
+                rate = lr_min + 0.5 * (lr - lr_min) * (1.0 + np.cos(np.pi * u))
+            self.adam_step(rate, weight_decay=weight_decay)
+            if (k + 1) % log_every == 0 or k + 1 == steps:
+                self.ctx.synchronize()
+                a = acc.cpu().numpy() / (k + 1 - last)
+                hist.append((k + 1, float(a[0]), float(a[1])))
+                acc.zero_()
+                torch.cuda.synchronize()
+                last = k + 1
+        self.ctx.synchronize()
+        return hist
+
+    def predict(self, params, batch=256, out=None, nx=CROP_H, ny=CROP_W):
+        """Templates (n x nbins FP32 device tensor, bin = ix + nx iy, per unit
+        stellar mass) for n rows of physical parameters."""
+        p = np.atleast_2d(np.asarray(params, dtype=np.float64))
+        n = p.shape[0]
+        if out is None:
+            out = self.torch.empty((n, nx * ny), dtype=self.torch.float32, device=self.dev)
+        for b0 in range(0, n, batch):
+            b1 = min(n, b0 + batch)
+            y = self.forward(p[b0:b1])
+            self._c(self.lib.sfh_nn_log_template(self.ctx.handle, y.data_ptr(), b1 - b0, OUT_H, OUT_W, nx, ny,
+                                                 LOG_EPS, LOG_OFFSET, LOG_SCALE, out[b0].data_ptr(),
+                                                 out.stride(0)))
+        self.ctx.synchronize()
+        return out
+
+    def save(self, path):
+        np.save(path, self.parameters())
+
+    @classmethod
+    def load(cls, path, **kws):
+        return cls(params=np.load(path), **kws)
+
     def parameters(self) -> np.ndarray:
         self.ctx.synchronize()
         return self.flat.cpu().numpy()
@@ -301,3 +416,125 @@ class TemplateSurrogate:
     def gradient(self) -> np.ndarray:
         self.ctx.synchronize()
         return self.grad.cpu().numpy()
+
+
+# ---------------------------------------------------------------- data and fills
+
+def sample_parameters(n, seed=0, lo=PARAM_LO, hi=PARAM_HI):
+    """n rows of (logAge, [M/H], dmod, error scale): a scrambled stratified
+    (Latin hypercube) design over the box lo..hi."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+    u = (np.stack([rng.permutation(n) for _ in range(lo.size)], axis=1) + rng.uniform(size=(n, lo.size))) / n
+    return lo + u * (hi - lo)
+
+
+def exact_templates(params, edges=None, npoints=2000, correlation=0.6, mmax_imf=120.0, chunk=1024, out=None,
+                    ctx: Context | None = None):
+    """partial_cmd_smooth of each (logAge, [M/H], dmod, error scale) row on
+    the 200 x 300 grid: synthetic isochrone (seed ISOCHRONE_SEED), host point
+    preparation and the GPU deposition kernel.  Returns an n x nbins FP32
+    device tensor (bin = ix + nx iy, per unit stellar mass)."""
+    import torch
+    from . import synth
+    from .templates import PointBatch, deposit_batch
+    ctx = ctx or default_context()
+    edges = edges or synth.cfg1_edges()
+    p = np.atleast_2d(np.asarray(params, dtype=np.float64))
+    n = p.shape[0]
+    nbins = (edges[0].length - 1) * (edges[1].length - 1)
+    if out is None:
+        out = torch.empty((n, nbins), dtype=torch.float32, device="cuda")
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        pts = [synth.template_points(synth.TemplateSpec(r[0], r[1], ISOCHRONE_SEED), npoints, edges, mmax_imf,
+                                     dmod=r[2], correlation=correlation, err_scale=r[3]) for r in p[c0:c1]]
+        batch = PointBatch([q for q, _ in pts], edges,
+                           cmult_list=[c for _, c in pts] if correlation is not None else None)
+        deposit_batch(batch, out=out[c0:c1], ctx=ctx)
+        ctx.synchronize()
+    return out
+
+
+def template_delta_lnL(exact, approx, n_stars=1.0, floor=LOG_EPS, ctx: Context | None = None):
+    """Per template (rows of the two n x nbins device tensors): the
+    log-likelihood lost by fitting n_stars stars drawn from ``exact`` with
+    ``approx`` (floored at ``floor``) instead, amplitude free:
+    n_stars KL(exact || approx).  Returns (dlnL, sum exact, sum approx)."""
+    import torch
+    ctx = ctx or default_context()
+    n, nb = exact.shape
+    res = torch.empty(3 * n, dtype=torch.float64, device=exact.device)
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.sfh_nn_template_kl(ctx.handle, exact.data_ptr(), exact.stride(0), approx.data_ptr(),
+                                         approx.stride(0), n, nb, float(floor), res.data_ptr()))
+    ctx.synchronize()
+    r = res.cpu().numpy().reshape(n, 3)
+    return np.asarray(n_stars, dtype=np.float64) * r[:, 0], r[:, 1], r[:, 2]
+
+
+class FillResult:
+    """Templates of a filled grid and how each was obtained."""
+
+    def __init__(self, templates, log_ages, mhs, exact_mask, checked_mask, check_dlnL, seconds):
+        self.templates = templates        # (na * nm) x nbins device tensor, row i * nm + j
+        self.log_ages, self.mhs = log_ages, mhs
+        self.exact_mask = exact_mask      # computed with partial_cmd_smooth (checks and fallbacks)
+        self.checked_mask = checked_mask  # lattice points where the network was checked
+        self.check_dlnL = check_dlnL      # dlnL of the network at each checked point (nan elsewhere)
+        self.seconds = seconds
+
+    @property
+    def fallback_rate(self):
+        """Fraction of templates computed with partial_cmd_smooth."""
+        return float(self.exact_mask.mean())
+
+
+def _lattice(n, stride):
+    idx = list(range(0, n, stride))
+    if idx[-1] != n - 1:
+        idx.append(n - 1)
+    return np.asarray(idx)
+
+
+def fill_templates(surrogate: TemplateSurrogate, log_ages, mhs, dmod, err_scale, n_stars, threshold=0.5,
+                   check_stride=(5, 4), batch=256, exact_kws=None):
+    """Templates of the grid log_ages x mhs at distance modulus ``dmod`` and
+    error scale ``err_scale``.  The network fills every point; on the check
+    lattice (every check_stride-th age and metallicity, ends included) the
+    exact template (partial_cmd_smooth) is also built and replaces the
+    network's, and dlnL = n_stars KL(exact || network) is measured.  Every
+    grid point of a lattice cell with a corner dlnL above ``threshold`` is
+    computed with partial_cmd_smooth as well.  Returns a FillResult."""
+    import time
+    la, mh = np.asarray(log_ages, dtype=np.float64), np.asarray(mhs, dtype=np.float64)
+    na, nm = la.size, mh.size
+    grid = np.stack(np.meshgrid(la, mh, indexing="ij"), axis=-1).reshape(-1, 2)
+    full = np.column_stack([grid, np.full(na * nm, float(dmod)), np.full(na * nm, float(err_scale))])
+    kws = exact_kws or {}
+    t0 = time.perf_counter()
+    T = surrogate.predict(full, batch=batch)
+    t_net = time.perf_counter() - t0
+    ia, im = _lattice(na, check_stride[0]), _lattice(nm, check_stride[1])
+    checked = np.zeros((na, nm), dtype=bool)
+    checked[np.ix_(ia, im)] = True
+    cidx = np.flatnonzero(checked.ravel())
+    E = exact_templates(full[cidx], ctx=surrogate.ctx, **kws)
+    dl, _, _ = template_delta_lnL(E, T[cidx], n_stars, ctx=surrogate.ctx)
+    dgrid = np.full((na, nm), np.nan)
+    dgrid.ravel()[cidx] = dl
+    bad = np.zeros((na, nm), dtype=bool)
+    for a in range(ia.size - 1):
+        for b in range(im.size - 1):
+            c = dgrid[np.ix_(ia[a:a + 2], im[b:b + 2])]
+            if np.any(c > threshold):
+                bad[ia[a]:ia[a + 1] + 1, im[b]:im[b + 1] + 1] = True
+    T[cidx] = E
+    del E
+    fb = np.flatnonzero(bad.ravel() & ~checked.ravel())
+    if fb.size:
+        T[fb] = exact_templates(full[fb], ctx=surrogate.ctx, **kws)
+    exact = checked | bad
+    surrogate.ctx.synchronize()
+    secs = {"network": t_net, "total": time.perf_counter() - t0}
+    return FillResult(T, la, mh, exact.ravel(), checked.ravel(), dgrid.ravel(), secs)

@@ -135,16 +135,18 @@ def cfg1_specs(seed=20261202, n_age=70, n_mh=10):
 
 
 def template_points(spec: TemplateSpec, npoints, edges, mmax_imf, dmod=24.0, normalize_value=1.0,
-                    correlation=None):
+                    correlation=None, err_scale=1.0):
     """Host preparation (partial_cmd_smooth_points) of one synthetic template.
     y = V, colour = B - V (cov_mult = +1).  With ``correlation`` set, also
     returns per-point covariance multipliers c = -rho/sqrt(1-rho^2) * sx/sy,
     which give the covariant kernel exactly that colour-magnitude correlation
-    (corr = -c sy / sqrt(sx^2 + c^2 sy^2))."""
+    (corr = -c sy / sqrt(sx^2 + c^2 sy^2)).  ``err_scale`` multiplies the
+    photometric error model."""
     from .templates import partial_cmd_smooth_points
     imf = Kroupa2001(0.08, mmax_imf)
     m, mags = isochrone(spec.log_age, spec.mh, npoints, spec.seed, mmax_imf=mmax_imf)
-    sp = partial_cmd_smooth_points(m, mags, [photerr, photerr], 1, (0, 1), imf,
+    err = photerr if err_scale == 1.0 else (lambda v, s=float(err_scale): s * photerr(v))
+    sp = partial_cmd_smooth_points(m, mags, [err, err], 1, (0, 1), imf,
                                    [completeness, completeness], None, dmod=dmod,
                                    normalize_value=normalize_value, mean_mass=imf.mean(), edges=edges)
     cm = None

@@ -1,0 +1,220 @@
+"""GPU primitives of the surrogate pipeline against NumPy restatements
+(template <-> log-image transposes, the per-template KL behind dlnL, row
+gathers, the indexed MSE, split-K products, column sums) and the bookkeeping
+of fill_templates against exact templates built directly."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NX, NY = 200, 300
+
+
+def _templates(n, seed):
+    rng = np.random.default_rng(seed)
+    T = rng.lognormal(-14.0, 3.0, (n, NX * NY)).astype(np.float32)
+    T[rng.uniform(size=T.shape) < 0.3] = 0.0
+    return T
+
+
+def test_log_image_roundtrip(sfh, gpu_ctx):
+    import torch
+    from sfh_amd import surrogate as S
+    T = _templates(3, 1)
+    net = S.TemplateSurrogate(dtype="bf16", max_batch=3)
+    img = net.images(torch.from_numpy(T).cuda()).cpu().numpy()
+    ref = ((np.log(T.astype(np.float64) + S.LOG_EPS) - S.LOG_OFFSET) / S.LOG_SCALE).reshape(3, NY, NX)
+    ref = ref.transpose(0, 2, 1)
+    assert np.max(np.abs(img - ref)) <= 2e-6 * np.max(np.abs(ref))
+    # network-output layout (208 x 304, junk outside the crop) back to templates
+    out = np.random.default_rng(2).normal(0, 1, (3, S.OUT_H, S.OUT_W)).astype(np.float32)
+    out[:, :NX, :NY] = img
+    Td = torch.empty((3, NX * NY), dtype=torch.float32, device="cuda")
+    o = torch.from_numpy(out).cuda()
+    torch.cuda.synchronize()
+    gpu_ctx.check(gpu_ctx.lib.sfh_nn_log_template(gpu_ctx.handle, o.data_ptr(), 3, S.OUT_H, S.OUT_W, NX, NY,
+                                                  S.LOG_EPS, S.LOG_OFFSET, S.LOG_SCALE, Td.data_ptr(), NX * NY))
+    gpu_ctx.synchronize()
+    back = Td.cpu().numpy().astype(np.float64)
+    assert np.all(back >= 0.0)
+    assert np.all(np.abs(back - T) <= 2e-5 * T + 1e-16)
+
+
+def test_template_kl(sfh, gpu_ctx):
+    import torch
+    from sfh_amd import surrogate as S
+    E = _templates(5, 3)
+    Sx = E * np.random.default_rng(4).lognormal(0.0, 0.2, E.shape).astype(np.float32)
+    Sx[:, ::7] = 0.0                       # zeros in the approximation meet the floor
+    E[4] = 0.0                             # an empty exact template gives 0
+    dl, se, ss = S.template_delta_lnL(torch.from_numpy(E).cuda(), torch.from_numpy(Sx).cuda(), n_stars=250.0)
+    e, s = E.astype(np.float64), np.maximum(Sx.astype(np.float64), S.LOG_EPS)
+    for k in range(5):
+        sek, ssk = e[k].sum(), s[k].sum()
+        if sek == 0:
+            assert dl[k] == 0.0
+            continue
+        pe, ps = e[k] / sek, s[k] / ssk
+        m = pe > 0
+        kl = np.sum(pe[m] * np.log(pe[m] / ps[m]))
+        assert abs(dl[k] - 250.0 * kl) <= 1e-9 * 250.0 * kl + 1e-12
+        assert abs(se[k] - sek) <= 1e-12 * sek and abs(ss[k] - ssk) <= 1e-12 * ssk
+    # identical templates above the floor lose nothing
+    E2 = torch.from_numpy(E[:4] + np.float32(1e-9)).cuda()
+    dl0, _, _ = S.template_delta_lnL(E2, E2, 1e4)
+    assert np.all(np.abs(dl0) < 1e-9)
+
+
+def test_gather_and_indexed_mse(sfh, gpu_ctx):
+    import ctypes as C
+    import torch
+    rng = np.random.default_rng(7)
+    src = rng.normal(size=(50, 4)).astype(np.float32)
+    rows = rng.integers(0, 50, 9).astype(np.int64)
+    s_d, r_d = torch.from_numpy(src).cuda(), torch.from_numpy(rows).cuda()
+    dst = torch.full((9, 16), 7.0, device="cuda")
+    torch.cuda.synchronize()
+    gpu_ctx.check(gpu_ctx.lib.sfh_nn_gather_rows(gpu_ctx.handle, s_d.data_ptr(), 4, r_d.data_ptr(), 9, 4,
+                                                 dst.data_ptr(), 16))
+    gpu_ctx.synchronize()
+    d = dst.cpu().numpy()
+    assert np.array_equal(d[:, :4], src[rows]) and np.all(d[:, 4:] == 7.0)
+    # MSE of 3 outputs (H2 x W2) against target rows picked by index
+    B, H2, W2, R, S_ = 3, 12, 10, 9, 7
+    out = rng.normal(size=(B, H2, W2)).astype(np.float32)
+    tgt = rng.normal(size=(6, R, S_)).astype(np.float32)
+    idx = np.array([4, 0, 4], dtype=np.int64)
+    o_d, t_d, i_d = torch.from_numpy(out).cuda(), torch.from_numpy(tgt).cuda(), torch.from_numpy(idx).cuda()
+    dout = torch.empty_like(o_d)
+    acc = torch.full((1,), 0.5, dtype=torch.float64, device="cuda")
+    loss = C.c_float()
+    torch.cuda.synchronize()
+    gpu_ctx.check(gpu_ctx.lib.sfh_nn_mse(gpu_ctx.handle, o_d.data_ptr(), t_d.data_ptr(), i_d.data_ptr(), B, H2, W2,
+                                         R, S_, dout.data_ptr(), C.byref(loss), acc.data_ptr()))
+    diff = np.zeros_like(out, dtype=np.float64)
+    diff[:, :R, :S_] = out[:, :R, :S_].astype(np.float64) - tgt[idx]
+    ref = np.sum(diff ** 2) / (B * R * S_)
+    assert abs(loss.value - ref) <= 1e-5 * ref
+    assert abs(float(acc.item()) - 0.5 - ref) <= 1e-5 * ref
+    assert np.allclose(dout.cpu().numpy(), 2 * diff / (B * R * S_), rtol=1e-5, atol=1e-9)
+
+
+@pytest.mark.parametrize("dtype", [0, 1])
+@pytest.mark.parametrize("M,N,K", [(200, 40, 20000), (16, 32, 50016), (300, 130, 64)])
+def test_gemm_split_k(sfh, gpu_ctx, dtype, M, N, K):
+    import torch
+    rng = np.random.default_rng(M + N + K)
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    Bm = rng.normal(size=(N, K)).astype(np.float32)
+    C0 = rng.normal(size=(M, N)).astype(np.float32)
+    tdt = torch.bfloat16 if dtype == 0 else torch.float32
+    a_d, b_d = torch.from_numpy(A).to(tdt).cuda(), torch.from_numpy(Bm).to(tdt).cuda()
+    c_d = torch.from_numpy(C0).cuda()
+    torch.cuda.synchronize()
+    gpu_ctx.check(gpu_ctx.lib.sfh_nn_gemm_nt(gpu_ctx.handle, dtype, a_d.data_ptr(), K, b_d.data_ptr(), K,
+                                             c_d.data_ptr(), N, M, N, K, 0.5, 2.0))
+    gpu_ctx.synchronize()
+    Aq = a_d.float().cpu().numpy().astype(np.float64)
+    Bq = b_d.float().cpu().numpy().astype(np.float64)
+    ref = 0.5 * Aq @ Bq.T + 2.0 * C0
+    scale = 0.5 * np.sqrt(K) + 2.0 * np.abs(C0)
+    assert np.max(np.abs(c_d.cpu().numpy() - ref) / scale) < 3e-5
+
+
+@pytest.mark.parametrize("rows,cols", [(70000, 32), (32, 63232), (5, 300)])
+def test_colsum(sfh, gpu_ctx, rows, cols):
+    import torch
+    rng = np.random.default_rng(rows)
+    X = rng.normal(size=(rows, cols)).astype(np.float32)
+    out0 = rng.normal(size=cols).astype(np.float32)
+    x_d, o_d = torch.from_numpy(X).cuda(), torch.from_numpy(out0).cuda()
+    torch.cuda.synchronize()
+    gpu_ctx.check(gpu_ctx.lib.sfh_nn_colsum(gpu_ctx.handle, x_d.data_ptr(), cols, rows, cols, 0.5, o_d.data_ptr()))
+    gpu_ctx.synchronize()
+    ref = X.astype(np.float64).sum(0) + 0.5 * out0
+    assert np.max(np.abs(o_d.cpu().numpy() - ref)) <= 1e-5 * np.sqrt(rows) + 1e-6
+
+
+def test_fill_templates_bookkeeping(sfh, gpu_ctx):
+    """An untrained network fails every check (all templates exact); an
+    infinite threshold keeps the network everywhere off the lattice."""
+    import torch
+    from sfh_amd import surrogate as S
+    net = S.TemplateSurrogate(dtype="bf16", seed=3, max_batch=32)
+    la = np.array([8.0, 8.1, 8.2, 8.3, 8.4])
+    mh = np.array([-1.5, -1.0, -0.5, 0.0])
+    full = np.column_stack([np.repeat(la, 4), np.tile(mh, 5), np.full(20, 26.0), np.full(20, 1.2)])
+    E = S.exact_templates(full).cpu().numpy()
+    r = S.fill_templates(net, la, mh, 26.0, 1.2, n_stars=100.0, check_stride=(2, 2))
+    assert r.exact_mask.all() and r.fallback_rate == 1.0
+    chk = np.zeros((5, 4), dtype=bool)
+    chk[np.ix_([0, 2, 4], [0, 2, 3])] = True
+    assert np.array_equal(r.checked_mask, chk.ravel())
+    assert np.all(np.isfinite(r.check_dlnL[r.checked_mask])) and np.all(r.check_dlnL[r.checked_mask] > 0.5)
+    T = r.templates.cpu().numpy()
+    assert np.allclose(T, E, rtol=1e-6, atol=0)
+    r2 = S.fill_templates(net, la, mh, 26.0, 1.2, n_stars=100.0, threshold=np.inf, check_stride=(2, 2))
+    assert np.array_equal(r2.exact_mask, chk.ravel())
+    T2 = r2.templates.cpu().numpy()
+    P = net.predict(full).cpu().numpy()
+    assert np.allclose(T2[chk.ravel()], E[chk.ravel()], rtol=1e-6, atol=0)
+    assert np.array_equal(T2[~chk.ravel()], P[~chk.ravel()])
+
+
+def test_train_lowers_loss_on_exact_templates(sfh, gpu_ctx):
+    from sfh_amd import surrogate as S
+    P = S.sample_parameters(96, seed=5, lo=[8.0, -1.5, 25.0, 0.8], hi=[9.0, -0.5, 27.0, 1.5])
+    net = S.TemplateSurrogate(dtype="bf16", seed=1, max_batch=16)
+    img = net.images(S.exact_templates(P))
+    hist = net.train(P, img, steps=300, batch=16, lr=1e-3, warmup=30, log_every=100, kl_weight=0.1)
+    assert len(hist) == 3
+    assert hist[-1][1] < 0.2 * hist[0][1] and hist[-1][2] < 0.5 * hist[0][2]
+
+
+def _tl_ref(y, t, sc, off):
+    """FP64 restatement: MSE and KL(p_e || p_s) of one image pair."""
+    sv, ev = np.exp(np.minimum(y, 15.0) * sc + off), np.exp(t * sc + off)
+    pe, ps = ev / ev.sum(), sv / sv.sum()
+    return np.mean((y - t) ** 2), np.sum(pe * (np.log(pe) - np.log(ps)))
+
+
+def test_template_loss_against_restatement_and_differences(sfh, gpu_ctx):
+    import torch
+    rng = np.random.default_rng(11)
+    B, H2, W2, R, S_ = 2, 14, 12, 11, 9
+    sc, off, wm, wk = 5.0, -20.0, 0.7, 0.3
+    out = rng.normal(0.0, 0.8, (B, H2, W2)).astype(np.float32)
+    tgt = rng.normal(0.0, 0.8, (3, R, S_)).astype(np.float32)
+    idx = np.array([2, 0], dtype=np.int64)
+    o_d, t_d, i_d = torch.from_numpy(out).cuda(), torch.from_numpy(tgt).cuda(), torch.from_numpy(idx).cuda()
+    dout = torch.empty_like(o_d)
+    acc = torch.zeros(2, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    gpu_ctx.check(gpu_ctx.lib.sfh_nn_template_loss(gpu_ctx.handle, o_d.data_ptr(), t_d.data_ptr(), i_d.data_ptr(), B,
+                                                   H2, W2, R, S_, sc, off, wm, wk, dout.data_ptr(), acc.data_ptr()))
+    gpu_ctx.synchronize()
+    y64, t64 = out.astype(np.float64), tgt.astype(np.float64)
+
+    def total(yy):
+        m = k = 0.0
+        for b in range(B):
+            mb, kb = _tl_ref(yy[b, :R, :S_], t64[idx[b]], sc, off)
+            m += mb / B
+            k += kb / B
+        return m, k
+
+    mse, kl = total(y64)
+    a = acc.cpu().numpy()
+    assert abs(a[0] - mse) <= 1e-6 * mse and abs(a[1] - kl) <= 1e-5 * kl
+    # the gradient of wm MSE + wk KL by central differences of the FP64 restatement
+    g = dout.cpu().numpy()
+    assert np.all(g[:, R:, :] == 0.0) and np.all(g[:, :, S_:] == 0.0)
+    h = 1e-5
+    for (b, yy, xx) in [(0, 0, 0), (0, 5, 3), (1, 10, 8), (1, 2, 7)]:
+        yp, ym = y64.copy(), y64.copy()
+        yp[b, yy, xx] += h
+        ym[b, yy, xx] -= h
+        fp = np.dot([wm, wk], total(yp))
+        fm = np.dot([wm, wk], total(ym))
+        num = (fp - fm) / (2 * h)
+        assert abs(g[b, yy, xx] - num) <= 1e-4 * abs(num) + 1e-7
