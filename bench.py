@@ -41,6 +41,11 @@ def parse():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--fits", type=int, default=256, help="independent fits in the uneven multi-fit run")
     ap.add_argument("--no-multifit", action="store_true")
+    ap.add_argument("--surrogate-steps", type=int, default=25000,
+                    help="training steps (batch 32) of the template surrogate on 20,000 templates")
+    ap.add_argument("--surrogate-stars", type=float, default=100.0,
+                    help="stars per template in the dlnL check of the filled grid")
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates HBM counters)")
     return ap.parse_args()
@@ -247,6 +252,123 @@ def multifit_bench(sfh, torch, ctx, O, args, ncpu):
     return out
 
 
+def surrogate_bench(sfh, torch, ctx, O, args, ncpu):
+    """The learned template surrogate: 20,000 exact templates over (logAge
+    6.6-10.1, [M/H] -2.5-0.3, dmod 24-30, error scale 0.5-2) from the
+    deposition kernel, BF16 training, held-out accuracy, and the fill of a
+    0.01-dex grid (351 ages x 34 metallicities = 11,934 templates at dmod 27,
+    error scale 1) with partial_cmd_smooth wherever the lattice check finds
+    dlnL > 0.5; an audit of 1,000 filled templates against exact ones; and
+    partial_cmd_smooth on the host cores for the same templates."""
+    from sfh_amd import surrogate as S
+    from sfh_amd import synth
+    out = {}
+    t0 = time.time()
+    P = S.sample_parameters(20000, seed=20261204)
+    T = S.exact_templates(P, ctx=ctx)
+    gen_s = time.time() - t0
+    net = S.TemplateSurrogate(dtype="bf16", seed=0, ctx=ctx, max_batch=32)
+    img = net.images(T)
+    del T
+    torch.cuda.empty_cache()
+    log(f"surrogate: 20,000 exact training templates in {gen_s:.1f} s ({20000 / gen_s:.0f} templates/s, "
+        f"host preparation + GPU deposition)")
+    steps, batch = args.surrogate_steps, 32
+    t0 = time.time()
+    hist = net.train(P, img, steps=steps, batch=batch, lr=1e-3, log_every=max(1, steps // 5))
+    train_s = time.time() - t0
+    log(f"surrogate training: {steps} steps x {batch} in {train_s:.1f} s = {steps * batch / train_s:.0f} "
+        f"templates/s; (step, MSE, KL/star, log-sum err^2): {[tuple(round(v, 6) for v in h) for h in hist]}")
+    del img
+    torch.cuda.empty_cache()
+    Pv = S.sample_parameters(500, seed=20261205)
+    Ev = S.exact_templates(Pv, ctx=ctx)
+    dl, se, ss = S.template_delta_lnL(Ev, net.predict(Pv), 1.0, ctx=ctx)
+    del Ev
+    q = [float(v) for v in np.quantile(dl, [0.5, 0.9, 0.99, 1.0])]
+    out.update({"parameters": int(net.nparams), "train_templates": 20000, "train_seconds": train_s,
+                "train_steps": steps, "batch": batch, "train_templates_per_s": steps * batch / train_s,
+                "exact_templates_per_s": 20000 / gen_s, "loss_history": hist,
+                "heldout_KL_per_star_q50_q90_q99_max": q,
+                "heldout_sum_ratio_q01_q50_q99": [float(v) for v in np.quantile(ss / se, [0.01, 0.5, 0.99])],
+                "heldout_fail_frac": {str(n): float((dl * n > 0.5).mean()) for n in (10, 100, 1000)}})
+    log(f"surrogate held-out (500 over the whole box): KL/star q50/q90/q99/max {q}; "
+        f"dlnL > 0.5 at 100 stars: {out['heldout_fail_frac']['100']:.3f}")
+    # the fill
+    la = np.round(6.6 + 0.01 * np.arange(351), 2)
+    mh = np.linspace(-2.5, 0.3, 34)
+    net.predict(np.array([[8.0, -1.0, 27.0, 1.0]]))        # warm-up of the inference shapes
+    for nstars in (args.surrogate_stars, 10.0 * args.surrogate_stars):
+        out[f"fill_{nstars:g}_stars"] = _fill_bench(S, torch, ctx, O, args, ncpu, net, la, mh, nstars)
+    return out
+
+
+def _fill_bench(S, torch, ctx, O, args, ncpu, net, la, mh, nstars):
+    from sfh_amd import synth
+    out = {}
+    fill = S.fill_templates(net, la, mh, 27.0, 1.0, n_stars=nstars, check_stride=(5, 4))
+    n = la.size * mh.size
+    chk = fill.check_dlnL[fill.checked_mask]
+    out["fill"] = {"grid": [int(la.size), int(mh.size)], "templates": n, "dmod": 27.0, "err_scale": 1.0,
+                   "n_stars": nstars, "network_seconds": fill.seconds["network"],
+                   "network_templates_per_s": n / fill.seconds["network"], "total_seconds": fill.seconds["total"],
+                   "templates_per_s": n / fill.seconds["total"], "checked": int(fill.checked_mask.sum()),
+                   "fallback_rate": fill.fallback_rate,
+                   "check_dlnL_q50_q90_max": [float(v) for v in np.quantile(chk, [0.5, 0.9, 1.0])]}
+    log(f"surrogate fill: {n} templates in {fill.seconds['total']:.2f} s (network {fill.seconds['network']:.2f} s); "
+        f"{int(fill.checked_mask.sum())} checked, exact fraction {fill.fallback_rate:.3f}")
+    # audit: 1,000 filled templates off the lattice against exact templates
+    grid = np.stack(np.meshgrid(la, mh, indexing="ij"), axis=-1).reshape(-1, 2)
+    off = np.flatnonzero(~fill.exact_mask)
+    rng = np.random.default_rng(3)
+    pick = rng.choice(off, size=min(1000, off.size), replace=False) if off.size else off
+    if pick.size:
+        full = np.column_stack([grid[pick], np.full(pick.size, 27.0), np.full(pick.size, 1.0)])
+        t0 = time.time()
+        Ea = S.exact_templates(full, ctx=ctx)
+        ex_s = time.time() - t0
+        idx = torch.from_numpy(pick.astype(np.int64)).to(fill.templates.device)
+        da, _, _ = S.template_delta_lnL(Ea, fill.templates.index_select(0, idx), nstars, ctx=ctx)
+        del Ea
+        out["fill"]["audit"] = {"templates": int(pick.size), "dlnL_q50_q90_q99_max":
+                                [float(v) for v in np.quantile(da, [0.5, 0.9, 0.99, 1.0])],
+                                "frac_within_0.5": float((da <= 0.5).mean())}
+        exact_gpu_rate = pick.size / ex_s
+        log(f"surrogate audit: {pick.size} network templates, dlnL at {nstars:g} stars "
+            f"q50/q90/q99/max {out['fill']['audit']['dlnL_q50_q90_q99_max']}, "
+            f"{out['fill']['audit']['frac_within_0.5']:.3f} within 0.5")
+    else:
+        exact_gpu_rate = 20000 / gen_s
+    out["fill"]["exact_gpu_seconds_projected"] = n / exact_gpu_rate
+    out["fill"]["speedup_vs_exact_gpu"] = n / exact_gpu_rate / fill.seconds["total"]
+    if not args.no_cpu and nstars == args.surrogate_stars:
+        # partial_cmd_smooth on the host: the same host preparation, then the C
+        # FP64 restatement of bin_cmd_smooth on all host cores; bounded sample
+        edges = synth.cfg1_edges()
+        ex_, ey_ = (O.Range(e.first, e.step, e.length) for e in edges)
+        rng = np.random.default_rng(4)
+        cpu_s, k = 0.0, 0
+        while cpu_s < args.cpu_seconds and k < 64:
+            r = grid[rng.integers(0, n)]
+            t1 = time.time()
+            sp, cm = synth.template_points(synth.TemplateSpec(r[0], r[1], S.ISOCHRONE_SEED), 2000, edges, 120.0,
+                                           dmod=27.0, correlation=0.6)
+            O.bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, sp.cov_mult, sp.weights, ex_, ey_,
+                             cmult=cm, nthreads=ncpu)
+            cpu_s += time.time() - t1
+            k += 1
+        rate = k / cpu_s
+        out["fill"]["cpu_partial_cmd_smooth"] = {"templates_per_s": rate, "cores": ncpu, "sample": k,
+                                                 "projected_seconds": n / rate}
+        out["fill"]["speedup_vs_cpu"] = n / rate / fill.seconds["total"]
+        log(f"surrogate: host partial_cmd_smooth {rate:.3g} templates/s on {ncpu} threads; the fill is "
+            f"{out['fill']['speedup_vs_cpu']:.0f}x the host and {out['fill']['speedup_vs_exact_gpu']:.1f}x "
+            f"exact templates on the GPU")
+    del fill
+    torch.cuda.empty_cache()
+    return out["fill"]
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -420,6 +542,8 @@ def main():
 
     if not args.no_extras and not args.no_multifit:
         extras["multifit"] = multifit_bench(sfh, torch, ctx, O, args, ncpu)
+    if not args.no_extras and not args.no_surrogate:
+        extras["surrogate"] = surrogate_bench(sfh, torch, ctx, O, args, ncpu)
 
     line = {
         "metric": "loglike+grad evals/s (700 tpl×60k bins) batched vs plain; templates/s; HBM GB/s",

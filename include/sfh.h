@@ -345,14 +345,16 @@ int sfh_nn_mse(sfh_ctx* ctx, const float* out, const float* target, const int64_
                int64_t W2, int64_t R, int64_t S, float* dout, float* loss, double* loss_acc);
 /* Training loss of the surrogate over the R x S crop of B outputs of H2 x W2
  * (log images, target rows chosen by rows as in sfh_nn_mse):
- *   w_mse MSE + w_kl mean_b KL(p_e || p_s),  s = exp(scale out + offset),
- *   e = exp(scale target + offset), p = s / sum s,
+ *   w_mse MSE + mean_b [w_kl KL(p_e || p_s) + w_sum (log sum s - log sum e)^2],
+ *   s = exp(scale out + offset), e = exp(scale target + offset), p = s / sum s,
  * the KL being the log-likelihood lost per star by fitting with s instead of
- * e.  dout gets the gradient (zero outside the crop); acc (device, 2 doubles,
- * may be NULL) has the MSE and the mean KL added.  Asynchronous. */
+ * e, the last term the error of the template's normalisation.  dout gets the
+ * gradient (zero outside the crop); acc (device, 3 doubles, may be NULL) has
+ * the MSE, the mean KL and the mean squared log-sum error added.
+ * Asynchronous. */
 int sfh_nn_template_loss(sfh_ctx* ctx, const float* out, const float* target, const int64_t* rows, int64_t B,
                          int64_t H2, int64_t W2, int64_t R, int64_t S, float scale, float offset, float w_mse,
-                         float w_kl, float* dout, double* acc);
+                         float w_kl, float w_sum, float* dout, double* acc);
 /* dst[i, :cols] = src[rows[i], :cols] (FP32). */
 int sfh_nn_gather_rows(sfh_ctx* ctx, const float* src, int64_t lds, const int64_t* rows, int64_t n, int64_t cols,
                        float* dst, int64_t ldd);

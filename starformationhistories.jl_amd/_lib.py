@@ -120,7 +120,7 @@ SIGNATURES = [
     ("sfh_nn_conv1_bwd", _int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp]),
     ("sfh_nn_mse", _int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, C.POINTER(C.c_float), _vp]),
     ("sfh_nn_template_loss", _int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, C.c_float, C.c_float,
-                                    C.c_float, C.c_float, _vp, _vp]),
+                                    C.c_float, C.c_float, C.c_float, _vp, _vp]),
     ("sfh_nn_gather_rows", _int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64]),
     ("sfh_nn_log_target", _int, [_vp, _vp, _i64, _i64, _i64, _i64, C.c_float, C.c_float, C.c_float, _vp]),
     ("sfh_nn_log_template", _int, [_vp, _vp, _i64, _i64, _i64, _i64, _i64, C.c_float, C.c_float, C.c_float, _vp,

@@ -346,12 +346,14 @@ __global__ __launch_bounds__(kWG) void k_mse(const float* __restrict__ out, cons
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// Template loss: w_mse MSE(out, target) + w_kl mean_b KL(p_e || p_s) over the
-// R x S crop, with s = exp(sc out + off), e = exp(sc target + off) (outputs
-// clamped at 15 inside the exponential) and p = s / sum s.  Pass 1 (grid
-// kTLChunks x B): per chunk of an image, the sums of s, e, e (t - y) and
-// (y - t)^2.  Pass 2: dout = w_mse 2 (y - t) / (B R S) + w_kl sc (p_s - p_e) / B
-// inside the crop, 0 outside.  Pass 3 adds the two losses to acc[0], acc[1].
+// Template loss: w_mse MSE(out, target) + mean_b [w_kl KL(p_e || p_s) +
+// w_sum (log Ss - log Se)^2] over the R x S crop, with s = exp(sc out + off),
+// e = exp(sc target + off) (outputs clamped at 15 inside the exponential),
+// Ss = sum s, Se = sum e and p = s / Ss.  Pass 1 (grid kTLChunks x B): per
+// chunk of an image, the sums of s, e, e (t - y) and (y - t)^2.  Pass 2:
+// dout = w_mse 2 (y - t) / (B R S) + sc [w_kl (p_s - p_e) +
+// 2 w_sum (log Ss - log Se) p_s] / B inside the crop, 0 outside.  Pass 3 adds
+// the three terms (unweighted) to acc[0..2].
 constexpr int kTLChunks = 16;
 
 __global__ __launch_bounds__(kWG) void k_tl_sums(const float* __restrict__ out, const float* __restrict__ target,
@@ -398,7 +400,8 @@ __device__ inline void tl_totals(const double* __restrict__ part, int64_t b, dou
 __global__ __launch_bounds__(kWG) void k_tl_grad(const float* __restrict__ out, const float* __restrict__ target,
                                                  const int64_t* __restrict__ rows, int64_t B, int64_t H2, int64_t W2,
                                                  int64_t R, int64_t S, float sc, float off, float w_mse, float w_kl,
-                                                 const double* __restrict__ part, float* __restrict__ dout)
+                                                 float w_sum, const double* __restrict__ part,
+                                                 float* __restrict__ dout)
 {
     const int64_t b = blockIdx.y, n2 = H2 * W2;
     const int64_t tb = rows ? rows[b] : b;
@@ -406,24 +409,26 @@ __global__ __launch_bounds__(kWG) void k_tl_grad(const float* __restrict__ out, 
     tl_totals(part, b, ssd, sed);
     const float inv_ss = (float)(1.0 / ssd), inv_se = (float)(1.0 / sed);
     const float gm = w_mse * 2.f / (float)(B * R * S), gk = w_kl * sc / (float)B;
+    const float gs = (float)(2.0 * w_sum * (log(ssd) - log(sed))) * sc / (float)B;
     for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n2; i += (int64_t)gridDim.x * kWG) {
         const int64_t Y = i / W2, X = i % W2;
         float d = 0.f;
         if (Y < R && X < S) {
             const float y = out[b * n2 + i], t = target[tb * R * S + Y * S + X];
             const float ps = expf(fminf(y, 15.f) * sc + off) * inv_ss, pe = expf(t * sc + off) * inv_se;
-            d = gm * (y - t) + gk * (ps - pe);
+            d = gm * (y - t) + gk * (ps - pe) + gs * ps;
         }
         dout[b * n2 + i] = d;
     }
 }
 
-// acc[0] += MSE, acc[1] += mean over images of KL = sc A / Se + log Ss - log Se
+// acc[0] += MSE, acc[1] += mean over images of KL = sc A / Se + log Ss - log Se,
+// acc[2] += mean of (log Ss - log Se)^2
 __global__ __launch_bounds__(64) void k_tl_accumulate(const double* __restrict__ part, int64_t B, int64_t R,
                                                       int64_t S, float sc, double* __restrict__ acc)
 {
     if (threadIdx.x != 0) return;
-    double q = 0.0, kl = 0.0;
+    double q = 0.0, kl = 0.0, sq = 0.0;
     for (int64_t b = 0; b < B; ++b) {
         double ss, se, a = 0.0;
         tl_totals(part, b, ss, se);
@@ -431,10 +436,13 @@ __global__ __launch_bounds__(64) void k_tl_accumulate(const double* __restrict__
             a += part[(b * kTLChunks + c) * 4 + 2];
             q += part[(b * kTLChunks + c) * 4 + 3];
         }
-        kl += (double)sc * a / se + log(ss) - log(se);
+        const double ls = log(ss) - log(se);
+        kl += (double)sc * a / se + ls;
+        sq += ls * ls;
     }
     acc[0] += q / (double)(B * R * S);
     acc[1] += kl / (double)B;
+    acc[2] += sq / (double)B;
 }
 
 // acc[0] += (sum of the nb partials, in a fixed order) / denom
@@ -816,7 +824,7 @@ int sfh_nn_mse(sfh_ctx* ctx, const float* out, const float* target, const int64_
 
 int sfh_nn_template_loss(sfh_ctx* ctx, const float* out, const float* target, const int64_t* rows, int64_t B,
                          int64_t H2, int64_t W2, int64_t R, int64_t S, float scale, float offset, float w_mse,
-                         float w_kl, float* dout, double* acc)
+                         float w_kl, float w_sum, float* dout, double* acc)
 {
     SFH_NN_BEGIN(ctx);
     SFH_ARG_CHECK(ctx, out && target && dout && B >= 1 && B <= 65535 && R >= 1 && S >= 1 && R <= H2 && S <= W2 &&
@@ -828,7 +836,7 @@ int sfh_nn_template_loss(sfh_ctx* ctx, const float* out, const float* target, co
     hipLaunchKernelGGL(k_tl_sums, dim3(kTLChunks, (unsigned)B), dim3(kWG), 0, ctx->stream, out, target, rows, H2, W2,
                        R, S, scale, offset, part);
     hipLaunchKernelGGL(k_tl_grad, dim3((unsigned)std::min<int64_t>(cdiv(H2 * W2, kWG), 64), (unsigned)B), dim3(kWG),
-                       0, ctx->stream, out, target, rows, B, H2, W2, R, S, scale, offset, w_mse, w_kl,
+                       0, ctx->stream, out, target, rows, B, H2, W2, R, S, scale, offset, w_mse, w_kl, w_sum,
                        (const double*)part, dout);
     if (acc)
         hipLaunchKernelGGL(k_tl_accumulate, dim3(1), dim3(64), 0, ctx->stream, (const double*)part, B, R, S, scale,

@@ -335,15 +335,17 @@ class TemplateSurrogate:
         return out
 
     def train(self, params, images, steps, batch=32, lr=1e-3, lr_min=1e-5, warmup=200, seed=0, log_every=250,
-              weight_decay=0.0, kl_weight=0.0):
+              weight_decay=0.0, kl_weight=1.0, sum_weight=0.0):
         """Adam on shuffled mini-batches of (params: n x 4 physical, images:
         n x 200 x 300 device tensor from ``images``) for ``steps`` steps; the
         learning rate rises linearly over ``warmup`` steps, then follows a
         cosine from lr to lr_min.  The loss is the MSE of the log image plus
         kl_weight times the mean KL(exact || network) per star (the
-        log-likelihood a fit loses per star).  Runs without host
+        log-likelihood a fit loses per star) plus sum_weight times the mean
+        squared error of the log template sums.  Runs without host
         synchronisation except every ``log_every`` steps, when the means since
-        the last report are read.  Returns [(step, MSE, KL per star), ...]."""
+        the last report are read.  Returns [(step, MSE, KL per star, squared
+        log-sum error), ...]."""
         torch = self.torch
         n = images.shape[0]
         if images.shape[1:] != (CROP_H, CROP_W) or len(params) != n:
@@ -355,7 +357,7 @@ class TemplateSurrogate:
         nep = -(-steps // per_epoch)
         rows = np.concatenate([rng.permutation(n)[:per_epoch * batch] for _ in range(nep)])
         rows_d = torch.from_numpy(rows.astype(np.int64)).to(self.dev)
-        acc = torch.zeros(2, dtype=torch.float64, device=self.dev)
+        acc = torch.zeros(3, dtype=torch.float64, device=self.dev)
         self.x0f.zero_()
         torch.cuda.synchronize()
         hist, last = [], 0
@@ -367,8 +369,8 @@ class TemplateSurrogate:
             self._forward_device(batch)
             self._c(self.lib.sfh_nn_template_loss(self.ctx.handle, self.out.data_ptr(), images.data_ptr(),
                                                   r.data_ptr(), batch, OUT_H, OUT_W, CROP_H, CROP_W, LOG_SCALE,
-                                                  LOG_OFFSET, 1.0, float(kl_weight), self.dout.data_ptr(),
-                                                  acc.data_ptr()))
+                                                  LOG_OFFSET, 1.0, float(kl_weight), float(sum_weight),
+                                                  self.dout.data_ptr(), acc.data_ptr()))
             self._backward_from_dout()
             if k < warmup:
                 rate = lr * (k + 1) / warmup
@@ -379,7 +381,7 @@ class TemplateSurrogate:
             if (k + 1) % log_every == 0 or k + 1 == steps:
                 self.ctx.synchronize()
                 a = acc.cpu().numpy() / (k + 1 - last)
-                hist.append((k + 1, float(a[0]), float(a[1])))
+                hist.append((k + 1, float(a[0]), float(a[1]), float(a[2])))
                 acc.zero_()
                 torch.cuda.synchronize()
                 last = k + 1

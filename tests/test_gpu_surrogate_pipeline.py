@@ -172,40 +172,37 @@ def test_train_lowers_loss_on_exact_templates(sfh, gpu_ctx):
 
 
 def _tl_ref(y, t, sc, off):
-    """FP64 restatement: MSE and KL(p_e || p_s) of one image pair."""
+    """FP64 restatement: MSE, KL(p_e || p_s) and squared log-sum error of one image pair."""
     sv, ev = np.exp(np.minimum(y, 15.0) * sc + off), np.exp(t * sc + off)
     pe, ps = ev / ev.sum(), sv / sv.sum()
-    return np.mean((y - t) ** 2), np.sum(pe * (np.log(pe) - np.log(ps)))
+    return (np.mean((y - t) ** 2), np.sum(pe * (np.log(pe) - np.log(ps))),
+            (np.log(sv.sum()) - np.log(ev.sum())) ** 2)
 
 
 def test_template_loss_against_restatement_and_differences(sfh, gpu_ctx):
     import torch
     rng = np.random.default_rng(11)
     B, H2, W2, R, S_ = 2, 14, 12, 11, 9
-    sc, off, wm, wk = 5.0, -20.0, 0.7, 0.3
+    sc, off, wm, wk, ws = 5.0, -20.0, 0.7, 0.3, 0.45
     out = rng.normal(0.0, 0.8, (B, H2, W2)).astype(np.float32)
     tgt = rng.normal(0.0, 0.8, (3, R, S_)).astype(np.float32)
     idx = np.array([2, 0], dtype=np.int64)
     o_d, t_d, i_d = torch.from_numpy(out).cuda(), torch.from_numpy(tgt).cuda(), torch.from_numpy(idx).cuda()
     dout = torch.empty_like(o_d)
-    acc = torch.zeros(2, dtype=torch.float64, device="cuda")
+    acc = torch.zeros(3, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
     gpu_ctx.check(gpu_ctx.lib.sfh_nn_template_loss(gpu_ctx.handle, o_d.data_ptr(), t_d.data_ptr(), i_d.data_ptr(), B,
-                                                   H2, W2, R, S_, sc, off, wm, wk, dout.data_ptr(), acc.data_ptr()))
+                                                   H2, W2, R, S_, sc, off, wm, wk, ws, dout.data_ptr(),
+                                                   acc.data_ptr()))
     gpu_ctx.synchronize()
     y64, t64 = out.astype(np.float64), tgt.astype(np.float64)
 
     def total(yy):
-        m = k = 0.0
-        for b in range(B):
-            mb, kb = _tl_ref(yy[b, :R, :S_], t64[idx[b]], sc, off)
-            m += mb / B
-            k += kb / B
-        return m, k
+        return np.mean([_tl_ref(yy[b, :R, :S_], t64[idx[b]], sc, off) for b in range(B)], axis=0)
 
-    mse, kl = total(y64)
+    mse, kl, sq = total(y64)
     a = acc.cpu().numpy()
-    assert abs(a[0] - mse) <= 1e-6 * mse and abs(a[1] - kl) <= 1e-5 * kl
+    assert abs(a[0] - mse) <= 1e-6 * mse and abs(a[1] - kl) <= 1e-5 * kl and abs(a[2] - sq) <= 1e-5 * sq
     # the gradient of wm MSE + wk KL by central differences of the FP64 restatement
     g = dout.cpu().numpy()
     assert np.all(g[:, R:, :] == 0.0) and np.all(g[:, :, S_:] == 0.0)
@@ -214,7 +211,7 @@ def test_template_loss_against_restatement_and_differences(sfh, gpu_ctx):
         yp, ym = y64.copy(), y64.copy()
         yp[b, yy, xx] += h
         ym[b, yy, xx] -= h
-        fp = np.dot([wm, wk], total(yp))
-        fm = np.dot([wm, wk], total(ym))
+        fp = np.dot([wm, wk, ws], total(yp))
+        fm = np.dot([wm, wk, ws], total(ym))
         num = (fp - fm) / (2 * h)
         assert abs(g[b, yy, xx] - num) <= 1e-4 * abs(num) + 1e-7
