@@ -299,11 +299,12 @@ def surrogate_bench(sfh, torch, ctx, O, args, ncpu):
     mh = np.linspace(-2.5, 0.3, 34)
     net.predict(np.array([[8.0, -1.0, 27.0, 1.0]]))        # warm-up of the inference shapes
     for nstars in (args.surrogate_stars, 10.0 * args.surrogate_stars):
-        out[f"fill_{nstars:g}_stars"] = _fill_bench(S, torch, ctx, O, args, ncpu, net, la, mh, nstars)
+        out[f"fill_{nstars:g}_stars"] = _fill_bench(S, torch, ctx, O, args, ncpu, net, la, mh, nstars,
+                                                    20000 / gen_s)
     return out
 
 
-def _fill_bench(S, torch, ctx, O, args, ncpu, net, la, mh, nstars):
+def _fill_bench(S, torch, ctx, O, args, ncpu, net, la, mh, nstars, exact_rate):
     from sfh_amd import synth
     out = {}
     fill = S.fill_templates(net, la, mh, 27.0, 1.0, n_stars=nstars, check_stride=(5, 4))
@@ -338,7 +339,7 @@ def _fill_bench(S, torch, ctx, O, args, ncpu, net, la, mh, nstars):
             f"q50/q90/q99/max {out['fill']['audit']['dlnL_q50_q90_q99_max']}, "
             f"{out['fill']['audit']['frac_within_0.5']:.3f} within 0.5")
     else:
-        exact_gpu_rate = 20000 / gen_s
+        exact_gpu_rate = exact_rate
     out["fill"]["exact_gpu_seconds_projected"] = n / exact_gpu_rate
     out["fill"]["speedup_vs_exact_gpu"] = n / exact_gpu_rate / fill.seconds["total"]
     if not args.no_cpu and nstars == args.surrogate_stars:

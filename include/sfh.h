@@ -297,6 +297,74 @@ int sfh_bin_cmd_smooth_batch_device(sfh_ctx* ctx, const double* d_colors, const 
                                     double cov_mult, sfh_range xedges, sfh_range yedges,
                                     float* d_out, int64_t ld_out, int64_t* pairs_out);
 
+/* ------------------------------------------------- mock observations */
+
+#define SFH_MOCK_MAX_FILTERS 8
+#define SFH_MOCK_MAX_IMF_SEGMENTS 8
+
+/* A piecewise power-law IMF: pdf(m) proportional to coefs[k] m^-slopes[k] on
+ * [breaks[k], breaks[k+1]), k < nseg (Salpeter1955: one segment; Kroupa2001:
+ * two, slopes 1.3 / 2.3, coefficients 1 / 0.5 around 0.5 Msun). */
+typedef struct sfh_imf {
+    int32_t nseg;
+    const double* breaks;   /* nseg + 1, increasing */
+    const double* slopes;   /* nseg */
+    const double* coefs;    /* nseg */
+} sfh_imf;
+
+/* kind 0 NoBinaries, 1 RandomBinaryPairs(fraction), 2 BinaryMassRatio(fraction,
+ * Uniform(qmin, qmax)) -- src/simulate.jl:153-259. */
+typedef struct sfh_binary_model {
+    int32_t kind;
+    double fraction, qmin, qmax;
+} sfh_binary_model;
+
+/* A device-resident star catalogue: n entries of nstars masses and nfilters
+ * magnitudes, with the index of the system (or input star) each came from. */
+typedef struct sfh_catalog sfh_catalog;
+
+/* generate_stars_mass (mode 0) / generate_stars_mag (mode 1) --
+ * src/simulate.jl:306-339 / :350-404.  The isochrone arrives as the reference
+ * prepares it before sampling (sort_ingested :54-63, distance modulus added):
+ * mini ascending without duplicates, mags npoints x nfilters row-major.  Mode
+ * 0 draws systems until their birth masses reach `limit` and returns those
+ * whose primary mass is >= mmin_cut (mass_limits :92-123, host side) with
+ * positive flux; mode 1 draws until the flux in limit_filter reaches `limit`
+ * (mag2flux of the apparent limit magnitude) and returns systems with positive
+ * flux that are brighter than mag_lim in mag_lim_filter (mag_lim = +inf: all).
+ * System i draws from Philox4x64-10(counter (i, 0), key (seed, stream)); the
+ * running total is kept in 64-bit fixed point (2^62 / limit), so the sample
+ * ends at the first system that brings it to the limit whatever the summation
+ * order.  *out receives a new catalogue (system order). */
+int sfh_generate_stars(sfh_ctx* ctx, int64_t npoints, const double* mini, int32_t nfilters, const double* mags,
+                       int32_t mode, double limit, int32_t limit_filter, double mmin_cut, double mag_lim,
+                       int32_t mag_lim_filter, const sfh_imf* imf, const sfh_binary_model* binary, uint64_t seed,
+                       uint64_t stream, sfh_catalog** out);
+/* A catalogue of nfilters magnitudes per star from host memory (row-major). */
+int sfh_catalog_from_host(sfh_ctx* ctx, int64_t n, int32_t nfilters, const double* mags, sfh_catalog** out);
+int sfh_catalog_shape(const sfh_catalog* c, int64_t* n, int32_t* nstars, int32_t* nfilters, int64_t* sampled);
+/* Copies masses (n x nstars), mags (n x nfilters) and source indices (n) to
+ * host memory; any pointer may be NULL. */
+int sfh_catalog_copy(const sfh_catalog* c, double* masses, double* mags, int64_t* index);
+int sfh_catalog_destroy(sfh_catalog* c);
+/* model_cmd(mags, errfuncs, completefuncs, biasfuncs) -- src/simulate.jl:621-674.
+ * Star j is kept when a uniform number (Philox block (j, 0), key (seed,
+ * stream), word 0) is <= the product of the filters' completeness; kept
+ * magnitudes become mag + z err(mag) + bias(mag), z standard normal (Box-
+ * Muller).  err, completeness (and bias, may be NULL) are nfilters tables
+ * each, interpolated linearly.  *out: the observed catalogue; its indices are
+ * the kept input stars. */
+int sfh_model_cmd(sfh_ctx* ctx, const sfh_catalog* in, const sfh_table* err, const sfh_table* completeness,
+                  const sfh_table* bias, uint64_t seed, uint64_t stream, sfh_catalog** out);
+/* bin_cmd(colors, mags; weights, edges) -- src/StarFormationHistories.jl:544-553:
+ * the left-closed weighted 2-D histogram (nx x ny, colour fastest), from host
+ * arrays (weights may be NULL: unit weights, exact counts) or from the
+ * catalogue's colour (filter a - filter b) and y filter. */
+int sfh_bin_cmd(sfh_ctx* ctx, int64_t n, const double* colors, const double* mags, const double* weights,
+                sfh_range xedges, sfh_range yedges, double* out);
+int sfh_bin_cmd_catalog(sfh_ctx* ctx, const sfh_catalog* c, int32_t color_a, int32_t color_b, int32_t y_filter,
+                        sfh_range xedges, sfh_range yedges, double* out);
+
 /* ------------------------------- learned template surrogate primitives (new) */
 
 /* No reference counterpart: the device operations the template surrogate

@@ -11,7 +11,7 @@ from . import _lib
 
 _lib.load()
 
-from ._lib import ArgumentError, DeviceError, InexactError  # noqa: E402
+from ._lib import ArgumentError, DeviceError, DomainError, InexactError  # noqa: E402
 from .core import (Context, HIPData, HIPTemplates, composite_, default_context, fg_,  # noqa: E402
                    fg_batched, fg_batched_device, fg_device, grad_loglikelihood,
                    grad_loglikelihood_, loglikelihood, stack_models)
@@ -20,6 +20,11 @@ from .fitting import (HMCModel, LogTransformFTResult, MCMCModel, construct_x0,  
                       renormalize_x0)
 from .multifit import MultiFit, StageInfo  # noqa: E402
 from . import hierarchical  # noqa: E402
+from . import mock  # noqa: E402
+from .mock import (BinaryMassRatio, PowerLawIMF, Salpeter1955, bin_cmd, generate_stars_mag,  # noqa: E402
+                   generate_stars_mag_composite, generate_stars_mass, generate_stars_mass_composite, ingest_mags,
+                   mass_limits, model_cmd, recycling_fraction, recycling_mass_fraction, sort_ingested,
+                   surviving_fraction, surviving_mass_fraction)
 from .templates import (Histogram, NoBinaries, PointBatch, RandomBinaryPairs, Range,  # noqa: E402
                         binary_hess, binary_mass_fraction, binary_number_fraction, binary_pairs_hist,
                         binary_system_fraction, bin_cmd_smooth, calculate_edges, flux2mag, mag2flux, trapz,
@@ -39,5 +44,9 @@ __all__ = [
     "MCMCModel", "construct_x0", "fit_templates", "fit_templates_fast", "fit_templates_lbfgsb",
     "mcmc_sample", "renormalize_x0", "MultiFit", "StageInfo", "NoBinaries", "RandomBinaryPairs",
     "binary_hess", "binary_mass_fraction", "binary_number_fraction", "binary_pairs_hist",
-    "binary_system_fraction", "flux2mag", "mag2flux", "trapz", "hierarchical",
+    "binary_system_fraction", "flux2mag", "mag2flux", "trapz", "hierarchical", "DomainError", "mock",
+    "BinaryMassRatio", "PowerLawIMF", "Salpeter1955", "bin_cmd", "generate_stars_mag",
+    "generate_stars_mag_composite", "generate_stars_mass", "generate_stars_mass_composite", "ingest_mags",
+    "mass_limits", "model_cmd", "recycling_fraction", "recycling_mass_fraction", "sort_ingested",
+    "surviving_fraction", "surviving_mass_fraction",
 ]

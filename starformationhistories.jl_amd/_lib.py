@@ -32,6 +32,10 @@ class InexactError(ValueError):
     """Input outside the reference's domain (e.g. round(Int, NaN))."""
 
 
+class DomainError(ValueError):
+    """An argument outside the function's domain (the reference's DomainError)."""
+
+
 class DeviceError(RuntimeError):
     """HIP runtime, launch or allocation failure."""
 
@@ -57,6 +61,16 @@ class sfh_fit_results(C.Structure):
 
 SFH_FIT_LOG = 0
 SFH_FIT_SQRT = 1
+class sfh_imf(C.Structure):
+    _fields_ = [("nseg", C.c_int32), ("breaks", C.c_void_p), ("slopes", C.c_void_p), ("coefs", C.c_void_p)]
+
+
+class sfh_binary_model(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("fraction", C.c_double), ("qmin", C.c_double), ("qmax", C.c_double)]
+
+
+MOCK_MAX_FILTERS = 8
+
 SFH_OPT_BFGS = 0
 SFH_OPT_LBFGS = 1
 FIT_STATUS = {0: "running", 1: "converged", 2: "max_iterations", 3: "linesearch_failed"}
@@ -109,6 +123,17 @@ SIGNATURES = [
      [_vp, _i64, _vp, _vp, _vp, _vp, _int, _dbl, C.POINTER(sfh_table), C.POINTER(sfh_table),
       C.POINTER(sfh_table), C.POINTER(sfh_table), sfh_range, sfh_range, _vp]),
     ("sfh_bin_cmd_smooth", _int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _int, sfh_range, sfh_range, _vp]),
+    ("sfh_generate_stars", _int, [_vp, _i64, _vp, C.c_int32, _vp, C.c_int32, _dbl, C.c_int32, _dbl, _dbl, C.c_int32,
+                                  C.POINTER(sfh_imf), C.POINTER(sfh_binary_model), C.c_uint64, C.c_uint64,
+                                  C.POINTER(_vp)]),
+    ("sfh_catalog_from_host", _int, [_vp, _i64, C.c_int32, _vp, C.POINTER(_vp)]),
+    ("sfh_catalog_shape", _int, [_vp, _p64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _p64]),
+    ("sfh_catalog_copy", _int, [_vp, _vp, _vp, _vp]),
+    ("sfh_catalog_destroy", _int, [_vp]),
+    ("sfh_model_cmd", _int, [_vp, _vp, C.POINTER(sfh_table), C.POINTER(sfh_table), C.POINTER(sfh_table),
+                             C.c_uint64, C.c_uint64, C.POINTER(_vp)]),
+    ("sfh_bin_cmd", _int, [_vp, _i64, _vp, _vp, _vp, sfh_range, sfh_range, _vp]),
+    ("sfh_bin_cmd_catalog", _int, [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, sfh_range, sfh_range, _vp]),
     ("sfh_nn_gemm_nt", _int, [_vp, _int, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _i64, C.c_float, C.c_float]),
     ("sfh_nn_bias_act", _int, [_vp, _int, _vp, _i64, _vp, _i64, _i64, _int, _vp, _vp, _i64]),
     ("sfh_nn_act_bwd", _int, [_vp, _vp, _vp, _i64, _int, _vp]),
