@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--surrogate-stars", type=float, default=100.0,
                     help="stars per template in the dlnL check of the filled grid")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--mock-mass", type=float, default=1e7,
+                    help="birth mass (Msun) of the mock population sampled, observed and binned on the GPU")
+    ap.add_argument("--no-mock", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates HBM counters)")
     return ap.parse_args()
@@ -304,6 +307,68 @@ def surrogate_bench(sfh, torch, ctx, O, args, ncpu):
     return out
 
 
+def mock_bench(sfh, ctx, args):
+    """The mock-observation path: a population of one config-1 isochrone
+    (logAge 9, [M/H] -1, 2000 points) with a Kroupa IMF and random binary
+    pairs (fraction 0.3) sampled to a birth mass on the GPU, observed with
+    the config-1 error and completeness model, and binned on the 200 x 300
+    grid; kernel times from HIP events.  The NumPy restatement of the same
+    sampling rule times the host on a bounded sample (one thread)."""
+    import importlib
+    from sfh_amd import mock, synth
+    m, (B, V) = synth.isochrone(9.0, -1.0, 2000, 7, mmax_imf=120.0)
+    g = np.stack([B, V], axis=1)
+    imf = mock.Kroupa2001(0.08, 120.0)
+    edges = synth.cfg1_edges()
+    limit = args.mock_mass
+    kw = dict(dist_mod=24.0, binary_model=sfh.RandomBinaryPairs(0.3), catalog=True, ctx=ctx)
+    sfh.generate_stars_mass(m, g, ["B", "V"], 1e4, imf, seed=1, **kw).close()          # warm-up
+    ctx.set_timing(True)
+    ctx.reset_stats()
+    t0 = time.time()
+    cat = sfh.generate_stars_mass(m, g, ["B", "V"], limit, imf, seed=2, **kw)
+    gen_s = time.time() - t0
+    t0 = time.time()
+    obs = sfh.model_cmd(cat, [synth.photerr, synth.photerr], [synth.completeness, synth.completeness], seed=3)
+    obs_s = time.time() - t0
+    t0 = time.time()
+    H = mock.bin_cmd_catalog(obs, (0, 1), 1, edges).weights
+    bin_s = time.time() - t0
+    gk, gn = ctx.kernel_stats("mock_generate")
+    ok, _ = ctx.kernel_stats("mock_observe")
+    bk, _ = ctx.kernel_stats("mock_bin")
+    ctx.set_timing(False)
+    drawn, kept, observed = cat.sampled, cat.n, obs.n
+    # algorithmic bytes of the sampling kernel per system: 8 (contribution) +
+    # 2 x 8 (masses) + 2 x 8 (magnitudes) + 1 (kept flag) written
+    gen_bytes = drawn * 41.0
+    out = {"birth_mass": limit, "systems_drawn": drawn, "systems_kept": kept, "stars_observed": observed,
+           "binned": float(H.sum()), "generate_kernel_ms": gk, "generate_launches": gn,
+           "generate_systems_per_s_kernel": drawn / (gk * 1e-3), "generate_wall_s": gen_s,
+           "generate_systems_per_s_wall": drawn / gen_s, "generate_GBps_algorithmic": gen_bytes / (gk * 1e-3) / 1e9,
+           "observe_kernel_ms": ok, "observe_stars_per_s_kernel": kept / (ok * 1e-3), "observe_wall_s": obs_s,
+           "bin_kernel_ms": bk, "bin_stars_per_s_kernel": observed / (bk * 1e-3), "bin_wall_s": bin_s}
+    log(f"mock: {limit:.3g} Msun -> {drawn} systems drawn ({gn} batches), {kept} kept, {observed} observed; "
+        f"generate {gk:.2f} ms kernel ({drawn / (gk * 1e-3):.3g} systems/s), {gen_s:.2f} s wall; "
+        f"model_cmd {ok:.2f} ms; bin_cmd {bk:.2f} ms")
+    cat.close()
+    obs.close()
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle"))
+        R = importlib.import_module("mock_ref")
+        gi = g + 24.0
+        oimf = R.PowerLaw(imf.breaks, imf.slopes, imf.coefs)
+        small = max(1e3, min(limit, 2e5))
+        t0 = time.time()
+        _, _, _, nd = R.generate(m, gi, 0, small, oimf, 1, 0.3, seed=2, mmin_cut=m[0])
+        dt = time.time() - t0
+        out["cpu_numpy_systems_per_s"] = nd / dt
+        out["cpu_sample"] = f"{small:.3g} Msun ({nd} systems) with the NumPy restatement, one thread, {dt:.2f} s"
+        out["gpu_over_cpu_kernel"] = out["generate_systems_per_s_kernel"] / (nd / dt)
+        log(f"mock host (NumPy): {nd / dt:.3g} systems/s; GPU kernel / host = {out['gpu_over_cpu_kernel']:.0f}x")
+    return out
+
+
 def _fill_bench(S, torch, ctx, O, args, ncpu, net, la, mh, nstars, exact_rate):
     from sfh_amd import synth
     out = {}
@@ -545,6 +610,8 @@ def main():
         extras["multifit"] = multifit_bench(sfh, torch, ctx, O, args, ncpu)
     if not args.no_extras and not args.no_surrogate:
         extras["surrogate"] = surrogate_bench(sfh, torch, ctx, O, args, ncpu)
+    if not args.no_extras and not args.no_mock:
+        extras["mock"] = mock_bench(sfh, ctx, args)
 
     line = {
         "metric": "loglike+grad evals/s (700 tpl×60k bins) batched vs plain; templates/s; HBM GB/s",

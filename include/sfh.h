@@ -346,6 +346,9 @@ int sfh_catalog_shape(const sfh_catalog* c, int64_t* n, int32_t* nstars, int32_t
 /* Copies masses (n x nstars), mags (n x nfilters) and source indices (n) to
  * host memory; any pointer may be NULL. */
 int sfh_catalog_copy(const sfh_catalog* c, double* masses, double* mags, int64_t* index);
+/* Per-filter minimum and maximum of the catalogue's finite magnitudes (lo,
+ * hi: nfilters each; +inf / -inf when there are none). */
+int sfh_catalog_range(const sfh_catalog* c, double* lo, double* hi);
 int sfh_catalog_destroy(sfh_catalog* c);
 /* model_cmd(mags, errfuncs, completefuncs, biasfuncs) -- src/simulate.jl:621-674.
  * Star j is kept when a uniform number (Philox block (j, 0), key (seed,

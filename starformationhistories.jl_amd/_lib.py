@@ -129,6 +129,7 @@ SIGNATURES = [
     ("sfh_catalog_from_host", _int, [_vp, _i64, C.c_int32, _vp, C.POINTER(_vp)]),
     ("sfh_catalog_shape", _int, [_vp, _p64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _p64]),
     ("sfh_catalog_copy", _int, [_vp, _vp, _vp, _vp]),
+    ("sfh_catalog_range", _int, [_vp, _vp, _vp]),
     ("sfh_catalog_destroy", _int, [_vp]),
     ("sfh_model_cmd", _int, [_vp, _vp, C.POINTER(sfh_table), C.POINTER(sfh_table), C.POINTER(sfh_table),
                              C.c_uint64, C.c_uint64, C.POINTER(_vp)]),

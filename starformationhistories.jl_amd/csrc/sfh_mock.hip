@@ -310,6 +310,38 @@ __global__ __launch_bounds__(kWG) void k_scatter(const unsigned char* __restrict
     oindex[o] = index0 + t;
 }
 
+// per-workgroup minimum and maximum of each filter's finite magnitudes
+__global__ __launch_bounds__(kWG) void k_mag_range(const double* __restrict__ mags, int64_t n, int nf,
+                                                   double* __restrict__ part)
+{
+    __shared__ double lo_s[kWG], hi_s[kWG];
+    for (int f = 0; f < nf; ++f) {
+        double lo = INFINITY, hi = -INFINITY;
+        for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) {
+            const double v = mags[i * nf + f];
+            if (isfinite(v)) {
+                lo = fmin(lo, v);
+                hi = fmax(hi, v);
+            }
+        }
+        lo_s[threadIdx.x] = lo;
+        hi_s[threadIdx.x] = hi;
+        __syncthreads();
+        for (int w = kWG / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                lo_s[threadIdx.x] = fmin(lo_s[threadIdx.x], lo_s[threadIdx.x + w]);
+                hi_s[threadIdx.x] = fmax(hi_s[threadIdx.x], hi_s[threadIdx.x + w]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            part[((int64_t)blockIdx.x * nf + f) * 2] = lo_s[0];
+            part[((int64_t)blockIdx.x * nf + f) * 2 + 1] = hi_s[0];
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------- binning
 // Left-closed bins: bin = floor((v - first) / step), in range when
 // 0 <= bin < n.  Weights in 64-bit fixed point (scale) or unit counts.
@@ -643,6 +675,36 @@ int sfh_catalog_copy(const sfh_catalog* c, double* masses, double* mags, int64_t
         SFH_HIP_TRY(ctx, hipMemcpy(mags, c->d_mags, (size_t)c->n * c->nfilters * sizeof(double),
                                    hipMemcpyDeviceToHost));
     if (index) SFH_HIP_TRY(ctx, hipMemcpy(index, c->d_index, (size_t)c->n * sizeof(int64_t), hipMemcpyDeviceToHost));
+    return SFH_OK;
+}
+
+int sfh_catalog_range(const sfh_catalog* c, double* lo, double* hi)
+{
+    if (!c || !lo || !hi) return SFH_ERR_ARGUMENT;
+    sfh_ctx* ctx = c->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (int rc = bind(ctx)) return rc;
+    const int nf = c->nfilters;
+    for (int f = 0; f < nf; ++f) {
+        lo[f] = INFINITY;
+        hi[f] = -INFINITY;
+    }
+    if (c->n == 0) return SFH_OK;
+    const int64_t nb = std::min<int64_t>(cdiv(c->n, kWG), 1024);
+    void* p = nullptr;
+    if (int rc = scratch(ctx, "mock_range", (size_t)nb * nf * 2 * sizeof(double), &p)) return rc;
+    hipLaunchKernelGGL(k_mag_range, dim3((unsigned)nb), dim3(kWG), 0, ctx->stream, (const double*)c->d_mags, c->n,
+                       nf, (double*)p);
+    if (int rc = check_launch(ctx, "k_mag_range")) return rc;
+    std::vector<double> part((size_t)nb * nf * 2);
+    SFH_HIP_TRY(ctx, hipMemcpyAsync(part.data(), p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    SFH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t b = 0; b < nb; ++b)
+        for (int f = 0; f < nf; ++f) {
+            lo[f] = std::min(lo[f], part[(b * nf + f) * 2]);
+            hi[f] = std::max(hi[f], part[(b * nf + f) * 2 + 1]);
+        }
     return SFH_OK;
 }
 

@@ -273,6 +273,12 @@ class Catalog:
                                                      g.ctypes.data, ix.ctypes.data))
         return m, g, ix
 
+    def mag_range(self):
+        """Per-filter (min, max) of the finite magnitudes, reduced on the device."""
+        lo, hi = np.empty(self.nfilters), np.empty(self.nfilters)
+        self.ctx.check(self.ctx.lib.sfh_catalog_range(self.handle, lo.ctypes.data, hi.ctypes.data))
+        return lo, hi
+
     def close(self):
         if getattr(self, "handle", None):
             self.ctx.lib.sfh_catalog_destroy(self.handle)
@@ -416,7 +422,6 @@ def model_cmd(mags, errfuncs, completefuncs, biasfuncs=None, *, seed=0, stream=0
     ctx = ctx or (mags.ctx if isinstance(mags, Catalog) else default_context())
     if isinstance(mags, Catalog):
         cat, own = mags, False
-        host = cat.arrays()[1]
     else:
         host = np.ascontiguousarray(mags, dtype=np.float64)
         if host.ndim != 2:
@@ -424,12 +429,13 @@ def model_cmd(mags, errfuncs, completefuncs, biasfuncs=None, *, seed=0, stream=0
         cat, own = Catalog.from_mags(host, ctx), True
     nf = cat.nfilters
     if not (nf == len(errfuncs) == len(completefuncs)) or (biasfuncs is not None and len(biasfuncs) != nf):
+        if own:
+            cat.close()
         raise ArgumentError("Arguments to `StarFormationHistories.model_cmd` must satisfy "
                             "`length(first(mags)) == length(errfuncs) == length(completefuncs)`.")
-    if host.shape[0]:
-        lo, hi = host.min(axis=0), host.max(axis=0)
-    else:
-        lo, hi = np.zeros(nf), np.ones(nf)
+    lo, hi = cat.mag_range()
+    empty = ~(np.isfinite(lo) & np.isfinite(hi))
+    lo, hi = np.where(empty, 0.0, lo), np.where(empty, 1.0, hi)
     et, k1 = _tables(errfuncs, lo, hi)
     ct, k2 = _tables(completefuncs, lo, hi)
     bt, k3 = _tables(biasfuncs, lo, hi) if biasfuncs is not None else (None, [])

@@ -163,6 +163,11 @@ def test_model_cmd_matches_oracle(sfh, gpu_ctx):
     assert abs(np.mean(e[:, 1])) < 5 * np.mean(0.5 * synth.photerr(mags[idx, 1])) / math.sqrt(idx.size) + 1e-4
     with pytest.raises(sfh.ArgumentError):
         sfh.model_cmd(mags, errs[:1], comps)
+    # the tables' magnitude range comes from the device and skips non-finite entries
+    cat = mock.Catalog.from_mags(np.array([[20.0, np.inf], [22.0, 19.0], [21.0, 18.5]]))
+    lo, hi = cat.mag_range()
+    assert lo.tolist() == [20.0, 18.5] and hi.tolist() == [22.0, 19.0]
+    cat.close()
 
 
 def test_bin_cmd_matches_oracle(sfh, gpu_ctx):
