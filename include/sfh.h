@@ -395,12 +395,16 @@ int sfh_nn_transpose(sfh_ctx* ctx, int dtype, const float* in, int64_t ldi, int6
 int sfh_nn_colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols, float beta, float* out);
 /* Stride-2 4x4 transposed convolution (padding 1), the two gathers around
  * its matrix products: col2im sums the 4 taps landing on each of the
- * B x 2H x 2W x Co outputs from cols (B*H*W rows, column co*16+ky*4+kx) and
+ * B x 2H x 2W x Co outputs from cols (B*H*W rows, column (ky*4+kx)*Co+co) and
  * adds bias; im2col builds those rows from an output gradient. */
 int sfh_nn_col2im(sfh_ctx* ctx, const float* cols, int64_t ldc, int64_t B, int64_t H, int64_t W, int64_t Co,
                   const float* bias, float* out);
 int sfh_nn_im2col(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t H, int64_t W, int64_t Co,
                   void* dcols, int64_t ldd);
+/* The transpose of im2col, as the weight gradient's operand: dcolsT (16 Co
+ * rows of ldo >= B H W, dtype, zero past B H W) with Co in {4, 8, 16, 32}. */
+int sfh_nn_im2col_t(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t H, int64_t W, int64_t Co,
+                    void* dcolsT, int64_t ldo);
 /* The same transposed convolution with one output channel (B x 2H x 2W) and a
  * per-pixel bias map; its backward gives din, dw (16 x Ci) and dbmap (any may be NULL). */
 int sfh_nn_conv1_fwd(sfh_ctx* ctx, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci, const float* w,

@@ -47,8 +47,8 @@ def dsilu(x):
 
 
 def _kernel(K, co):
-    """(co*16 + ky*4 + kx, ci) rows -> (co, 4, 4, ci)."""
-    return K.reshape(co, 4, 4, -1)
+    """((ky*4 + kx)*co + o, ci) rows (tap-major) -> (co, 4, 4, ci)."""
+    return K.reshape(4, 4, co, -1).transpose(2, 0, 1, 3)
 
 
 def convT(x, K, co):
@@ -75,7 +75,7 @@ def convT_bwd(x, K, co, dout):
             g = dfull[:, ky:ky + 2 * H:2, kx:kx + 2 * W:2, :]          # (B, H, W, co)
             dx += np.einsum("bhwo,oc->bhwc", g, k[:, ky, kx, :])
             dk[:, ky, kx, :] = np.einsum("bhwo,bhwc->oc", g, x)
-    return dx, dk.reshape(co * 16, ci)
+    return dx, dk.transpose(1, 2, 0, 3).reshape(16 * co, ci)
 
 
 def scale_params(p):

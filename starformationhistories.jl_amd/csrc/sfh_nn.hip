@@ -8,11 +8,13 @@
 // operands (used to check gradients), FP32 accumulation either way.  Callers
 // keep K a multiple of 16 with zero padding.  The stride-2, 4x4 transposed
 // convolutions of the decoder become such products plus gathers:
-//   forward   cols[p, (co,ky,kx)] = in[p, :] . W[(co,ky,kx), :]    (GEMM)
+//   forward   cols[p, (ky,kx,co)] = in[p, :] . W[(ky,kx,co), :]    (GEMM)
 //             out[b,Y,X,co] = sum of the 4 taps (ky,kx) that land on (Y,X) (col2im)
-//   backward  dcols[p, (co,ky,kx)] = dout[b, 2y-1+ky, 2x-1+kx, co]    (im2col)
+//   backward  dcols[p, (ky,kx,co)] = dout[b, 2y-1+ky, 2x-1+kx, co]    (im2col)
 //             din = dcols W,  dW = dcols^T in                          (GEMMs)
-// Activations are channels-last (pixel-major, channel fastest).
+// Activations are channels-last (pixel-major, channel fastest) and weight
+// rows tap-major ((ky*4 + kx)*Co + co), so both gathers move runs of Co
+// consecutive floats.
 #include "sfh_internal.h"
 
 #include <cmath>
@@ -85,14 +87,19 @@ __device__ inline f32x16 mma(const Frag<float>& a, const Frag<float>& b, f32x16 
     return c;
 }
 
+// Workgroup tile (WM * TM * 32) x (WN * TN * 32): WM x WN waves, each TM x TN
+// MFMA tiles of 32 x 32.  Wide outputs use 2 x 2 waves of 2 x 2 tiles (128 x
+// 128); outputs at most 32 wide (a layer's channels, a kernel's taps) use 4 x 1
+// waves of 2 x 1 tiles (256 x 32), which wastes no matrix-core work on padding.
 // Split K: workgroup z covers k in [z kchunk, min(K, (z + 1) kchunk)); with
-// more than one slice each writes its raw sums to C + z M N (ldc = N) and
-// k_splitk_reduce applies alpha and beta.
-template <typename T>
+// more than one slice each writes its raw sums to C + z M N (ldc = N), summed
+// later in order with alpha and beta.
+template <typename T, int WM, int WN, int TM, int TN>
 __global__ __launch_bounds__(kWG) void k_gemm_nt(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
                                                  int64_t ldb, float* __restrict__ C, int64_t ldc, int64_t M,
                                                  int64_t N, int64_t K, float alpha, float beta, int64_t kchunk)
 {
+    static_assert(WM * WN * 64 == kWG, "four waves");
     {
         const int64_t kb = (int64_t)blockIdx.z * kchunk;
         A += kb;
@@ -101,55 +108,55 @@ __global__ __launch_bounds__(kWG) void k_gemm_nt(const T* __restrict__ A, int64_
         if (gridDim.z > 1) C += (int64_t)blockIdx.z * M * N;
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t m0 = (int64_t)blockIdx.y * 128 + (wave >> 1) * 64;
-    const int64_t n0 = (int64_t)blockIdx.x * 128 + (wave & 1) * 64;
+    const int64_t m0 = (int64_t)blockIdx.y * (WM * TM * 32) + (wave / WN) * (TM * 32);
+    const int64_t n0 = (int64_t)blockIdx.x * (WN * TN * 32) + (wave % WN) * (TN * 32);
     const int r = lane & 31, h = lane >> 5;
-    f32x16 acc[2][2];
+    f32x16 acc[TM][TN];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < TN; ++b)
 #pragma unroll
             for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.f;
-    const T* pa[2];
-    const T* pb[2];
-    bool oka[2], okb[2];
+    const T* pa[TM];
+    const T* pb[TN];
+    bool oka[TM], okb[TN];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
+    for (int a = 0; a < TM; ++a) {
         const int64_t row = m0 + a * 32 + r;
         oka[a] = row < M;
         pa[a] = A + (oka[a] ? row : 0) * lda + 8 * h;
-        const int64_t col = n0 + a * 32 + r;
-        okb[a] = col < N;
-        pb[a] = B + (okb[a] ? col : 0) * ldb + 8 * h;
     }
-    Frag<T> fa[2], fb[2], na[2], nb[2];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        load_frag(pa[a], oka[a] && K > 0, fa[a]);
-        load_frag(pb[a], okb[a] && K > 0, fb[a]);
+    for (int b = 0; b < TN; ++b) {
+        const int64_t col = n0 + b * 32 + r;
+        okb[b] = col < N;
+        pb[b] = B + (okb[b] ? col : 0) * ldb + 8 * h;
     }
+    Frag<T> fa[TM], fb[TN], na[TM], nb[TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a) load_frag(pa[a], oka[a] && K > 0, fa[a]);
+#pragma unroll
+    for (int b = 0; b < TN; ++b) load_frag(pb[b], okb[b] && K > 0, fb[b]);
     for (int64_t k = 0; k < K; k += 16) {
         const bool more = k + 16 < K;
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            load_frag(pa[a] + (more ? k + 16 : 0), oka[a] && more, na[a]);
-            load_frag(pb[a] + (more ? k + 16 : 0), okb[a] && more, nb[a]);
-        }
+        for (int a = 0; a < TM; ++a) load_frag(pa[a] + (more ? k + 16 : 0), oka[a] && more, na[a]);
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < TN; ++b) load_frag(pb[b] + (more ? k + 16 : 0), okb[b] && more, nb[b]);
 #pragma unroll
-            for (int b = 0; b < 2; ++b) acc[a][b] = mma(fa[a], fb[b], acc[a][b]);
+        for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            fa[a] = na[a];
-            fb[a] = nb[a];
-        }
+            for (int b = 0; b < TN; ++b) acc[a][b] = mma(fa[a], fb[b], acc[a][b]);
+#pragma unroll
+        for (int a = 0; a < TM; ++a) fa[a] = na[a];
+#pragma unroll
+        for (int b = 0; b < TN; ++b) fb[b] = nb[b];
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < TN; ++b)
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 const int64_t row = m0 + a * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
@@ -266,7 +273,7 @@ __global__ __launch_bounds__(kWG) void k_colsum_final(const float* __restrict__ 
 
 // Transposed convolution, stride 2, 4x4 kernel, padding 1: input H x W,
 // output 2H x 2W; output row Y gets input rows y = (Y + 1 - ky) / 2 for the
-// two ky of parity (Y + 1) % 2.  cols: [B*H*W][ldc], column co*16 + ky*4 + kx.
+// two ky of parity (Y + 1) % 2.  cols: [B*H*W][ldc], column (ky*4 + kx)*Co + co.
 __global__ __launch_bounds__(kWG) void k_col2im(const float* __restrict__ cols, int64_t ldc, int64_t B, int64_t H,
                                                 int64_t W, int64_t Co, const float* __restrict__ bias,
                                                 const float* __restrict__ bmap, float* __restrict__ out)
@@ -287,14 +294,14 @@ __global__ __launch_bounds__(kWG) void k_col2im(const float* __restrict__ cols, 
                 const int kx = kx0 + 2 * dx;
                 const int64_t x = (X + 1 - kx) / 2;
                 if (x < 0 || x >= W) continue;
-                s += cols[((b * H + y) * W + x) * ldc + co * 16 + ky * 4 + kx];
+                s += cols[((b * H + y) * W + x) * ldc + (ky * 4 + kx) * Co + co];
             }
         }
         out[i] = s;
     }
 }
 
-// dcols[(b,y,x)][co*16+ky*4+kx] = dout[b][2y-1+ky][2x-1+kx][co] (0 outside),
+// dcols[(b,y,x)][(ky*4+kx)*Co + co] = dout[b][2y-1+ky][2x-1+kx][co] (0 outside),
 // row length ldd (>= 16 Co, zero padded)
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_im2col(const float* __restrict__ dout, int64_t B, int64_t H, int64_t W,
@@ -305,12 +312,109 @@ __global__ __launch_bounds__(kWG) void k_im2col(const float* __restrict__ dout, 
         const int64_t j = i % ldd, p = i / ldd;
         float v = 0.f;
         if (j < 16 * Co) {
-            const int64_t co = j / 16, ky = (j / 4) % 4, kx = j % 4;
+            const int64_t co = j % Co, tap = j / Co, ky = tap / 4, kx = tap % 4;
             const int64_t x = p % W, y = (p / W) % H, b = p / (W * H);
             const int64_t Y = 2 * y - 1 + ky, X = 2 * x - 1 + kx;
             if (Y >= 0 && Y < H2 && X >= 0 && X < W2) v = dout[((b * H2 + Y) * W2 + X) * Co + co];
         }
         dcols[i] = from_f32<T>(v);
+    }
+}
+
+// Four consecutive channels per thread (Co % 4 == 0, ldc / ldd % 4 == 0),
+// 32-bit index arithmetic (P * ldd < 2^31).
+template <typename T>
+__device__ inline void store4(T* p, f4 v);
+template <>
+__device__ inline void store4<float>(float* p, f4 v) { *reinterpret_cast<f4*>(p) = v; }
+template <>
+__device__ inline void store4<uint16_t>(uint16_t* p, f4 v)
+{
+    const uint32_t a = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
+    const uint32_t b = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+    *reinterpret_cast<uint2*>(p) = make_uint2(a, b);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_im2col4(const float* __restrict__ dout, int B, int H, int W, int Co,
+                                                 T* __restrict__ dcols, int ldd)
+{
+    const int H2 = 2 * H, W2 = 2 * W, q4 = ldd / 4, n = B * H * W * q4;
+    for (int i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
+        const int j = (i % q4) * 4, p = i / q4;
+        f4 v = {0.f, 0.f, 0.f, 0.f};
+        if (j < 16 * Co) {
+            const int co = j % Co, tap = j / Co, ky = tap >> 2, kx = tap & 3;
+            const int x = p % W, y = (p / W) % H, b = p / (W * H);
+            const int Y = 2 * y - 1 + ky, X = 2 * x - 1 + kx;
+            if (Y >= 0 && Y < H2 && X >= 0 && X < W2)
+                v = *reinterpret_cast<const f4*>(dout + ((int64_t)(b * H2 + Y) * W2 + X) * Co + co);
+        }
+        store4<T>(dcols + (int64_t)p * ldd + j, v);
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_col2im4(const float* __restrict__ cols, int ldc, int B, int H, int W,
+                                                 int Co, const float* __restrict__ bias, float* __restrict__ out)
+{
+    const int H2 = 2 * H, W2 = 2 * W, c4 = Co / 4, n = B * H2 * W2 * c4;
+    for (int i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
+        const int co = (i % c4) * 4, X = (i / c4) % W2, Y = (i / (c4 * W2)) % H2, b = i / (c4 * W2 * H2);
+        f4 s = bias ? *reinterpret_cast<const f4*>(bias + co) : f4{0.f, 0.f, 0.f, 0.f};
+        const int ky0 = (Y + 1) & 1, kx0 = (X + 1) & 1;
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            const int ky = ky0 + 2 * dy, y = (Y + 1 - ky) / 2;
+            if (y < 0 || y >= H) continue;
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const int kx = kx0 + 2 * dx, x = (X + 1 - kx) / 2;
+                if (x < 0 || x >= W) continue;
+                s += *reinterpret_cast<const f4*>(cols + (int64_t)((b * H + y) * W + x) * ldc + (ky * 4 + kx) * Co +
+                                                  co);
+            }
+        }
+        *reinterpret_cast<f4*>(out + ((int64_t)(b * H2 + Y) * W2 + X) * Co + co) = s;
+    }
+}
+
+// The weight-gradient operand directly: dcolsT[j][p] = dcols[p][j] for
+// j < 16 Co, p < ldo (zero for p >= P); Co in {4, 8, 16, 32}.  Workgroup
+// (x, y) gathers TP pixels x the 256 / Co taps of block y into LDS (float4
+// reads of 4 consecutive channels), then writes each of its rows' TP pixels
+// contiguously (8 threads per row).
+template <typename T, int TP>
+__global__ __launch_bounds__(kWG) void k_im2col_t(const float* __restrict__ dout, int B, int H, int W, int Co,
+                                                  T* __restrict__ dcolsT, int ldo)
+{
+    __shared__ T tile[256][TP + 2];
+    const int H2 = 2 * H, W2 = 2 * W, P = B * H * W, c4 = Co / 4;
+    const int tpb = 256 / Co, tap0 = blockIdx.y * tpb, ntap = tpb < 16 - tap0 ? tpb : 16 - tap0;
+    const int p0 = blockIdx.x * TP;
+    const int units = TP * ntap * c4;
+    for (int u = threadIdx.x; u < units; u += kWG) {
+        const int co = (u % c4) * 4, tl = (u / c4) % ntap, pl = u / (c4 * ntap);
+        const int tap = tap0 + tl, p = p0 + pl;
+        f4 v = {0.f, 0.f, 0.f, 0.f};
+        if (p < P) {
+            const int x = p % W, y = (p / W) % H, b = p / (W * H);
+            const int Y = 2 * y - 1 + (tap >> 2), X = 2 * x - 1 + (tap & 3);
+            if (Y >= 0 && Y < H2 && X >= 0 && X < W2)
+                v = *reinterpret_cast<const f4*>(dout + ((int64_t)(b * H2 + Y) * W2 + X) * Co + co);
+        }
+        const int jl = tl * Co + co;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) tile[jl + e][pl] = from_f32<T>(v[e]);
+    }
+    __syncthreads();
+    constexpr int per = TP / 8;
+    const int rows = ntap * Co;
+    for (int r = threadIdx.x / 8; r < rows; r += kWG / 8) {
+        const int c0 = (threadIdx.x % 8) * per;
+        T* dst = dcolsT + (int64_t)(tap0 * Co + r) * ldo + p0 + c0;
+#pragma unroll
+        for (int e = 0; e < per; ++e)
+            if (p0 + c0 + e < ldo) dst[e] = tile[r][c0 + e];
     }
 }
 
@@ -583,6 +687,8 @@ using namespace sfh::nn;
 
 namespace {
 
+bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
 int colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols, float alpha, float beta, float* out,
            int64_t N, int64_t ldo, const char* name);
 
@@ -593,7 +699,9 @@ int gemm(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* B, int
          int64_t M, int64_t N, int64_t K, float alpha, float beta)
 {
     if (M == 0 || N == 0) return SFH_OK;
-    const int64_t tiles = cdiv(N, 128) * cdiv(M, 128);
+    const bool narrow = N <= 32;
+    const int64_t tm = narrow ? 256 : 128, tn = narrow ? 32 : 128;
+    const int64_t tiles = cdiv(N, tn) * cdiv(M, tm);
     int64_t splits = 1, kchunk = std::max<int64_t>(K, 16);
     if (tiles < 512 && K >= 1024) {
         splits = std::min<int64_t>(std::max<int64_t>(1, 1024 / tiles), K / 256);
@@ -608,17 +716,28 @@ int gemm(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* B, int
         out = (float*)p;
         ldo = N;
     }
-    dim3 grid((unsigned)cdiv(N, 128), (unsigned)cdiv(M, 128), (unsigned)splits);
+    dim3 grid((unsigned)cdiv(N, tn), (unsigned)cdiv(M, tm), (unsigned)splits);
     // slices hold raw sums; alpha and beta are applied when they are added
     const float ka = splits > 1 ? 1.f : alpha, kb = splits > 1 ? 0.f : beta;
     {
         TimedLaunch tl(ctx, "nn_gemm");
-        if (dtype == 0)
-            hipLaunchKernelGGL(k_gemm_nt<uint16_t>, grid, dim3(kWG), 0, ctx->stream, (const uint16_t*)A, lda,
-                               (const uint16_t*)B, ldb, out, ldo, M, N, K, ka, kb, kchunk);
-        else
-            hipLaunchKernelGGL(k_gemm_nt<float>, grid, dim3(kWG), 0, ctx->stream, (const float*)A, lda,
-                               (const float*)B, ldb, out, ldo, M, N, K, ka, kb, kchunk);
+        if (dtype == 0) {
+            const uint16_t *a = (const uint16_t*)A, *b = (const uint16_t*)B;
+            if (narrow)
+                hipLaunchKernelGGL((k_gemm_nt<uint16_t, 4, 1, 2, 1>), grid, dim3(kWG), 0, ctx->stream, a, lda, b, ldb,
+                                   out, ldo, M, N, K, ka, kb, kchunk);
+            else
+                hipLaunchKernelGGL((k_gemm_nt<uint16_t, 2, 2, 2, 2>), grid, dim3(kWG), 0, ctx->stream, a, lda, b, ldb,
+                                   out, ldo, M, N, K, ka, kb, kchunk);
+        } else {
+            const float *a = (const float*)A, *b = (const float*)B;
+            if (narrow)
+                hipLaunchKernelGGL((k_gemm_nt<float, 4, 1, 2, 1>), grid, dim3(kWG), 0, ctx->stream, a, lda, b, ldb,
+                                   out, ldo, M, N, K, ka, kb, kchunk);
+            else
+                hipLaunchKernelGGL((k_gemm_nt<float, 2, 2, 2, 2>), grid, dim3(kWG), 0, ctx->stream, a, lda, b, ldb,
+                                   out, ldo, M, N, K, ka, kb, kchunk);
+        }
     }
     if (int rc = check_launch(ctx, "k_gemm_nt")) return rc;
     // the slices are the rows of a splits x (M N) matrix
@@ -709,6 +828,12 @@ int sfh_nn_col2im(sfh_ctx* ctx, const float* cols, int64_t ldc, int64_t B, int64
     SFH_NN_BEGIN(ctx);
     SFH_ARG_CHECK(ctx, ldc >= 16 * Co, "sfh_nn_col2im: ldc >= 16 Co");
     const int64_t n = B * 4 * H * W * Co;
+    if (Co % 4 == 0 && ldc % 4 == 0 && B * H * W * ldc < ((int64_t)1 << 31) && n < ((int64_t)1 << 31) &&
+        aligned16(cols) && aligned16(out) && (!bias || aligned16(bias))) {
+        hipLaunchKernelGGL(k_col2im4, dim3(grid_for(n / 4)), dim3(kWG), 0, ctx->stream, cols, (int)ldc, (int)B,
+                           (int)H, (int)W, (int)Co, bias, out);
+        return check_launch(ctx, "k_col2im4");
+    }
     hipLaunchKernelGGL(k_col2im, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, cols, ldc, B, H, W, Co, bias,
                        (const float*)nullptr, out);
     return check_launch(ctx, "k_col2im");
@@ -720,6 +845,16 @@ int sfh_nn_im2col(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t
     SFH_NN_BEGIN(ctx);
     SFH_ARG_CHECK(ctx, ldd >= 16 * Co, "sfh_nn_im2col: ldd >= 16 Co");
     const int64_t n = B * H * W * ldd;
+    if (Co % 4 == 0 && ldd % 4 == 0 && n < ((int64_t)1 << 31) && aligned16(dout) && aligned16(dcols)) {
+        const int64_t n4 = n / 4;
+        if (dtype == 0)
+            hipLaunchKernelGGL(k_im2col4<uint16_t>, dim3(grid_for(n4)), dim3(kWG), 0, ctx->stream, dout, (int)B,
+                               (int)H, (int)W, (int)Co, (uint16_t*)dcols, (int)ldd);
+        else
+            hipLaunchKernelGGL(k_im2col4<float>, dim3(grid_for(n4)), dim3(kWG), 0, ctx->stream, dout, (int)B, (int)H,
+                               (int)W, (int)Co, (float*)dcols, (int)ldd);
+        return check_launch(ctx, "k_im2col4");
+    }
     if (dtype == 0)
         hipLaunchKernelGGL(k_im2col<uint16_t>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, dout, B, H, W, Co,
                            (uint16_t*)dcols, ldd);
@@ -727,6 +862,24 @@ int sfh_nn_im2col(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t
         hipLaunchKernelGGL(k_im2col<float>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, dout, B, H, W, Co,
                            (float*)dcols, ldd);
     return check_launch(ctx, "k_im2col");
+}
+
+int sfh_nn_im2col_t(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t H, int64_t W, int64_t Co,
+                    void* dcolsT, int64_t ldo)
+{
+    SFH_NN_BEGIN(ctx);
+    const int64_t P = B * H * W;
+    SFH_ARG_CHECK(ctx, dout && dcolsT && (Co == 4 || Co == 8 || Co == 16 || Co == 32) && ldo >= P &&
+                       16 * Co * ldo < ((int64_t)1 << 31) && aligned16(dout),
+                  "sfh_nn_im2col_t: Co in {4, 8, 16, 32}, ldo >= B H W");
+    const unsigned gy = (unsigned)cdiv(16, 256 / Co);
+    if (dtype == 0)
+        hipLaunchKernelGGL((k_im2col_t<uint16_t, 64>), dim3((unsigned)cdiv(ldo, 64), gy), dim3(kWG), 0, ctx->stream,
+                           dout, (int)B, (int)H, (int)W, (int)Co, (uint16_t*)dcolsT, (int)ldo);
+    else
+        hipLaunchKernelGGL((k_im2col_t<float, 32>), dim3((unsigned)cdiv(ldo, 32), gy), dim3(kWG), 0, ctx->stream,
+                           dout, (int)B, (int)H, (int)W, (int)Co, (float*)dcolsT, (int)ldo);
+    return check_launch(ctx, "k_im2col_t");
 }
 
 // The single-channel output layer as the other layers: a matrix product to

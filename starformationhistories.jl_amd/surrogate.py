@@ -273,10 +273,10 @@ class TemplateSurrogate:
             self._colsum(d, CH, Po, CH, g[cb])
             self._c(self.lib.sfh_nn_im2col(self.ctx.handle, self.dt, d.data_ptr(), B, Hi, Wi, CH,
                                            self.dcols.data_ptr(), 16 * CH))
-            # dK = dcols^T a_in: transposed copies (K = pixels, padded)
+            # dK = dcols^T a_in: transposed operands (K = pixels, padded)
             Pp = _pad16(Pi)
-            self._c(self.lib.sfh_nn_im2col(self.ctx.handle, 1, d.data_ptr(), B, Hi, Wi, CH, self.z.data_ptr(), 16 * CH))
-            self._transpose(self.z, 16 * CH, Pi, 16 * CH, self.dcolsT, Pp)
+            self._c(self.lib.sfh_nn_im2col_t(self.ctx.handle, self.dt, d.data_ptr(), B, Hi, Wi, CH,
+                                             self.dcolsT.data_ptr(), Pp))
             self._transpose(pre_in, CH, Pi, CH, self.actT, Pp, act=SILU)
             self._gemm(self.dcolsT, Pp, self.actT, Pp, g[K], CH, 16 * CH, CH, Pp)
             # d a_in = dcols K

@@ -100,7 +100,8 @@ def test_gather_and_indexed_mse(sfh, gpu_ctx):
 
 
 @pytest.mark.parametrize("dtype", [0, 1])
-@pytest.mark.parametrize("M,N,K", [(200, 40, 20000), (16, 32, 50016), (300, 130, 64)])
+@pytest.mark.parametrize("M,N,K", [(200, 40, 20000), (16, 32, 50016), (300, 130, 64), (1000, 24, 48), (777, 32, 2048),
+                                   (70000, 16, 32)])
 def test_gemm_split_k(sfh, gpu_ctx, dtype, M, N, K):
     import torch
     rng = np.random.default_rng(M + N + K)

@@ -45,3 +45,23 @@ def test_transposed_convolution_shape_and_single_pixel():
     assert out.shape == (1, 6, 6, 1)
     assert np.array_equal(out[0, 1:5, 1:5, 0], np.arange(16.0).reshape(4, 4))
     assert out.sum() == K.sum()
+
+
+def test_transposed_convolution_against_direct_loops():
+    """The oracle's scatter form equals the definition written as loops, with
+    weight rows (ky*4 + kx)*co + o (tap-major)."""
+    rng = np.random.default_rng(0)
+    B, H, W, ci, co = 2, 3, 4, 5, 3
+    x = rng.normal(size=(B, H, W, ci))
+    K = rng.normal(size=(16 * co, ci))
+    ref = np.zeros((B, 2 * H, 2 * W, co))
+    for b in range(B):
+        for y in range(H):
+            for xx in range(W):
+                for ky in range(4):
+                    for kx in range(4):
+                        Y, X = 2 * y - 1 + ky, 2 * xx - 1 + kx
+                        if 0 <= Y < 2 * H and 0 <= X < 2 * W:
+                            for o in range(co):
+                                ref[b, Y, X, o] += x[b, y, xx] @ K[(ky * 4 + kx) * co + o]
+    assert np.max(np.abs(R.convT(x, K, co) - ref)) < 1e-12
