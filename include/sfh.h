@@ -75,6 +75,15 @@ const char* sfh_last_error(const sfh_ctx* ctx);
 int sfh_ctx_synchronize(sfh_ctx* ctx);
 /* Per-kernel timing with HIP events on the context's stream. */
 int sfh_ctx_set_timing(sfh_ctx* ctx, int enable);
+/* No reference counterpart.  Capture the context's stream into a HIP graph:
+ * every call between begin and end is recorded instead of run (calls that
+ * allocate or wait invalidate the capture, so run them once beforehand);
+ * sfh_graph_launch replays the graph `times` times on the stream. */
+typedef struct sfh_graph sfh_graph;
+int sfh_ctx_capture_begin(sfh_ctx* ctx);
+int sfh_ctx_capture_end(sfh_ctx* ctx, sfh_graph** out);
+int sfh_graph_launch(sfh_graph* graph, int64_t times);
+int sfh_graph_destroy(sfh_graph* graph);
 int sfh_ctx_kernel_stats(sfh_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
 int sfh_ctx_reset_stats(sfh_ctx* ctx);
 int sfh_version(void);
@@ -447,6 +456,13 @@ int sfh_nn_log_template(sfh_ctx* ctx, const float* out, int64_t B, int64_t H2, i
  * from E with S' instead of E (amplitude free). */
 int sfh_nn_template_kl(sfh_ctx* ctx, const float* E, int64_t lde, const float* S, int64_t lds, int64_t n,
                        int64_t nbins, double floor_value, double* out);
+/* Step-indexed forms for captured training loops (counter: a device int64):
+ * rows_step[i] = rows_all[counter n + i]; counter += inc; and the Adam step
+ * with (lr, 1 - b1^t, 1 - b2^t) read from sched[3 counter .. +2]. */
+int sfh_nn_select_rows(sfh_ctx* ctx, const int64_t* rows_all, const int64_t* counter, int64_t n, int64_t* rows_step);
+int sfh_nn_counter_add(sfh_ctx* ctx, int64_t* counter, int64_t inc);
+int sfh_nn_adam_sched(sfh_ctx* ctx, float* p, const float* g, float* m, float* v, int64_t n, const float* sched,
+                      const int64_t* counter, float b1, float b2, float eps, float weight_decay);
 /* One Adam step (bias-corrected, step >= 1) on FP32 parameters. */
 int sfh_nn_adam(sfh_ctx* ctx, float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
                 float eps, int64_t step, float weight_decay);

@@ -148,6 +148,14 @@ SIGNATURES = [
     ("sfh_nn_mse", _int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, C.POINTER(C.c_float), _vp]),
     ("sfh_nn_template_loss", _int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, C.c_float, C.c_float,
                                     C.c_float, C.c_float, C.c_float, _vp, _vp]),
+    ("sfh_ctx_capture_begin", _int, [_vp]),
+    ("sfh_ctx_capture_end", _int, [_vp, C.POINTER(_vp)]),
+    ("sfh_graph_launch", _int, [_vp, _i64]),
+    ("sfh_graph_destroy", _int, [_vp]),
+    ("sfh_nn_select_rows", _int, [_vp, _vp, _vp, _i64, _vp]),
+    ("sfh_nn_counter_add", _int, [_vp, _vp, _i64]),
+    ("sfh_nn_adam_sched", _int, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, C.c_float, C.c_float, C.c_float,
+                                 C.c_float]),
     ("sfh_nn_gather_rows", _int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64]),
     ("sfh_nn_log_target", _int, [_vp, _vp, _i64, _i64, _i64, _i64, C.c_float, C.c_float, C.c_float, _vp]),
     ("sfh_nn_log_template", _int, [_vp, _vp, _i64, _i64, _i64, _i64, _i64, C.c_float, C.c_float, C.c_float, _vp,

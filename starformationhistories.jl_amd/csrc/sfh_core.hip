@@ -531,6 +531,67 @@ int sfh_ctx_synchronize(sfh_ctx* ctx)
     return SFH_OK;
 }
 
+int sfh_ctx_capture_begin(sfh_ctx* ctx)
+{
+    if (!ctx) return SFH_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (int rc = bind(ctx)) return rc;
+    SFH_ARG_CHECK(ctx, !ctx->timing, "sfh_ctx_capture_begin: switch launch timing off first");
+    SFH_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    return SFH_OK;
+}
+
+int sfh_ctx_capture_end(sfh_ctx* ctx, sfh_graph** out)
+{
+    if (!ctx || !out) return SFH_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    *out = nullptr;
+    if (int rc = bind(ctx)) return rc;
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+    if (ce != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return set_error(ctx, SFH_ERR_DEVICE, "sfh_ctx_capture_end: capture failed (%s); an allocation or "
+                         "synchronisation inside the captured calls invalidates it", hipGetErrorString(ce));
+    }
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return set_error(ctx, SFH_ERR_DEVICE, "sfh_ctx_capture_end: graph instantiation failed");
+    }
+    sfh_graph* gr = new sfh_graph{ctx, graph, exec};
+    ctx_retain(ctx);
+    *out = gr;
+    return SFH_OK;
+}
+
+int sfh_graph_launch(sfh_graph* gr, int64_t times)
+{
+    if (!gr || times < 0) return SFH_ERR_ARGUMENT;
+    sfh_ctx* ctx = gr->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (int rc = bind(ctx)) return rc;
+    for (int64_t k = 0; k < times; ++k) SFH_HIP_TRY(ctx, hipGraphLaunch(gr->exec, ctx->stream));
+    return SFH_OK;
+}
+
+int sfh_graph_destroy(sfh_graph* gr)
+{
+    if (!gr) return SFH_OK;
+    sfh_ctx* ctx = gr->ctx;
+    {
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        hipSetDevice(ctx->device);
+        hipStreamSynchronize(ctx->stream);
+        (void)hipGraphExecDestroy(gr->exec);
+        (void)hipGraphDestroy(gr->graph);
+        delete gr;
+    }
+    ctx_release(ctx);
+    return SFH_OK;
+}
+
 int sfh_ctx_set_timing(sfh_ctx* ctx, int enable)
 {
     if (!ctx) return SFH_ERR_ARGUMENT;

@@ -61,6 +61,11 @@ struct sfh_templates {
     float* d_models = nullptr;          // ld x ntemplates, column-major, ld = nbins rounded up to 256
 };
 
+struct sfh_graph {
+    sfh_ctx* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
 struct sfh_data {
     sfh_ctx* ctx = nullptr;
     int64_t nbins = 0;

@@ -531,9 +531,10 @@ __global__ __launch_bounds__(kWG) void k_tl_grad(const float* __restrict__ out, 
 __global__ __launch_bounds__(64) void k_tl_accumulate(const double* __restrict__ part, int64_t B, int64_t R,
                                                       int64_t S, float sc, double* __restrict__ acc)
 {
-    if (threadIdx.x != 0) return;
+    // thread t handles images t, t + 64, ...; the per-thread sums are added in order
+    __shared__ double red[3][64];
     double q = 0.0, kl = 0.0, sq = 0.0;
-    for (int64_t b = 0; b < B; ++b) {
+    for (int64_t b = threadIdx.x; b < B; b += 64) {
         double ss, se, a = 0.0;
         tl_totals(part, b, ss, se);
         for (int c = 0; c < kTLChunks; ++c) {
@@ -544,6 +545,16 @@ __global__ __launch_bounds__(64) void k_tl_accumulate(const double* __restrict__
         kl += (double)sc * a / se + ls;
         sq += ls * ls;
     }
+    red[0][threadIdx.x] = q;
+    red[1][threadIdx.x] = kl;
+    red[2][threadIdx.x] = sq;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int i = 1; i < 64; ++i) {
+        q += red[0][i];
+        kl += red[1][i];
+        sq += red[2][i];
+    }
     acc[0] += q / (double)(B * R * S);
     acc[1] += kl / (double)B;
     acc[2] += sq / (double)B;
@@ -553,9 +564,13 @@ __global__ __launch_bounds__(64) void k_tl_accumulate(const double* __restrict__
 __global__ __launch_bounds__(64) void k_mse_accumulate(const float* __restrict__ part, int nb, double denom,
                                                        double* __restrict__ acc)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ double red[64];
     double s = 0.0;
-    for (int i = 0; i < nb; ++i) s += (double)part[i];
+    for (int i = threadIdx.x; i < nb; i += 64) s += (double)part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int i = 1; i < 64; ++i) s += red[i];
     acc[0] += s / denom;
 }
 
@@ -654,6 +669,38 @@ __global__ __launch_bounds__(kWG) void k_template_kl(const float* __restrict__ E
         out[3 * k] = se > 0.0 ? a / se - log(se) + log(ss) : 0.0;
         out[3 * k + 1] = se;
         out[3 * k + 2] = ss;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_select_rows(const int64_t* __restrict__ rows_all,
+                                                    const int64_t* __restrict__ counter, int64_t n,
+                                                    int64_t* __restrict__ rows_step)
+{
+    const int64_t base = counter[0] * n;
+    for (int64_t i = threadIdx.x; i < n; i += 64) rows_step[i] = rows_all[base + i];
+}
+
+__global__ __launch_bounds__(64) void k_counter_add(int64_t* counter, int64_t inc)
+{
+    if (threadIdx.x == 0) counter[0] += inc;
+}
+
+// Adam with (lr, c1, c2) from a per-step schedule
+__global__ __launch_bounds__(kWG) void k_adam_sched(float* __restrict__ p, const float* __restrict__ g,
+                                                    float* __restrict__ m, float* __restrict__ v, int64_t n,
+                                                    const float* __restrict__ sched,
+                                                    const int64_t* __restrict__ counter, float b1, float b2,
+                                                    float eps, float wd)
+{
+    const int64_t c = counter[0];
+    const float lr = sched[3 * c], c1 = sched[3 * c + 1], c2 = sched[3 * c + 2];
+    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) {
+        const float gi = g[i] + wd * p[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     }
 }
 
@@ -1046,6 +1093,32 @@ int sfh_nn_template_kl(sfh_ctx* ctx, const float* E, int64_t lde, const float* S
     hipLaunchKernelGGL(k_template_kl, dim3((unsigned)n), dim3(kWG), 0, ctx->stream, E, lde, S, lds, nbins,
                        floor_value, out);
     return check_launch(ctx, "k_template_kl");
+}
+
+int sfh_nn_select_rows(sfh_ctx* ctx, const int64_t* rows_all, const int64_t* counter, int64_t n, int64_t* rows_step)
+{
+    SFH_NN_BEGIN(ctx);
+    SFH_ARG_CHECK(ctx, rows_all && counter && rows_step && n >= 1, "sfh_nn_select_rows: bad arguments");
+    hipLaunchKernelGGL(k_select_rows, dim3(1), dim3(64), 0, ctx->stream, rows_all, counter, n, rows_step);
+    return check_launch(ctx, "k_select_rows");
+}
+
+int sfh_nn_counter_add(sfh_ctx* ctx, int64_t* counter, int64_t inc)
+{
+    SFH_NN_BEGIN(ctx);
+    SFH_ARG_CHECK(ctx, counter, "sfh_nn_counter_add: null counter");
+    hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, ctx->stream, counter, inc);
+    return check_launch(ctx, "k_counter_add");
+}
+
+int sfh_nn_adam_sched(sfh_ctx* ctx, float* p, const float* g, float* m, float* v, int64_t n, const float* sched,
+                      const int64_t* counter, float b1, float b2, float eps, float weight_decay)
+{
+    SFH_NN_BEGIN(ctx);
+    SFH_ARG_CHECK(ctx, p && g && m && v && sched && counter, "sfh_nn_adam_sched: null pointer");
+    hipLaunchKernelGGL(k_adam_sched, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, p, g, m, v, n, sched, counter, b1,
+                       b2, eps, weight_decay);
+    return check_launch(ctx, "k_adam_sched");
 }
 
 int sfh_nn_adam(sfh_ctx* ctx, float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,

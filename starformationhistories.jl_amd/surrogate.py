@@ -335,17 +335,20 @@ class TemplateSurrogate:
         return out
 
     def train(self, params, images, steps, batch=32, lr=1e-3, lr_min=1e-5, warmup=200, seed=0, log_every=250,
-              weight_decay=0.0, kl_weight=1.0, sum_weight=0.0):
+              weight_decay=0.0, kl_weight=1.0, sum_weight=0.0, graph=True, b1=0.9, b2=0.999, eps=1e-8):
         """Adam on shuffled mini-batches of (params: n x 4 physical, images:
         n x 200 x 300 device tensor from ``images``) for ``steps`` steps; the
         learning rate rises linearly over ``warmup`` steps, then follows a
         cosine from lr to lr_min.  The loss is the MSE of the log image plus
         kl_weight times the mean KL(exact || network) per star (the
         log-likelihood a fit loses per star) plus sum_weight times the mean
-        squared error of the log template sums.  Runs without host
-        synchronisation except every ``log_every`` steps, when the means since
-        the last report are read.  Returns [(step, MSE, KL per star, squared
-        log-sum error), ...]."""
+        squared error of the log template sums.  Each step reads its rows and
+        its (lr, bias corrections) from device arrays through a device step
+        counter, so after one eager step the step is captured in a HIP graph
+        and replayed (graph=False runs every step eagerly; both give the same
+        numbers).  The host waits only every ``log_every`` steps, to read the
+        means since the last report.  Returns [(step, MSE, KL per star,
+        squared log-sum error), ...]."""
         torch = self.torch
         n = images.shape[0]
         if images.shape[1:] != (CROP_H, CROP_W) or len(params) != n:
@@ -357,34 +360,79 @@ class TemplateSurrogate:
         nep = -(-steps // per_epoch)
         rows = np.concatenate([rng.permutation(n)[:per_epoch * batch] for _ in range(nep)])
         rows_d = torch.from_numpy(rows.astype(np.int64)).to(self.dev)
-        acc = torch.zeros(3, dtype=torch.float64, device=self.dev)
-        self.x0f.zero_()
-        torch.cuda.synchronize()
-        hist, last = [], 0
-        self.B = batch
+        sched = np.empty((steps, 3), dtype=np.float64)
         for k in range(steps):
-            r = rows_d[k * batch:(k + 1) * batch]
-            self._c(self.lib.sfh_nn_gather_rows(self.ctx.handle, xs.data_ptr(), 4, r.data_ptr(), batch, 4,
-                                                self.x0f.data_ptr(), 16))
-            self._forward_device(batch)
-            self._c(self.lib.sfh_nn_template_loss(self.ctx.handle, self.out.data_ptr(), images.data_ptr(),
-                                                  r.data_ptr(), batch, OUT_H, OUT_W, CROP_H, CROP_W, LOG_SCALE,
-                                                  LOG_OFFSET, 1.0, float(kl_weight), float(sum_weight),
-                                                  self.dout.data_ptr(), acc.data_ptr()))
-            self._backward_from_dout()
             if k < warmup:
                 rate = lr * (k + 1) / warmup
             else:
                 u = (k - warmup) / max(1, steps - warmup)
                 rate = lr_min + 0.5 * (lr - lr_min) * (1.0 + np.cos(np.pi * u))
-            self.adam_step(rate, weight_decay=weight_decay)
-            if (k + 1) % log_every == 0 or k + 1 == steps:
-                self.ctx.synchronize()
-                a = acc.cpu().numpy() / (k + 1 - last)
-                hist.append((k + 1, float(a[0]), float(a[1]), float(a[2])))
-                acc.zero_()
-                torch.cuda.synchronize()
-                last = k + 1
+            t = self.step_count + k + 1
+            sched[k] = (rate, 1.0 - b1 ** t, 1.0 - b2 ** t)
+        sched_d = torch.from_numpy(sched.astype(np.float32).ravel()).to(self.dev)
+        counter = torch.zeros(1, dtype=torch.int64, device=self.dev)
+        rows_step = torch.zeros(batch, dtype=torch.int64, device=self.dev)
+        acc = torch.zeros(3, dtype=torch.float64, device=self.dev)
+        self.x0f.zero_()
+        torch.cuda.synchronize()
+        self.B = batch
+        L, h = self.lib, self.ctx.handle
+
+        def body():
+            self._c(L.sfh_nn_select_rows(h, rows_d.data_ptr(), counter.data_ptr(), batch, rows_step.data_ptr()))
+            self._c(L.sfh_nn_gather_rows(h, xs.data_ptr(), 4, rows_step.data_ptr(), batch, 4, self.x0f.data_ptr(),
+                                         16))
+            self._forward_device(batch)
+            self._c(L.sfh_nn_template_loss(h, self.out.data_ptr(), images.data_ptr(), rows_step.data_ptr(), batch,
+                                           OUT_H, OUT_W, CROP_H, CROP_W, LOG_SCALE, LOG_OFFSET, 1.0,
+                                           float(kl_weight), float(sum_weight), self.dout.data_ptr(),
+                                           acc.data_ptr()))
+            self._backward_from_dout()
+            self._c(L.sfh_nn_adam_sched(h, self.flat.data_ptr(), self.grad.data_ptr(), self.m.data_ptr(),
+                                        self.v.data_ptr(), self.nparams, sched_d.data_ptr(), counter.data_ptr(),
+                                        b1, b2, eps, weight_decay))
+            self._c(L.sfh_nn_counter_add(h, counter.data_ptr(), 1))
+
+        hist, last = [], 0
+
+        def report(k):
+            nonlocal last
+            self.ctx.synchronize()
+            a = acc.cpu().numpy() / (k - last)
+            hist.append((k, float(a[0]), float(a[1]), float(a[2])))
+            acc.zero_()
+            torch.cuda.synchronize()
+            last = k
+
+        body()                                   # also sizes every scratch buffer
+        k = 1
+        if k % log_every == 0 or k == steps:
+            report(k)
+        g = None
+        if graph and steps > 1:
+            self.ctx.synchronize()
+            gh = C.c_void_p()
+            self._c(L.sfh_ctx_capture_begin(h))
+            try:
+                body()
+            finally:
+                rc = L.sfh_ctx_capture_end(h, C.byref(gh))
+            self._c(rc)
+            g = gh
+        try:
+            while k < steps:
+                nxt = min(steps, (k // log_every + 1) * log_every)
+                if g is not None:
+                    self._c(L.sfh_graph_launch(g, nxt - k))
+                else:
+                    for _ in range(nxt - k):
+                        body()
+                k = nxt
+                report(k)
+        finally:
+            if g is not None:
+                L.sfh_graph_destroy(g)
+        self.step_count += steps
         self.ctx.synchronize()
         return hist
 

@@ -216,3 +216,22 @@ def test_template_loss_against_restatement_and_differences(sfh, gpu_ctx):
         fm = np.dot([wm, wk, ws], total(ym))
         num = (fp - fm) / (2 * h)
         assert abs(g[b, yy, xx] - num) <= 1e-4 * abs(num) + 1e-7
+
+
+def test_captured_training_equals_eager(sfh, gpu_ctx):
+    """The training step replayed from a HIP graph gives the same parameters
+    and losses as the same steps run eagerly (every replay checked through the
+    loss history and the final parameters)."""
+    from sfh_amd import surrogate as S
+    P = S.sample_parameters(48, seed=6, lo=[8.0, -1.5, 25.0, 0.8], hi=[9.0, -0.5, 27.0, 1.5])
+    T = S.exact_templates(P)
+    out = []
+    for graph in (False, True):
+        net = S.TemplateSurrogate(dtype="bf16", seed=4, max_batch=16)
+        img = net.images(T)
+        hist = net.train(P, img, steps=45, batch=16, warmup=10, log_every=5, graph=graph)
+        out.append((hist, net.parameters(), net.step_count))
+    (h0, p0, s0), (h1, p1, s1) = out
+    assert s0 == s1 == 45 and len(h0) == len(h1) == 9
+    assert h0 == h1
+    assert np.array_equal(p0, p1)
