@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--fits", type=int, default=256, help="independent fits in the uneven multi-fit run")
     ap.add_argument("--no-multifit", action="store_true")
-    ap.add_argument("--surrogate-steps", type=int, default=25000,
+    ap.add_argument("--surrogate-steps", type=int, default=40000,
                     help="training steps (batch 32) of the template surrogate on 20,000 templates")
     ap.add_argument("--surrogate-stars", type=float, default=100.0,
                     help="stars per template in the dlnL check of the filled grid")
