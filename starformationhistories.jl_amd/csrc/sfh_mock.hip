@@ -603,6 +603,14 @@ int sfh_generate_stars(sfh_ctx* ctx, int64_t npoints, const double* mini, int32_
         }
         if (int rc = compact_append(c, d_keep, kend, d_m, d_g, s0)) return fail(rc);
         s0 += kend;
+        // a limit no system can reach would keep the reference's loop running
+        // for ever: stop when a whole batch added nothing, or past 2^40 systems
+        if (!done && total == 0)
+            return fail(set_error(ctx, SFH_ERR_DOMAIN, "sfh_generate_stars: %lld systems drawn and none contributes "
+                                  "to the limit (no isochrone mass in the IMF range, or no flux in the limit filter)",
+                                  (long long)s0));
+        if (!done && s0 > ((int64_t)1 << 40))
+            return fail(set_error(ctx, SFH_ERR_DOMAIN, "sfh_generate_stars: limit not reached after 2^40 systems"));
         if (!done) {
             // next batch from the mean contribution so far
             const double per = (double)total / (double)s0;

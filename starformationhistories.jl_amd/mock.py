@@ -313,10 +313,13 @@ def _generate(mini_vec, mags, mag_names, mode, limit, limit_filter, imf, dist_mo
     m = np.ascontiguousarray(m)
     g = np.ascontiguousarray(g)
     h = C.c_void_p()
-    ctx.check(ctx.lib.sfh_generate_stars(ctx.handle, m.size, m.ctypes.data, g.shape[1], g.ctypes.data, mode,
-                                         float(limit), int(limit_filter), float(mmin_cut), float(mag_lim), mlf,
-                                         C.byref(imf.as_c()), C.byref(bm), int(seed) & (2 ** 64 - 1),
-                                         int(stream) & (2 ** 64 - 1), C.byref(h)))
+    try:
+        ctx.check(ctx.lib.sfh_generate_stars(ctx.handle, m.size, m.ctypes.data, g.shape[1], g.ctypes.data, mode,
+                                             float(limit), int(limit_filter), float(mmin_cut), float(mag_lim), mlf,
+                                             C.byref(imf.as_c()), C.byref(bm), int(seed) & (2 ** 64 - 1),
+                                             int(stream) & (2 ** 64 - 1), C.byref(h)))
+    except _lib.InexactError as e:          # the library's domain errors: an unreachable limit
+        raise DomainError(str(e)) from None
     return Catalog(h, ctx)
 
 

@@ -125,6 +125,10 @@ def test_reference_statistics_through_gpu(sfh, gpu_ctx):
         sfh.generate_stars_mag(M_INI, mags, NAMES, -8.0, "V", imf)
     with pytest.raises(sfh.ArgumentError):
         sfh.generate_stars_mag(M_INI, mags, NAMES, -8.0, "F814W", imf, mag_lim=25.0, mag_lim_name="V")
+    # a luminosity no system can reach (IMF above every isochrone mass) is an error, not an endless loop
+    with pytest.raises(sfh.DomainError):
+        sfh.generate_stars_mag(M_INI, mags, NAMES, -8.0, "F814W", mock.Salpeter1955(1.0, 100.0),
+                               binary_model=sfh.NoBinaries())
     # composite: two spoofed isochrones, half the mass each (:219-238)
     cm = [M_INI, M_INI + 0.01]
     cg = [mags, [F606W, F814W - 0.02]]
