@@ -441,8 +441,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
-              "described in DESIGN.md but not run.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU sharding of "
+              "these paths is a design, not something this benchmark runs.", file=sys.stderr)
         sys.exit(2)
     import torch
     import __graft_entry__ as ge
