@@ -534,17 +534,17 @@ int sfh_generate_stars(sfh_ctx* ctx, int64_t npoints, const double* mini, int32_
     a.mini = d_iso;
     a.mags = d_iso + npoints;
 
+    // the catalogue takes its reference on the context only once it is complete,
+    // so a failure never releases the context while its lock is held
     sfh_catalog* c = new sfh_catalog();
     c->ctx = ctx;
     c->nstars = a.nst;
     c->nfilters = nfilters;
-    ctx_retain(ctx);
     auto fail = [&](int rc) {
         hipFree(c->d_masses);
         hipFree(c->d_mags);
         hipFree(c->d_index);
         delete c;
-        ctx_release(ctx);
         return rc;
     };
 
@@ -620,6 +620,7 @@ int sfh_generate_stars(sfh_ctx* ctx, int64_t npoints, const double* mini, int32_
     }
     c->sampled = s0;
     resolve_timing(ctx);
+    ctx_retain(ctx);
     *out = c;
     return SFH_OK;
 }
@@ -636,24 +637,19 @@ int sfh_catalog_from_host(sfh_ctx* ctx, int64_t n, int32_t nfilters, const doubl
     c->ctx = ctx;
     c->nstars = 0;
     c->nfilters = nfilters;
-    ctx_retain(ctx);
-    if (int rc = catalog_reserve(c, std::max<int64_t>(n, 1))) {
+    int rc = catalog_reserve(c, std::max<int64_t>(n, 1));
+    if (!rc && n > 0 &&
+        hipMemcpy(c->d_mags, mags, (size_t)n * nfilters * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        rc = set_error(ctx, SFH_ERR_DEVICE, "sfh_catalog_from_host: copy failed");
+    if (rc) {
         hipFree(c->d_mags);
         hipFree(c->d_index);
         delete c;
-        ctx_release(ctx);
         return rc;
-    }
-    if (n > 0 && hipMemcpy(c->d_mags, mags, (size_t)n * nfilters * sizeof(double), hipMemcpyHostToDevice) !=
-                     hipSuccess) {
-        hipFree(c->d_mags);
-        hipFree(c->d_index);
-        delete c;
-        ctx_release(ctx);
-        return set_error(ctx, SFH_ERR_DEVICE, "sfh_catalog_from_host: copy failed");
     }
     c->n = n;
     c->sampled = n;
+    ctx_retain(ctx);
     *out = c;
     return SFH_OK;
 }
@@ -776,7 +772,6 @@ int sfh_model_cmd(sfh_ctx* ctx, const sfh_catalog* in, const sfh_table* err, con
     c->nstars = 0;
     c->nfilters = nf;
     c->sampled = n;
-    ctx_retain(ctx);
     if (n > 0) {
         {
             TimedLaunch tl(ctx, "mock_observe");
@@ -789,11 +784,11 @@ int sfh_model_cmd(sfh_ctx* ctx, const sfh_catalog* in, const sfh_table* err, con
             hipFree(c->d_mags);
             hipFree(c->d_index);
             delete c;
-            ctx_release(ctx);
             return rc;
         }
     }
     resolve_timing(ctx);
+    ctx_retain(ctx);
     *out = c;
     return SFH_OK;
 }
