@@ -247,10 +247,11 @@ __global__ __launch_bounds__(kWG) void k_colsum_part(const float* __restrict__ X
     __syncthreads();
     if (g == 0 && c < cols) part[(int64_t)blockIdx.y * cols + c] = ((red[0][cl] + red[1][cl]) + red[2][cl]) + red[3][cl];
 }
-// out[(c / N) ldo + c % N] = alpha (sum of the partials of column c) + beta out[...]
-__global__ __launch_bounds__(kWG) void k_colsum_final(const float* __restrict__ part, int64_t nparts, int64_t cols,
-                                                      float alpha, float beta, float* __restrict__ out, int64_t N,
-                                                      int64_t ldo)
+// out[(c / N) ldo + c % N] = alpha (sum of the partials of column c) + beta out[...];
+// partial p of column c at part[p ldp + c]
+__global__ __launch_bounds__(kWG) void k_colsum_final(const float* __restrict__ part, int64_t nparts, int64_t ldp,
+                                                      int64_t cols, float alpha, float beta, float* __restrict__ out,
+                                                      int64_t N, int64_t ldo)
 {
     __shared__ float red[4][64];
     const int g = threadIdx.x >> 6, cl = threadIdx.x & 63;
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(kWG) void k_colsum_final(const float* __restrict__ 
     float s = 0.f;
     if (c < cols) {
 #pragma unroll 8
-        for (int64_t p = g; p < nparts; p += 4) s += part[p * cols + c];
+        for (int64_t p = g; p < nparts; p += 4) s += part[p * ldp + c];
     }
     red[g][cl] = s;
     __syncthreads();
@@ -798,6 +799,12 @@ int colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols,
            int64_t N, int64_t ldo, const char* name)
 {
     if (cols == 0) return SFH_OK;
+    if (rows <= 256) {
+        // few rows (bias gradients of a batch, short split-K sums): one pass
+        hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)cdiv(cols, 64)), dim3(kWG), 0, ctx->stream, X, rows, ld,
+                           cols, alpha, beta, out, N, ldo);
+        return check_launch(ctx, "k_colsum");
+    }
     const int64_t nparts = std::min<int64_t>(std::max<int64_t>(1, cdiv(rows, 64)), 1024);
     const int64_t chunk = std::max<int64_t>(1, cdiv(rows, nparts));
     void* p = nullptr;
@@ -805,7 +812,7 @@ int colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols,
     hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)cdiv(cols, 64), (unsigned)nparts), dim3(kWG), 0, ctx->stream, X,
                        ld, rows, cols, chunk, (float*)p);
     hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)cdiv(cols, 64)), dim3(kWG), 0, ctx->stream, (const float*)p,
-                       nparts, cols, alpha, beta, out, N, ldo);
+                       nparts, cols, cols, alpha, beta, out, N, ldo);
     return check_launch(ctx, "k_colsum");
 }
 
