@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--mock-mass", type=float, default=1e7,
                     help="birth mass (Msun) of the mock population sampled, observed and binned on the GPU")
     ap.add_argument("--no-mock", action="store_true")
+    ap.add_argument("--no-binaries", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates HBM counters)")
     return ap.parse_args()
@@ -304,6 +305,45 @@ def surrogate_bench(sfh, torch, ctx, O, args, ncpu):
     for nstars in (args.surrogate_stars, 10.0 * args.surrogate_stars):
         out[f"fill_{nstars:g}_stars"] = _fill_bench(S, torch, ctx, O, args, ncpu, net, la, mh, nstars,
                                                     20000 / gen_s)
+    return out
+
+
+def binaries_bench(sfh, ctx, args):
+    """Unresolved-binary templates (RandomBinaryPairs(0.3)) of the 700
+    config-1 isochrones, downsampled 4x as partial_cmd_smooth's binary branch
+    does: all pair histograms in one launch and all templates in one
+    deposition (binary_hess_batch), against binary_hess one template at a
+    time on a sample."""
+    from sfh_amd import synth
+    edges = synth.cfg1_edges()
+    imf = synth.Kroupa2001(0.08, 120.0)
+    pts = [synth.template_points(s, 2000, edges, 120.0)[0] for s in synth.cfg1_specs()]
+    isos = [(np.asarray(sp.new_mini)[::4], [np.asarray(m)[::4] for m in sp.new_iso_mags]) for sp in pts]
+    kw = dict(normalize_value=1.0, mean_mass=imf.mean(), ctx=ctx)
+    model = sfh.RandomBinaryPairs(0.3)
+    errs, comp = [synth.photerr, synth.photerr], [synth.completeness, synth.completeness]
+    sfh.binary_hess_batch(model, isos[:4], errs, 1, (0, 1), imf, comp, None, edges, **kw)        # warm-up
+    ctx.set_timing(True)
+    ctx.reset_stats()
+    t0 = time.time()
+    T = sfh.binary_hess_batch(model, isos, errs, 1, (0, 1), imf, comp, None, edges, **kw)
+    batch_s = time.time() - t0
+    pk, _ = ctx.kernel_stats("binary_pairs")
+    dk, _ = ctx.kernel_stats("deposit_cov")
+    ctx.set_timing(False)
+    n = len(isos)
+    t0 = time.time()
+    k = 8
+    for m, g in isos[:k]:
+        sfh.binary_hess(model, m, g, errs, 1, (0, 1), imf, comp, None, edges, **kw)
+    single_s = (time.time() - t0) / k
+    out = {"templates": n, "points_per_isochrone": int(np.mean([m.size for m, _ in isos])),
+           "batch_seconds": batch_s, "batch_templates_per_s": n / batch_s, "pairs_kernel_ms": pk,
+           "deposit_kernel_ms": dk, "single_seconds_per_template": single_s,
+           "batch_over_single": single_s * n / batch_s, "sum_check": float(T.sum().item())}
+    log(f"binaries: {n} templates in {batch_s:.2f} s batched ({n / batch_s:.0f} templates/s; pair kernel "
+        f"{pk:.2f} ms, deposition {dk:.2f} ms) vs {single_s * 1e3:.1f} ms per template one at a time "
+        f"({out['batch_over_single']:.1f}x)")
     return out
 
 
@@ -612,6 +652,8 @@ def main():
         extras["surrogate"] = surrogate_bench(sfh, torch, ctx, O, args, ncpu)
     if not args.no_extras and not args.no_mock:
         extras["mock"] = mock_bench(sfh, ctx, args)
+    if not args.no_extras and not args.no_binaries:
+        extras["binaries"] = binaries_bench(sfh, ctx, args)
 
     line = {
         "metric": "loglike+grad evals/s (700 tpl×60k bins) batched vs plain; templates/s; HBM GB/s",

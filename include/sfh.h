@@ -279,6 +279,15 @@ int sfh_binary_pairs_hist(sfh_ctx* ctx, int64_t npoints, const double* mini, con
                           const double* mag_a, const double* mag_b, int y_is_a, double prefac,
                           const sfh_table* compl_a, const sfh_table* compl_b, const sfh_table* bias_a,
                           const sfh_table* bias_b, sfh_range xedges, sfh_range yedges, double* out);
+/* The same pair histograms for many isochrones in one launch (new API):
+ * template t owns points offsets[t] .. offsets[t+1]-1 of the concatenated
+ * mini / imf_pdf / mag_a / mag_b and the weight factor prefac[t]; the tables
+ * are shared.  out: ntemplates x (nx ny) FP64, template-major. */
+int sfh_binary_pairs_hist_batch(sfh_ctx* ctx, int64_t ntemplates, const int64_t* offsets, const double* mini,
+                                const double* imf_pdf, const double* mag_a, const double* mag_b, int y_is_a,
+                                const double* prefac, const sfh_table* compl_a, const sfh_table* compl_b,
+                                const sfh_table* bias_a, const sfh_table* bias_b, sfh_range xedges, sfh_range yedges,
+                                double* out);
 
 /* bin_cmd_smooth(colors, mags, color_err, mag_err, cov_mult; weights, edges)
  * -- src/StarFormationHistories.jl:574-621.  cov_mult in {-1,0,1} as the

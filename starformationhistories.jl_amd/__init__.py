@@ -26,7 +26,8 @@ from .mock import (BinaryMassRatio, PowerLawIMF, Salpeter1955, bin_cmd, generate
                    mass_limits, model_cmd, recycling_fraction, recycling_mass_fraction, sort_ingested,
                    surviving_fraction, surviving_mass_fraction)
 from .templates import (Histogram, NoBinaries, PointBatch, RandomBinaryPairs, Range,  # noqa: E402
-                        binary_hess, binary_mass_fraction, binary_number_fraction, binary_pairs_hist,
+                        binary_hess, binary_hess_batch, binary_mass_fraction, binary_number_fraction,
+                        binary_pairs_hist, binary_pairs_hist_batch,
                         binary_system_fraction, bin_cmd_smooth, calculate_edges, flux2mag, mag2flux, trapz,
                         calculate_weights, deposit_batch, edges_range, exp_photerr,
                         histogram_data, histogram_pix, interpolate_mini, Martin2016_complete,
@@ -43,7 +44,7 @@ __all__ = [
     "partial_cmd_smooth_points", "templates_from_device", "HMCModel", "LogTransformFTResult",
     "MCMCModel", "construct_x0", "fit_templates", "fit_templates_fast", "fit_templates_lbfgsb",
     "mcmc_sample", "renormalize_x0", "MultiFit", "StageInfo", "NoBinaries", "RandomBinaryPairs",
-    "binary_hess", "binary_mass_fraction", "binary_number_fraction", "binary_pairs_hist",
+    "binary_hess", "binary_hess_batch", "binary_pairs_hist_batch", "binary_mass_fraction", "binary_number_fraction", "binary_pairs_hist",
     "binary_system_fraction", "flux2mag", "mag2flux", "trapz", "hierarchical", "DomainError", "mock",
     "BinaryMassRatio", "PowerLawIMF", "Salpeter1955", "bin_cmd", "generate_stars_mag",
     "generate_stars_mag_composite", "generate_stars_mass", "generate_stars_mass_composite", "ingest_mags",

@@ -122,6 +122,9 @@ SIGNATURES = [
     ("sfh_binary_pairs_hist", _int,
      [_vp, _i64, _vp, _vp, _vp, _vp, _int, _dbl, C.POINTER(sfh_table), C.POINTER(sfh_table),
       C.POINTER(sfh_table), C.POINTER(sfh_table), sfh_range, sfh_range, _vp]),
+    ("sfh_binary_pairs_hist_batch", _int,
+     [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _int, _vp, C.POINTER(sfh_table), C.POINTER(sfh_table),
+      C.POINTER(sfh_table), C.POINTER(sfh_table), sfh_range, sfh_range, _vp]),
     ("sfh_bin_cmd_smooth", _int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _int, sfh_range, sfh_range, _vp]),
     ("sfh_generate_stars", _int, [_vp, _i64, _vp, C.c_int32, _vp, C.c_int32, _dbl, C.c_int32, _dbl, _dbl, C.c_int32,
                                   C.POINTER(sfh_imf), C.POINTER(sfh_binary_model), C.c_uint64, C.c_uint64,

@@ -463,6 +463,114 @@ def binary_hess(model, m_ini, mags, mag_err_funcs, y_index, color_indices, imf, 
     return bin_cmd_smooth(h_colors, h_mags, color_err, ymag_err, cov_mult, weights=hw[good], edges=edges, ctx=ctx)
 
 
+def binary_pairs_hist_batch(isochrones, imf, mag_filters, y_is_a, prefacs, completeness_funcs, bias_funcs, edges,
+                            ctx: Context | None = None) -> np.ndarray:
+    """The pair histograms of binary_hess for many isochrones in one launch
+    (sfh_binary_pairs_hist_batch): ``isochrones`` is a list of (m_ini, mags)
+    with mags indexed by filter, ``mag_filters`` the colour filters (a, b).
+    The completeness / bias callables are tabulated once over all the pairs'
+    magnitudes.  Returns (ntemplates, nx, ny) FP64."""
+    ctx = ctx or default_context()
+    a, b = mag_filters
+    ms = [np.asarray(m, dtype=np.float64) for m, _ in isochrones]
+    ga = [np.asarray(g[a], dtype=np.float64) for _, g in isochrones]
+    gb = [np.asarray(g[b], dtype=np.float64) for _, g in isochrones]
+    if not all(x.size == y.size == z.size for x, y, z in zip(ms, ga, gb)):
+        raise ArgumentError("all(length(i) == length(m_ini) for i in mags)")
+    nt = len(ms)
+    ex, ey = edges
+    nx, ny = ex.length - 1, ey.length - 1
+    out = np.zeros(nt * nx * ny)
+    if nt == 0:
+        return out.reshape(0, nx, ny)
+    imfv = [np.asarray(_apply(imf, m), dtype=np.float64) for m in ms]
+    offsets = np.concatenate([[0], np.cumsum([m.size for m in ms])]).astype(np.int64)
+    cat = lambda xs: np.ascontiguousarray(np.concatenate(xs), dtype=np.float64)
+    m_all, i_all, a_all, b_all = cat(ms), cat(imfv), cat(ga), cat(gb)
+    pre = np.ascontiguousarray(prefacs, dtype=np.float64)
+    tabs, keep = [], []
+    for mags, f_c, f_b in ((ga, completeness_funcs[0], bias_funcs[0]), (gb, completeness_funcs[1], bias_funcs[1])):
+        body = [g[:-1] for g in mags if g.size > 1]
+        if body:
+            lo, hi = min(float(g.min()) for g in body) - 0.76, max(float(g.max()) for g in body) + 1e-9
+        else:
+            lo, hi = 0.0, 1.0
+        for f in (f_c, f_b):
+            t, v = _table(f, lo, hi)
+            tabs.append(t)
+            keep.append(v)
+    ctx.check(ctx.lib.sfh_binary_pairs_hist_batch(ctx.handle, nt, offsets.ctypes.data, m_all.ctypes.data,
+                                                  i_all.ctypes.data, a_all.ctypes.data, b_all.ctypes.data,
+                                                  1 if y_is_a else 0, pre.ctypes.data, C.byref(tabs[0]),
+                                                  C.byref(tabs[2]), C.byref(tabs[1]), C.byref(tabs[3]), ex.as_c(),
+                                                  ey.as_c(), out.ctypes.data))
+    return out.reshape(nt, ny, nx).transpose(0, 2, 1)
+
+
+class _BinPoints:
+    """Kernel points of a binary template (bin centres of its pair histogram)."""
+
+    def __init__(self, colors, mags, color_err, mag_err, weights, cov_mult):
+        self.colors, self.mags, self.color_err, self.mag_err = colors, mags, color_err, mag_err
+        self.weights, self.cov_mult = weights, cov_mult
+
+
+def binary_hess_batch(model, isochrones, mag_err_funcs, y_index, color_indices, imf, completeness_funcs=None,
+                      bias_funcs=None, edges=None, *, normalize_value=1.0, mean_mass=None, out=None,
+                      ctx: Context | None = None):
+    """binary_hess(::RandomBinaryPairs, ...) (src/StarFormationHistories.jl:627-706)
+    for many isochrones ((m_ini, mags) pairs, mags indexed by filter): every
+    pair histogram in one launch, then the non-empty bins of all of them
+    deposited in one batched launch.  Returns an FP32 device tensor
+    (ntemplates x nbins), as deposit_batch does."""
+    import torch
+    if not isinstance(model, RandomBinaryPairs):
+        raise NotImplementedError(f"binary_hess(model::{type(model).__name__}, ...) not implemented.")
+    ctx = ctx or default_context()
+    c0, c1 = color_indices
+    if y_index not in color_indices:
+        raise NotImplementedError("`binary_hess(model::RandomBinaryPairs, ...)` for "
+                                  "`!(y_index in color_indices)` not yet implemented.")
+    nf = len(isochrones[0][1]) if isochrones else 2
+    if completeness_funcs is None:
+        completeness_funcs = [lambda m: np.ones_like(np.asarray(m, dtype=np.float64))] * nf
+    if bias_funcs is None:
+        bias_funcs = [lambda m: np.zeros_like(np.asarray(m, dtype=np.float64))] * nf
+    if mean_mass is None:
+        mean_mass = imf_mean(imf)
+    ex, ey = edges
+    nx, ny = ex.length - 1, ey.length - 1
+    pre = [normalize_value / mean_mass / 4 / trapz(m, _apply(imf, np.asarray(m, dtype=np.float64)))
+           for m, _ in isochrones]                                           # :642-643
+    hist = binary_pairs_hist_batch(isochrones, imf, (c0, c1), y_index == c0, pre,
+                                   [completeness_funcs[c0], completeness_funcs[c1]],
+                                   [bias_funcs[c0], bias_funcs[c1]], edges, ctx=ctx)
+    cx = ex.first + np.arange(nx) * ex.step + ex.step / 2                    # :685-686
+    cy = ey.first + np.arange(ny) * ey.step + ey.step / 2
+    x_c_idx = c1 if c0 == y_index else c0
+    cov_mult = -1 if y_index == c0 else 1
+    ycol = _apply(mag_err_funcs[y_index], cy)
+    pts = []
+    for t in range(len(isochrones)):
+        hw = hist[t].ravel(order="F")
+        good = np.flatnonzero(hw != 0)                                       # :683
+        h_colors = np.tile(cx, ny)[good]
+        h_mags = np.repeat(cy, nx)[good]
+        xmags = h_mags - h_colors if y_index == c0 else h_mags + h_colors    # :693-697
+        color_err = _apply(mag_err_funcs[x_c_idx], xmags) if good.size else np.zeros(0)
+        pts.append(_BinPoints(h_colors, h_mags, color_err, np.repeat(ycol, nx)[good], hw[good], cov_mult))
+    nt = len(pts)
+    if out is None:
+        out = torch.zeros((nt, nx * ny), dtype=torch.float32, device="cuda")
+    if sum(p.colors.size for p in pts) == 0:
+        out.zero_()
+        torch.cuda.synchronize()
+        return out
+    deposit_batch(PointBatch(pts, edges), out=out, ctx=ctx)
+    ctx.synchronize()
+    return out
+
+
 class PointBatch:
     """Concatenated kernel points of many templates, resident on the device
     (torch tensors), ready for ``deposit_batch``."""

@@ -88,3 +88,38 @@ def test_binary_model_edges(sfh, gpu_ctx):
     with pytest.raises(NotImplementedError):
         sfh.binary_hess(sfh.RandomBinaryPairs(0.5), sp.new_mini, sp.new_iso_mags + [sp.new_iso_mags[0]],
                         [synth.photerr] * 3, 2, (0, 1), imf, edges=edges)
+
+
+def test_binary_hess_batch_matches_single_templates(sfh, gpu_ctx):
+    """Many isochrones in one pair-histogram launch and one deposition launch
+    give the per-template binary_hess results (tables shared over all the
+    isochrones' magnitudes; FP32 templates), including an isochrone entirely
+    off the grid (an empty template)."""
+    from sfh_amd import synth
+    edges = synth.cfg1_edges()
+    imf = synth.Kroupa2001(0.08, 120.0)
+    isos = []
+    for la, mh, dm in [(8.0, -1.0, 24.0), (9.0, -0.5, 25.0), (10.0, -2.0, 26.0), (9.5, -1.5, 45.0), (7.2, 0.0, 27.0)]:
+        m, (B, V) = synth.isochrone(la, mh, 500, 3, mmax_imf=120.0)
+        isos.append((m, [B + dm, V + dm]))
+    errs = [synth.photerr, synth.photerr]
+    comp = [synth.completeness, lambda v: synth.completeness(v, 0.4)]
+    model = sfh.RandomBinaryPairs(0.3)
+    T = sfh.binary_hess_batch(model, isos, errs, 1, (0, 1), imf, comp, None, edges,
+                              normalize_value=1e3).cpu().numpy().astype(np.float64)
+    H = sfh.binary_pairs_hist_batch(isos, imf, (0, 1), False,
+                                    [1e3 / imf.mean() / 4 / sfh.trapz(m, imf(m)) for m, _ in isos], comp,
+                                    [lambda v: 0 * v] * 2, edges)
+    for t, (m, g) in enumerate(isos):
+        ref = sfh.binary_hess(model, m, g, errs, 1, (0, 1), imf, comp, None, edges,
+                              normalize_value=1e3).weights.ravel(order="F")
+        href = sfh.binary_pairs_hist(m, imf(m), g[0], g[1], False, 1e3 / imf.mean() / 4 / sfh.trapz(m, imf(m)),
+                                     comp, [lambda v: 0 * v] * 2, edges)
+        if t == 3:
+            assert np.all(T[t] == 0) and np.all(ref == 0) and np.all(H[t] == 0)
+            continue
+        # the shared tables span every isochrone (up to the one at dmod 45), so
+        # their 16385 points are ~3x coarser than a single template's: the
+        # linear interpolation of the steep completeness differs by ~1e-7
+        assert np.max(np.abs(H[t] - href)) <= 1e-6 * np.max(np.abs(href))
+        assert np.max(np.abs(T[t] - ref)) <= 1e-5 * np.max(np.abs(ref))
