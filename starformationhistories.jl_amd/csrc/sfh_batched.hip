@@ -338,6 +338,8 @@ __device__ inline void wait_vmcnt(int n)
 {
     switch (n) {
     case 0: __builtin_amdgcn_s_waitcnt(0xF70); break;
+    case 2: __builtin_amdgcn_s_waitcnt(0xF72); break;
+    case 3: __builtin_amdgcn_s_waitcnt(0xF73); break;
     case 4: __builtin_amdgcn_s_waitcnt(0xF74); break;
     case 6: __builtin_amdgcn_s_waitcnt(0xF76); break;
     case 8: __builtin_amdgcn_s_waitcnt(0xF78); break;
@@ -418,25 +420,36 @@ struct EpiGrad {
     int64_t Bpad, Tpad2;
 };
 
-template <int M, bool LL>
-__global__ __launch_bounds__(kWG, 1) void k_gemm(const uint4* __restrict__ A_, const uint4* __restrict__ B_,
-                                                 int64_t nrb_total, int64_t ncb_total, int64_t kb_per_split,
-                                                 int64_t nkb_total, Swizzle sw, EpiLL el, EpiGrad eg)
+// Workgroup tile (32 kTR WR) x (32 kTC WC): WR x WC waves of 64 x 64.  The
+// composite forms use 2 x 2 waves (128 x 128); the split-K gradient uses
+// 4 x 2 waves (256 templates x 128 walkers, 512 threads, one workgroup per
+// CU): a quarter fewer LDS and L2 bytes per matrix-core product and half the
+// re-reads of the R operand, at the same two waves per SIMD.
+// Chunks of rowblocks / column blocks past the operand (partial edge tiles)
+// are read from the last valid block and their results are not written.
+template <int M, bool LL, int WR = kWR, int WC = kWC>
+__global__ __launch_bounds__(WR * WC * 64, 1) void k_gemm(const uint4* __restrict__ A_, const uint4* __restrict__ B_,
+                                                          int64_t nrb_total, int64_t ncb_total, int64_t kb_per_split,
+                                                          int64_t nkb_total, Swizzle sw, EpiLL el, EpiGrad eg)
 {
+    constexpr int NW = WR * WC;                 // waves
+    constexpr int RB = kTR * WR;                // 32-row blocks per tile
+    constexpr int CBK = kTC * WC;               // 32-column blocks per tile
+    constexpr int BN = 32 * CBK;
     constexpr int PA = pieces_a(M);
     constexpr int PB = pieces_b(M);
-    constexpr int CA = kRB * PA;                // A chunks (1 KiB) per K-block
-    constexpr int CB = kCB * PB;                // B chunks per K-block
+    constexpr int CA = RB * PA;                 // A chunks (1 KiB) per K-block
+    constexpr int CB = CBK * PB;                // B chunks per K-block
     constexpr int CH = CA + CB;                 // chunks per stage
-    constexpr int LPW = CH / 4;                 // DMA instructions per wave per stage
+    constexpr int LPW = CH / NW;                // DMA instructions per wave per stage
     // ring depth: the split-K gradient streams R from HBM and wants a deep ring;
     // the composite GEMMs hit L2 and prefer the occupancy of a shallow one
     constexpr int S = (M == GRAD_BF16 || M == GRAD_FP32) ? 4 : 2;
     constexpr int KP = (M == GRAD_BF16 || M == GRAD_FP32) ? 8 : 0;
-    static_assert(CH % 4 == 0, "stage size");
+    static_assert(CH % NW == 0, "stage size");
 
     __shared__ u32x4 ring[S][CH * 64];
-    __shared__ double sLL[kWR][kBN];
+    __shared__ double sLL[LL ? WR : 1][LL ? BN : 1];
     const u32x4* A = reinterpret_cast<const u32x4*>(A_);
     const u32x4* Bm = reinterpret_cast<const u32x4*>(B_);
 
@@ -448,7 +461,7 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm(const uint4* __restrict__ A_, c
     if (kb1 > nkb_total) kb1 = nkb_total;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave / kWC, wc = wave % kWC;
+    const int wr = wave / WC, wc = wave % WC;
 
     f32x16 acc[kTR][kTC];
     double acc64[kTR][kTC][16];
@@ -469,8 +482,16 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm(const uint4* __restrict__ A_, c
 #pragma unroll
         for (int i = 0; i < LPW; ++i) {
             const int c = wave * LPW + i;
-            const u32x4* g = c < CA ? A + ((kb * nrb_total + (int64_t)rt * kRB) * PA + c) * 64
-                                    : Bm + ((kb * ncb_total + (int64_t)ct * kCB) * PB + (c - CA)) * 64;
+            const u32x4* g;
+            if (c < CA) {
+                int64_t rb = (int64_t)rt * RB + c / PA;
+                if (rb >= nrb_total) rb = nrb_total - 1;
+                g = A + ((kb * nrb_total + rb) * PA + c % PA) * 64;
+            } else {
+                int64_t cb = (int64_t)ct * CBK + (c - CA) / PB;
+                if (cb >= ncb_total) cb = ncb_total - 1;
+                g = Bm + ((kb * ncb_total + cb) * PB + (c - CA) % PB) * 64;
+            }
             dma_chunk(g, st + c * 64, lane);
         }
     };
@@ -553,10 +574,10 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm(const uint4* __restrict__ A_, c
         for (int b = 0; b < kTC; ++b) part[b] = 0.0;
 #pragma unroll
         for (int a = 0; a < kTR; ++a) {
-            const int64_t rbk = (int64_t)rt * kRB + wr * kTR + a;   // 32-bin block
+            const int64_t rbk = (int64_t)rt * RB + wr * kTR + a;   // 32-bin block
 #pragma unroll
             for (int b = 0; b < kTC; ++b) {
-                const int64_t cbk = (int64_t)ct * kCB + wc * kTC + b;
+                const int64_t cbk = (int64_t)ct * CBK + wc * kTC + b;
                 const int64_t w = cbk * 32 + col;
                 const bool wok = w < el.B;
                 float rv[16];
@@ -602,20 +623,22 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm(const uint4* __restrict__ A_, c
             if (h == 0) sLL[wr][(wc * kTC + b) * 32 + col] = part[b];
         }
         __syncthreads();
-        for (int k = tid; k < kBN; k += kWG) {
+        for (int k = tid; k < BN; k += NW * 64) {
             double s = 0.0;
 #pragma unroll
-            for (int r = 0; r < kWR; ++r) s += sLL[r][k];
-            el.ll_part[(int64_t)rt * el.Bpad + (int64_t)ct * kBN + k] = s;
+            for (int r = 0; r < WR; ++r) s += sLL[r][k];
+            el.ll_part[(int64_t)rt * el.Bpad + (int64_t)ct * BN + k] = s;
         }
     } else {
         // rows = templates, cols = walkers; write FP32 partials [split][walker][template]
 #pragma unroll
         for (int a = 0; a < kTR; ++a) {
-            const int64_t rbk = (int64_t)rt * kRB + wr * kTR + a;
+            const int64_t rbk = (int64_t)rt * RB + wr * kTR + a;
+            if (rbk * 32 >= eg.Tpad2) continue;
 #pragma unroll
             for (int b = 0; b < kTC; ++b) {
-                const int64_t w = ((int64_t)ct * kCB + wc * kTC + b) * 32 + col;
+                const int64_t w = ((int64_t)ct * CBK + wc * kTC + b) * 32 + col;
+                if (w >= eg.Bpad) continue;
                 float* o = eg.gpart + (split * eg.Bpad + w) * eg.Tpad2 + rbk * 32 + 4 * h;
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
@@ -986,24 +1009,38 @@ static int fg_batched_impl(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_dat
 
     int nsplit = 1;
     if (d_G) {
+        // 256 x 128 tiles (4 x 2 waves), one 512-thread workgroup per CU; the
+        // bins are split so the workgroups fill whole waves of the 256 CUs
+        constexpr int GWR = 4, GWC = 2, GRB = kTR * GWR, GCB = kTC * GWC;
         const int64_t nrb2 = ops->tpad2 / 32;
-        const int nrt = (int)(ops->tpad2 / kBM);
-        // enough workgroups for the chip: split the bins
-        nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(cdiv(2048, (int64_t)nrt * nct), nkb2 / 8));
-        const int64_t kps = cdiv(nkb2, nsplit);
+        const int nrt = (int)cdiv(nrb2, GRB), nct2 = (int)cdiv(ncb, GCB);
+        const int64_t tiles = (int64_t)nrt * nct2;
+        int best = 1;
+        double best_cost = 1e300;
+        for (int ns = 1; ns <= 64 && ns <= std::max<int64_t>(1, nkb2 / 32); ++ns) {
+            const int64_t wg = tiles * ns;
+            const double waves = (double)cdiv(wg, 256);
+            // time ~ waves of workgroups x K-blocks per workgroup
+            const double cost = waves * (double)cdiv(nkb2, ns);
+            if (cost < best_cost * 0.999) {
+                best_cost = cost;
+                best = ns;
+            }
+        }
+        const int64_t kps = cdiv(nkb2, best);
         nsplit = (int)cdiv(nkb2, kps);
         if (int rc = scratch(ctx, "bat_g", (size_t)nsplit * Bpad * ops->tpad2 * sizeof(float), &pg)) return rc;
-        const Swizzle sw = make_swizzle(nrt, (int)nct, nrt, std::max(1, 32 / nrt));
+        const Swizzle sw = make_swizzle(nrt, nct2, nrt, std::max(1, 8 / nrt));
         EpiLL el{};
         EpiGrad eg{(float*)pg, Bpad, ops->tpad2};
         const dim3 grid(swizzle_grid(sw), (unsigned)nsplit);
         TimedLaunch tl(ctx, "batched_gradient");
         if (mode == FP32)
-            hipLaunchKernelGGL((k_gemm<GRAD_FP32, false>), grid, dim3(kWG), 0, ctx->stream, ops->a2[1],
-                               (const uint4*)pr, nrb2, ncb, kps, nkb2, sw, el, eg);
+            hipLaunchKernelGGL((k_gemm<GRAD_FP32, false, GWR, GWC>), grid, dim3(GWR * GWC * 64), 0, ctx->stream,
+                               ops->a2[1], (const uint4*)pr, nrb2, ncb, kps, nkb2, sw, el, eg);
         else
-            hipLaunchKernelGGL((k_gemm<GRAD_BF16, false>), grid, dim3(kWG), 0, ctx->stream, ops->a2[0],
-                               (const uint4*)pr, nrb2, ncb, kps, nkb2, sw, el, eg);
+            hipLaunchKernelGGL((k_gemm<GRAD_BF16, false, GWR, GWC>), grid, dim3(GWR * GWC * 64), 0, ctx->stream,
+                               ops->a2[0], (const uint4*)pr, nrb2, ncb, kps, nkb2, sw, el, eg);
         if (int rc = check_launch(ctx, "gradient GEMM")) return rc;
     }
     {
