@@ -10,7 +10,11 @@
 //   data    : FP64 n_i, length nbins.
 //   partial : FP64 composite partial sums, nsplit x ld (template split-K).
 //
-// Kernels (one fg! = 4 launches, each an HBM stream or a tiny reduction)
+// fg! and loglikelihood from coefficients with T <= 704 run in one pass over
+// the templates (k_single_fg + k_single_reduce, sfh_multifit.hip).  The
+// kernels here serve composite!, ∇loglikelihood(!) from a given composite and
+// fg! of larger template sets (4 launches, each an HBM stream or a tiny
+// reduction):
 //   k_composite_partial : m_s[i] = sum_{j in split s} C[i,j] r_j      (streams C once)
 //   k_composite_finish  : m_i = sum_s m_s[i]; LL term; R_i = 1-n_i/max(m_i,eps)
 //   k_grad_partial      : g[j,chunk] = sum_{i in chunk} C[i,j] R_i      (streams C once)
@@ -355,6 +359,10 @@ static int run_fg(sfh_ctx* ctx, const sfh_templates* tpl, const FgArgs& a)
     const int64_t nbins = a.nbins;
     const int64_t ld = ld_of(nbins);
     const int64_t T = tpl ? tpl->ntemplates : 0;
+    // fg! from coefficients: one pass over the templates where the shape allows
+    if (a.d_coeffs && a.d_data && (a.want_ll || a.d_G) && single_pass_covers(tpl))
+        return single_pass_fg(ctx, tpl, a.d_data, a.d_coeffs, a.eps, a.want_ll, a.d_negLL, a.d_ll, a.d_G, a.gsign,
+                              a.d_comp_out, a.clobber_with_R);
     int nsplit = 1;
     const double* part = a.d_comp_in;
     if (a.d_coeffs) {

@@ -123,6 +123,14 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // Frees the batched-path operand layouts built for a template set.
 void release_batched(const sfh_templates* t);
 
+// fg! of one template set in one pass over its column-major templates (the
+// grouped fg! kernel of sfh_multifit.hip with one fit).  single_pass_covers
+// says whether the shape fits it (1..1024 templates, 32-bit offsets).
+bool single_pass_covers(const sfh_templates* tpl);
+int single_pass_fg(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_data, const double* d_coeffs, double eps,
+                   int want_ll, double* d_negLL, double* d_ll, double* d_G, double gsign, double* d_comp_out,
+                   int clobber_with_R);
+
 // Launchers implemented per translation unit.
 int launch_composite(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_coeffs,
                      double* d_partials, int nsplit);

@@ -42,6 +42,8 @@ constexpr int kWG = 256;
 constexpr int kSB = 32;                  // bins per block: one 128-byte segment of a column
 constexpr int kQ = kSB / 4;              // float4 quads per block: thread (slice s, quad q) = 8 s + q
 constexpr int kMaxT = 1024;
+constexpr int kSingleNW = 8;              // waves per workgroup of the single-fit fg!
+constexpr int kSingleMaxK = 11;           // its widest variant (T <= 704) that does not spill
 
 // Pointers read from a FitDesc in memory are generic (flat) to the compiler;
 // flat loads also count against the LDS counter, so every LDS access would
@@ -54,13 +56,18 @@ constexpr int kMaxMem = 32;
 
 struct FitDesc {
     const float* C;      // ld x T templates, column-major (the caller's)
-    const float* Cb;     // the group's blocked copy: [32-bin block][T][32 bins]
+    const float* Cb;     // templates as fg! reads them: a group's blocked copy
+                         // [32-bin block][T][32 bins], or C itself for one fit
     const double* n;     // data
     int64_t ld, N, T;
     int64_t xoff;        // offset of the fit's T-vectors
     int64_t hoff;        // offset of its T x T matrix
     int64_t item0, nitems;
     int64_t poff;        // offset of its gradient partials (nitems x T)
+    int64_t tstride;     // Cb elements from one template to the next (32, or ld)
+    int64_t bstride;     // Cb elements from one 32-bin block to the next (32 T, or 32)
+    double* rout;        // optional per-bin output: the composite m, or R when rout_R
+    int32_t rout_R;
 };
 
 struct Item {
@@ -115,6 +122,7 @@ __device__ __forceinline__ void fg_block(const FitDesc& fd, int64_t b0, double e
             const double n = nv;
             ll += (n > 0.0) ? (n - cl - n * log(n / cl)) : -cl;
             r = 1.0 - n / cl;
+            if (fd.rout) fd.rout[bin] = fd.rout_R ? r : m;
         }
         sr[tid] = r;
     }
@@ -148,14 +156,16 @@ __device__ __forceinline__ void fg_item(const FitDesc& fd, const Item& item, int
     constexpr int SL = 8 * NW;    // column slices
     const int tid = threadIdx.x, q = tid & (kQ - 1), s = tid >> 3, wave = tid >> 6;
     const int T = (int)fd.T;
-    // element offsets of this thread's columns inside a block of the blocked copy
+    // element offsets of this thread's columns inside a block (the host keeps
+    // (T - 1) tstride + 32 below 2^31)
+    const int32_t tstride = (int32_t)fd.tstride;
     int32_t col[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int t = s + SL * k;
-        col[k] = (t < T ? t : T - 1) * kSB + 4 * q;
+        col[k] = (t < T ? t : T - 1) * tstride + 4 * q;
     }
-    const int64_t bstride = (int64_t)T * kSB;
+    const int64_t bstride = fd.bstride;
     gfloat* blk0 = (gfloat*)fd.Cb + (item.bin0 / kSB) * bstride;
     gdouble* nd = (gdouble*)fd.n;
     const int last = item.nblk - 1;
@@ -805,7 +815,153 @@ void launch_fg_t(hipStream_t s, int grid, const FitDesc* fits, const Item* items
                        want_G, llp, gp);
 }
 
+// ------------------------------------------- one fit, templates read in place
+
+// fg! of one template set in one pass over its templates: the grouped
+// kernel's work items over the caller's column-major templates (template
+// stride ld, block stride 32 -- each thread's float4 is still one 16-byte
+// piece of a 128-byte run of one template), item `it` = blocks
+// [it nblk / nitems, (it + 1) nblk / nitems) (one item per resident
+// workgroup, lengths differing by at most one block), K = the column count T
+// needs, the coefficients staged once per workgroup.
+template <int NW, int K>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(fg_waves_per_eu(K))))
+void k_single_fg(FitDesc fd, int64_t nitems, const double* __restrict__ X, double eps, int want_G,
+                 double* __restrict__ ll_part, double* __restrict__ g_part)
+{
+    constexpr int NT = 64 * NW, SL = 8 * NW;
+    __shared__ double sx[K * SL];
+    __shared__ double sm[NW][kSB];
+    __shared__ double sr[kSB];
+    for (int t = threadIdx.x; t < K * SL; t += NT) sx[t] = t < fd.T ? ((gdouble*)X)[t] : 0.0;
+    __syncthreads();
+    const int64_t nblk = (fd.N + kSB - 1) / kSB;
+    for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int64_t b0 = it * nblk / nitems, b1 = (it + 1) * nblk / nitems;
+        const Item item{b0 * kSB, 0, (int32_t)(b1 - b0)};
+        fg_item<K, NW>(fd, item, it, eps, want_G, sx, sm, sr, ll_part, g_part);
+    }
+}
+
+// Sums of the single-fit partials in a fixed order.  Workgroup w: templates
+// 16 w .. 16 w + 15, thread = 16 item slices x 16 templates (each slice sums
+// items slice, slice + 16, ... in order; slices then add in order).
+// Workgroup 0 also sums the LL partials: LL = -Inf for an exact zero sum
+// (fitting_base.jl:95), negLL = -LL.
+__global__ __launch_bounds__(kWG) void k_single_reduce(const double* __restrict__ ll_part,
+                                                       const double* __restrict__ g_part, int64_t nitems, int64_t T,
+                                                       int want_ll, double* __restrict__ negLL,
+                                                       double* __restrict__ ll_out, double gsign,
+                                                       double* __restrict__ G)
+{
+    __shared__ double sh[kWG];
+    __shared__ double red[4];
+    const int tt = threadIdx.x & 15, sl = threadIdx.x >> 4;
+    if (G) {
+        const int64_t t = (int64_t)blockIdx.x * 16 + tt;
+        double v = 0.0;
+        if (t < T) {
+#pragma unroll 8
+            for (int64_t i = sl; i < nitems; i += 16) v += g_part[i * T + t];
+        }
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        if (sl == 0 && t < T) {
+            double g = sh[tt];
+#pragma unroll
+            for (int k = 1; k < 16; ++k) g += sh[16 * k + tt];
+            G[t] = gsign * g;
+        }
+    }
+    if (want_ll && blockIdx.x == 0) {
+        double v = 0.0;
+        for (int64_t i = threadIdx.x; i < nitems; i += kWG) v += ll_part[i];
+        const double s = block_sum(v, red);
+        if (threadIdx.x == 0) {
+            const double ll = (s != 0.0) ? s : -INFINITY;
+            if (negLL) *negLL = -ll;
+            if (ll_out) *ll_out = ll;
+        }
+    }
+}
+
+template <int K>
+void launch_single_t(hipStream_t s, int cus, const FitDesc& fd, int64_t nblk, const double* X, double eps,
+                     int want_G, double* llp, double* gp, int64_t* nitems_out)
+{
+    // workgroups resident per CU: two of 8 waves where K leaves <= 128 VGPRs
+    const int64_t grid = (int64_t)cus * (K <= 4 ? 2 : 1);
+    const int64_t nitems = nblk < grid ? nblk : grid;
+    *nitems_out = nitems;
+    hipLaunchKernelGGL((k_single_fg<kSingleNW, K>), dim3((unsigned)nitems), dim3(64 * kSingleNW), 0, s, fd, nitems,
+                       X, eps, want_G, llp, gp);
+}
+
 }  // namespace mf
+
+bool single_pass_covers(const sfh_templates* tpl)
+{
+    const int64_t ld = cdiv(tpl->nbins, 256) * 256;
+    return tpl->ntemplates >= 1 && tpl->ntemplates <= 8 * mf::kSingleNW * mf::kSingleMaxK && tpl->nbins >= 1 &&
+           (tpl->ntemplates - 1) * ld + mf::kSB < ((int64_t)1 << 31);
+}
+
+int single_pass_fg(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_data, const double* d_coeffs, double eps,
+                   int want_ll, double* d_negLL, double* d_ll, double* d_G, double gsign, double* d_comp_out,
+                   int clobber_with_R)
+{
+    using namespace mf;
+    static int cus_of[64] = {};
+    if (ctx->device < 0 || ctx->device >= 64) return set_error(ctx, SFH_ERR_ARGUMENT, "device index out of range");
+    int cus = __atomic_load_n(&cus_of[ctx->device], __ATOMIC_RELAXED);
+    if (!cus) {
+        SFH_HIP_TRY(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        __atomic_store_n(&cus_of[ctx->device], cus, __ATOMIC_RELAXED);
+    }
+    const int64_t T = tpl->ntemplates, N = tpl->nbins;
+    FitDesc fd{};
+    fd.C = fd.Cb = tpl->d_models;
+    fd.n = d_data;
+    fd.ld = cdiv(N, 256) * 256;
+    fd.N = N;
+    fd.T = T;
+    fd.tstride = fd.ld;
+    fd.bstride = kSB;
+    fd.rout = d_comp_out;
+    fd.rout_R = clobber_with_R;
+    // one item (a run of blocks, the next block's loads in flight) per
+    // resident workgroup; partials sized for two per CU
+    const int64_t nblk = cdiv(N, kSB);
+    const int64_t cap = std::min<int64_t>(nblk, 2 * (int64_t)cus);
+    void *llp = nullptr, *gp = nullptr;
+    if (int rc = scratch(ctx, "fg1_llpart", (size_t)cap * sizeof(double), &llp)) return rc;
+    if (d_G)
+        if (int rc = scratch(ctx, "fg1_gpart", (size_t)cap * T * sizeof(double), &gp)) return rc;
+    const int want_G = d_G ? 1 : 0;
+    const int need = (int)cdiv(T, 8 * kSingleNW);
+    int64_t nitems = 0;
+    {
+        TimedLaunch tl(ctx, "single_fg");
+        hipStream_t s = ctx->stream;
+        double* l = (double*)llp;
+        double* g = (double*)gp;
+        if (need <= 1) launch_single_t<1>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
+        else if (need <= 2) launch_single_t<2>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
+        else if (need <= 3) launch_single_t<3>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
+        else if (need <= 4) launch_single_t<4>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
+        else if (need <= 6) launch_single_t<6>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
+        else if (need <= 8) launch_single_t<8>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
+        else launch_single_t<kSingleMaxK>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
+    }
+    if (int rc = check_launch(ctx, "k_single_fg")) return rc;
+    if (want_ll || d_G) {
+        TimedLaunch tl(ctx, "single_reduce");
+        hipLaunchKernelGGL(k_single_reduce, dim3((unsigned)(d_G ? cdiv(T, 16) : 1)), dim3(kWG), 0, ctx->stream,
+                           (const double*)llp, (const double*)gp, nitems, T, want_ll, d_negLL, d_ll, gsign, d_G);
+    }
+    return check_launch(ctx, "k_single_reduce");
+}
+
 }  // namespace sfh
 
 using namespace sfh;
@@ -970,6 +1126,10 @@ int sfh_multifit_create(sfh_ctx* ctx, int64_t nfits, const sfh_templates* const*
         fd.ld = cdiv(t->nbins, 256) * 256;
         fd.N = t->nbins;
         fd.T = t->ntemplates;
+        fd.tstride = kSB;
+        fd.bstride = fd.T * kSB;
+        fd.rout = nullptr;
+        fd.rout_R = 0;
         fd.xoff = mf->sumT;
         fd.hoff = mf->sumT2;
         mf->sumT += fd.T;

@@ -53,7 +53,8 @@ def test_mpmath_golden(sfh, gpu_ctx):
     assert np.allclose(G, g["grad"], rtol=1e-12)
 
 
-@pytest.mark.parametrize("N,T", [(60000, 700), (7500, 50), (1, 1), (257, 3), (100003, 13)])
+# 700 and below run fg! in one pass over the templates; 900 takes the two-pass kernels
+@pytest.mark.parametrize("N,T", [(60000, 700), (7500, 50), (1, 1), (257, 3), (100003, 13), (20000, 900), (3000, 704)])
 def test_fg_against_oracle(sfh, O, gpu_ctx, N, T):
     from sfh_amd import synth
     models = synth.synthetic_templates(N, T, seed=N + T)
@@ -70,6 +71,35 @@ def test_fg_against_oracle(sfh, O, gpu_ctx, N, T):
     assert np.max(np.abs(G - rG)) <= 1e-10 * np.max(np.abs(rG)) + 1e-300
     # the clobbered composite R = 1 - n/c: absolute error ~ ulp(n/c)
     assert np.max(np.abs(comp - rcomp)) <= 1e-13 * max(1.0, np.max(np.abs(1.0 - rcomp)))
+
+
+@pytest.mark.parametrize("T", [40, 700, 900])
+def test_fg_output_combinations(sfh, O, gpu_ctx, T):
+    """fg! with F only, G only and both agree; the composite output holds m
+    without a gradient and R = 1 - n/max(m, eps) with one (solvers.jl:26-37)."""
+    from sfh_amd import synth
+    N = 9000
+    models = synth.synthetic_templates(N, T, seed=T)
+    x = np.random.default_rng(T).uniform(0.1, 2.0, T) * (1e5 / T)
+    data = synth.poisson_data(models.astype(np.float64) @ x, seed=3)
+    tpl = sfh.HIPTemplates(models)
+    hd = sfh.HIPData(data)
+    G = np.empty(T)
+    comp_R = np.empty(N)
+    nl = sfh.fg_(True, G, x, tpl, hd, comp_R)
+    G2 = np.empty(T)
+    # the G-only branch returns G, the value of its last expression `G .*= -1` (solvers.jl:31-33)
+    assert sfh.fg_(None, G2, x, tpl, hd) is G2
+    assert np.array_equal(G, G2)
+    comp_m = np.empty(N)
+    assert sfh.fg_(True, None, x, tpl, hd, comp_m) == nl
+    m = O.composite(models, x)
+    assert np.max(np.abs(comp_m - m)) <= 1e-13 * np.max(m)
+    rnl, rG, rcomp = O.fg(models, data, x, nthreads=8)
+    assert nl == pytest.approx(rnl, rel=1e-12)
+    assert np.max(np.abs(G - rG)) <= 1e-10 * np.max(np.abs(rG))
+    assert np.max(np.abs(comp_R - rcomp)) <= 1e-13 * max(1.0, np.max(np.abs(1.0 - rcomp)))
+    assert sfh.loglikelihood(x, tpl, hd) == -nl
 
 
 def test_device_path_matches_host_path(sfh, O, gpu_ctx):
