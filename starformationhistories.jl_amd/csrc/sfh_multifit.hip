@@ -56,18 +56,14 @@ constexpr int kMaxMem = 32;
 
 struct FitDesc {
     const float* C;      // ld x T templates, column-major (the caller's)
-    const float* Cb;     // templates as fg! reads them: a group's blocked copy
-                         // [32-bin block][T][32 bins], or C itself for one fit
+    const float* Cb;     // templates as fg! reads them: the group's blocked copy
+                         // [32-bin block][T][32 bins] (a single fit: C itself)
     const double* n;     // data
     int64_t ld, N, T;
     int64_t xoff;        // offset of the fit's T-vectors
     int64_t hoff;        // offset of its T x T matrix
     int64_t item0, nitems;
     int64_t poff;        // offset of its gradient partials (nitems x T)
-    int64_t tstride;     // Cb elements from one template to the next (32, or ld)
-    int64_t bstride;     // Cb elements from one 32-bin block to the next (32 T, or 32)
-    double* rout;        // optional per-bin output: the composite m, or R when rout_R
-    int32_t rout_R;
 };
 
 struct Item {
@@ -80,10 +76,11 @@ struct Item {
 
 // One block of one work item on registers c (already loaded): composite,
 // LL and R on 32 threads, gradient accumulation into g.
-template <int K, int NW>
+// OUT: also write the composite m (or R when rout_R) of every bin to rout.
+template <int K, int NW, bool OUT = false>
 __device__ __forceinline__ void fg_block(const FitDesc& fd, int64_t b0, double eps, int want_G, const double* sx,
                                          double (*sm)[kSB], double* sr, f4 (&c)[K], double nv, double (&g)[K],
-                                         double& ll)
+                                         double& ll, double* rout = nullptr, int rout_R = 0)
 {
     constexpr int SL = 8 * NW;
     const int tid = threadIdx.x, q = tid & (kQ - 1), s = tid >> 3, lane = tid & 63, wave = tid >> 6;
@@ -122,7 +119,9 @@ __device__ __forceinline__ void fg_block(const FitDesc& fd, int64_t b0, double e
             const double n = nv;
             ll += (n > 0.0) ? (n - cl - n * log(n / cl)) : -cl;
             r = 1.0 - n / cl;
-            if (fd.rout) fd.rout[bin] = fd.rout_R ? r : m;
+            if constexpr (OUT) {
+                if (rout) rout[bin] = rout_R ? r : m;
+            }
         }
         sr[tid] = r;
     }
@@ -148,24 +147,30 @@ __device__ __forceinline__ void fg_block(const FitDesc& fd, int64_t b0, double e
 // past the item's last are clamped to it -- and two register sets alternate,
 // so the loads of the next block are in flight while a block is processed and
 // each wait covers only the older set.
-template <int K, int NW>
+// COLMAJOR: the templates are the caller's ld x T column-major matrix (one
+// fit read in place: template stride ld, block stride 32) instead of the
+// blocked copy (template stride 32, block stride 32 T), and the per-bin
+// composite output of fg_block is on.
+template <int K, int NW, bool COLMAJOR = false>
 __device__ __forceinline__ void fg_item(const FitDesc& fd, const Item& item, int64_t it, double eps, int want_G,
                                         const double* sx, double (*sm)[kSB], double* sr, double* __restrict__ ll_part,
-                                        double* __restrict__ g_part)
+                                        double* __restrict__ g_part, double* rout = nullptr, int rout_R = 0)
 {
     constexpr int SL = 8 * NW;    // column slices
     const int tid = threadIdx.x, q = tid & (kQ - 1), s = tid >> 3, wave = tid >> 6;
     const int T = (int)fd.T;
-    // element offsets of this thread's columns inside a block (the host keeps
-    // (T - 1) tstride + 32 below 2^31)
-    const int32_t tstride = (int32_t)fd.tstride;
+    // element offsets of this thread's columns inside a block (column-major:
+    // the host keeps (T - 1) ld + 32 below 2^31)
     int32_t col[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int t = s + SL * k;
-        col[k] = (t < T ? t : T - 1) * tstride + 4 * q;
+        if constexpr (COLMAJOR)
+            col[k] = (t < T ? t : T - 1) * (int32_t)fd.ld + 4 * q;
+        else
+            col[k] = (t < T ? t : T - 1) * kSB + 4 * q;
     }
-    const int64_t bstride = fd.bstride;
+    const int64_t bstride = COLMAJOR ? (int64_t)kSB : (int64_t)T * kSB;
     gfloat* blk0 = (gfloat*)fd.Cb + (item.bin0 / kSB) * bstride;
     gdouble* nd = (gdouble*)fd.n;
     const int last = item.nblk - 1;
@@ -191,10 +196,12 @@ __device__ __forceinline__ void fg_item(const FitDesc& fd, const Item& item, int
     SFH_LOAD(ca, na, 0);
     for (int b = 0; b <= last; b += 2) {
         SFH_LOAD(cb, nb, b + 1);
-        fg_block<K, NW>(fd, item.bin0 + (int64_t)b * kSB, eps, want_G, sx, sm, sr, ca, na, g, ll);
+        fg_block<K, NW, COLMAJOR>(fd, item.bin0 + (int64_t)b * kSB, eps, want_G, sx, sm, sr, ca, na, g, ll, rout,
+                                  rout_R);
         SFH_LOAD(ca, na, b + 2);
         if (b + 1 <= last)
-            fg_block<K, NW>(fd, item.bin0 + (int64_t)(b + 1) * kSB, eps, want_G, sx, sm, sr, cb, nb, g, ll);
+            fg_block<K, NW, COLMAJOR>(fd, item.bin0 + (int64_t)(b + 1) * kSB, eps, want_G, sx, sm, sr, cb, nb, g,
+                                      ll, rout, rout_R);
     }
 #undef SFH_LOAD
     if (wave == 0) {
@@ -802,7 +809,7 @@ int kmax_for(int64_t maxT, int nw)
 {
     const int need = (int)cdiv(maxT, 8 * nw);
     const int cap = nw == 4 ? 32 : nw == 8 ? 16 : 8;
-    for (int v : {2, 4, 8, 16, 32})
+    for (int v : {2, 4, 8, 11, 16, 32})
         if (need <= v && v <= cap) return v;
     return 0;
 }
@@ -827,7 +834,7 @@ void launch_fg_t(hipStream_t s, int grid, const FitDesc* fits, const Item* items
 template <int NW, int K>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(fg_waves_per_eu(K))))
 void k_single_fg(FitDesc fd, int64_t nitems, const double* __restrict__ X, double eps, int want_G,
-                 double* __restrict__ ll_part, double* __restrict__ g_part)
+                 double* __restrict__ ll_part, double* __restrict__ g_part, double* rout, int rout_R)
 {
     constexpr int NT = 64 * NW, SL = 8 * NW;
     __shared__ double sx[K * SL];
@@ -839,7 +846,7 @@ void k_single_fg(FitDesc fd, int64_t nitems, const double* __restrict__ X, doubl
     for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int64_t b0 = it * nblk / nitems, b1 = (it + 1) * nblk / nitems;
         const Item item{b0 * kSB, 0, (int32_t)(b1 - b0)};
-        fg_item<K, NW>(fd, item, it, eps, want_G, sx, sm, sr, ll_part, g_part);
+        fg_item<K, NW, true>(fd, item, it, eps, want_G, sx, sm, sr, ll_part, g_part, rout, rout_R);
     }
 }
 
@@ -887,14 +894,14 @@ __global__ __launch_bounds__(kWG) void k_single_reduce(const double* __restrict_
 
 template <int K>
 void launch_single_t(hipStream_t s, int cus, const FitDesc& fd, int64_t nblk, const double* X, double eps,
-                     int want_G, double* llp, double* gp, int64_t* nitems_out)
+                     int want_G, double* llp, double* gp, double* rout, int rout_R, int64_t* nitems_out)
 {
     // workgroups resident per CU: two of 8 waves where K leaves <= 128 VGPRs
     const int64_t grid = (int64_t)cus * (K <= 4 ? 2 : 1);
     const int64_t nitems = nblk < grid ? nblk : grid;
     *nitems_out = nitems;
     hipLaunchKernelGGL((k_single_fg<kSingleNW, K>), dim3((unsigned)nitems), dim3(64 * kSingleNW), 0, s, fd, nitems,
-                       X, eps, want_G, llp, gp);
+                       X, eps, want_G, llp, gp, rout, rout_R);
 }
 
 }  // namespace mf
@@ -925,10 +932,6 @@ int single_pass_fg(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_data,
     fd.ld = cdiv(N, 256) * 256;
     fd.N = N;
     fd.T = T;
-    fd.tstride = fd.ld;
-    fd.bstride = kSB;
-    fd.rout = d_comp_out;
-    fd.rout_R = clobber_with_R;
     // one item (a run of blocks, the next block's loads in flight) per
     // resident workgroup; partials sized for two per CU
     const int64_t nblk = cdiv(N, kSB);
@@ -945,13 +948,13 @@ int single_pass_fg(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_data,
         hipStream_t s = ctx->stream;
         double* l = (double*)llp;
         double* g = (double*)gp;
-        if (need <= 1) launch_single_t<1>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
-        else if (need <= 2) launch_single_t<2>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
-        else if (need <= 3) launch_single_t<3>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
-        else if (need <= 4) launch_single_t<4>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
-        else if (need <= 6) launch_single_t<6>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
-        else if (need <= 8) launch_single_t<8>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
-        else launch_single_t<kSingleMaxK>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, &nitems);
+        if (need <= 1) launch_single_t<1>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        else if (need <= 2) launch_single_t<2>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        else if (need <= 3) launch_single_t<3>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        else if (need <= 4) launch_single_t<4>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        else if (need <= 6) launch_single_t<6>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        else if (need <= 8) launch_single_t<8>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        else launch_single_t<kSingleMaxK>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
     }
     if (int rc = check_launch(ctx, "k_single_fg")) return rc;
     if (want_ll || d_G) {
@@ -1057,9 +1060,10 @@ void launch_fg(sfh_multifit* mf, const double* d_X, const int32_t* active, doubl
 #define SFH_L(NW, KM) launch_fg_t<NW, KM>(s, grid, mf->d_fits, mf->d_items, mf->nitems, d_X, active, eps, want_G, mf->d_llp, mf->d_gp)
     if (mf->nw == 4) {
         if (km <= 2) SFH_L(4, 2); else if (km <= 4) SFH_L(4, 4); else if (km <= 8) SFH_L(4, 8);
-        else if (km <= 16) SFH_L(4, 16); else SFH_L(4, 32);
+        else if (km <= 11) SFH_L(4, 11); else if (km <= 16) SFH_L(4, 16); else SFH_L(4, 32);
     } else if (mf->nw == 8) {
-        if (km <= 2) SFH_L(8, 2); else if (km <= 4) SFH_L(8, 4); else if (km <= 8) SFH_L(8, 8); else SFH_L(8, 16);
+        if (km <= 2) SFH_L(8, 2); else if (km <= 4) SFH_L(8, 4); else if (km <= 8) SFH_L(8, 8);
+        else if (km <= 11) SFH_L(8, 11); else SFH_L(8, 16);
     } else {
         if (km <= 2) SFH_L(16, 2); else if (km <= 4) SFH_L(16, 4); else SFH_L(16, 8);
     }
@@ -1126,10 +1130,6 @@ int sfh_multifit_create(sfh_ctx* ctx, int64_t nfits, const sfh_templates* const*
         fd.ld = cdiv(t->nbins, 256) * 256;
         fd.N = t->nbins;
         fd.T = t->ntemplates;
-        fd.tstride = kSB;
-        fd.bstride = fd.T * kSB;
-        fd.rout = nullptr;
-        fd.rout_R = 0;
         fd.xoff = mf->sumT;
         fd.hoff = mf->sumT2;
         mf->sumT += fd.T;
