@@ -827,19 +827,37 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm_f64_ll(const uint4* __restrict_
     }
 }
 
-// negLL[b] = -(sum_rt ll_part[rt][b]) (-> +Inf for an exact 0, fitting_base.jl:95);
-// G[t + T*b] = sum_split gpart[split][b][t]  (the gradient of -logL).
+// Workgroups [0, nll_blocks): negLL[b] = -(sum_rt ll_part[rt][b]) (-> +Inf
+// for an exact 0, fitting_base.jl:95), 16 walkers x 16 row slices per
+// workgroup (slice s sums rows s, s + 16, ... in order; the slices then add in
+// order).  The rest: G[t + T*b] = sum_split gpart[split][b][t] (the gradient
+// of -logL), one element per thread.
+constexpr int kFinW = 16, kFinS = 16;
 __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ ll_part, int64_t nrt,
                                                 const float* __restrict__ gpart, int nsplit, int64_t T,
-                                                int64_t B, int64_t Bpad, int64_t Tpad2,
+                                                int64_t B, int64_t Bpad, int64_t Tpad2, int64_t nll_blocks,
                                                 double* __restrict__ negLL, double* __restrict__ G)
 {
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (g < B) {
+    if ((int64_t)blockIdx.x < nll_blocks) {
+        __shared__ double sh[kFinS][kFinW];
+        const int wl = threadIdx.x % kFinW, sl = threadIdx.x / kFinW;
+        const int64_t b = (int64_t)blockIdx.x * kFinW + wl;
         double s = 0.0;
-        for (int64_t r = 0; r < nrt; ++r) s += ll_part[r * Bpad + g];
-        negLL[g] = (s != 0.0) ? -s : INFINITY;
+        if (b < B) {
+#pragma unroll 8
+            for (int64_t r = sl; r < nrt; r += kFinS) s += ll_part[r * Bpad + b];
+        }
+        sh[sl][wl] = s;
+        __syncthreads();
+        if (sl == 0 && b < B) {
+            double v = sh[0][wl];
+#pragma unroll
+            for (int k = 1; k < kFinS; ++k) v += sh[k][wl];
+            negLL[b] = (v != 0.0) ? -v : INFINITY;
+        }
+        return;
     }
+    const int64_t g = ((int64_t)blockIdx.x - nll_blocks) * 256 + threadIdx.x;
     if (G && g < T * B) {
         const int64_t b = g / T, t = g - b * T;
         double s = 0.0;
@@ -1044,10 +1062,10 @@ static int fg_batched_impl(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_dat
         if (int rc = check_launch(ctx, "gradient GEMM")) return rc;
     }
     {
-        const int64_t n = std::max<int64_t>(B, d_G ? T * B : 0);
+        const int64_t nll = cdiv(B, kFinW), ng = d_G ? cdiv(T * B, 256) : 0;
         TimedLaunch tl(ctx, "batched_finish");
-        hipLaunchKernelGGL(k_finish, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, ctx->stream, (const double*)pll,
-                           nrt1, (const float*)pg, nsplit, T, B, Bpad, ops->tpad2, d_negLL, d_G);
+        hipLaunchKernelGGL(k_finish, dim3((unsigned)(nll + ng)), dim3(256), 0, ctx->stream, (const double*)pll, nrt1,
+                           (const float*)pg, nsplit, T, B, Bpad, ops->tpad2, nll, d_negLL, d_G);
     }
     return check_launch(ctx, "k_finish");
 }
