@@ -657,16 +657,28 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 // Coefficients X (T x B, column-major FP64) -> [kblock][col tile][16 k][128 walkers] FP64.
+// One workgroup per (kblock, col tile): reads run along the templates (16
+// consecutive t of a walker, 128 bytes), writes along the walkers, through an
+// LDS tile.
 __global__ __launch_bounds__(256) void k_prep_x64(const double* __restrict__ X, int64_t T, int64_t B,
                                                   int64_t nct, int64_t nkb, double* __restrict__ out)
 {
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (g >= nkb * nct * 16 * kBN) return;
-    const int c = (int)(g % kBN);
-    const int k = (int)((g / kBN) % 16);
-    const int64_t ct = (g / (16 * kBN)) % nct, kb = g / (16 * kBN * nct);
-    const int64_t t = kb * 16 + k, w = ct * kBN + c;
-    out[g] = (t < T && w < B) ? X[w * T + t] : 0.0;
+    __shared__ double tile[16][kBN + 1];
+    const int64_t ct = blockIdx.x % nct, kb = blockIdx.x / nct;
+#pragma unroll
+    for (int j = 0; j < 16 * kBN / 256; ++j) {
+        const int idx = threadIdx.x + 256 * j;
+        const int c = idx / 16, k = idx % 16;
+        const int64_t t = kb * 16 + k, w = ct * kBN + c;
+        tile[k][c] = (t < T && w < B) ? X[w * T + t] : 0.0;
+    }
+    __syncthreads();
+    double* o = out + (kb * nct + ct) * 16 * kBN;
+#pragma unroll
+    for (int j = 0; j < 16 * kBN / 256; ++j) {
+        const int idx = threadIdx.x + 256 * j;
+        o[idx] = tile[idx / kBN][idx % kBN];
+    }
 }
 
 // float index of element (row r in [0,128), k in [0,16)) of an FP32 A tile
@@ -992,8 +1004,7 @@ static int fg_batched_impl(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_dat
     {
         TimedLaunch tl(ctx, "batched_prep_x");
         if (mode == F64) {
-            const int64_t n = nkb1 * nct * 16 * kBN;
-            hipLaunchKernelGGL(k_prep_x64, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, ctx->stream, d_X, T, B, nct,
+            hipLaunchKernelGGL(k_prep_x64, dim3((unsigned)(nkb1 * nct)), dim3(256), 0, ctx->stream, d_X, T, B, nct,
                                nkb1, (double*)px);
         } else {
             const int64_t n = nkb1 * ncb * 64;
