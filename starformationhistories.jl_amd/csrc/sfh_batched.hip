@@ -343,6 +343,7 @@ __device__ inline void wait_vmcnt(int n)
     case 4: __builtin_amdgcn_s_waitcnt(0xF74); break;
     case 6: __builtin_amdgcn_s_waitcnt(0xF76); break;
     case 8: __builtin_amdgcn_s_waitcnt(0xF78); break;
+    case 9: __builtin_amdgcn_s_waitcnt(0xF79); break;
     case 12: __builtin_amdgcn_s_waitcnt(0xF7C); break;
     default: __builtin_amdgcn_s_waitcnt(0xF70); break;
     }
