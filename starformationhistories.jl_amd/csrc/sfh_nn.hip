@@ -322,6 +322,184 @@ __global__ __launch_bounds__(kWG) void k_im2col(const float* __restrict__ dout, 
     }
 }
 
+// ------------------------------------------ single-channel output layer, 32 in
+
+// The last transposed convolution has 32 input channels and one output
+// channel.  As matrix products its operands would be a P x 16 column matrix
+// and FP32 transposes of the 32-channel input; these three kernels work on the
+// pixel grid directly, each reading the 32-channel tensor once.
+constexpr int kC1Ci = 32;
+constexpr int kC1TY = 8, kC1TX = 32;                       // input tile of the forward kernel
+constexpr int kC1HY = kC1TY + 2, kC1HX = kC1TX + 2;        // with a one-pixel halo
+
+// out[b][Y][X] = bmap[Y][X] + sum over the (y, x, tap) that reach (Y, X) of
+// in[b][y][x][:] . w[tap][:].  Workgroup = one input tile of one image: the 16
+// tap values of every tile (+ halo) pixel go to LDS (thread = tap x pixel
+// slice, its tap's 32 weights in registers), then every output pixel of the
+// tile's 2x-scaled footprint adds its four.
+__global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in, int B, int H, int W,
+                                                   const float* __restrict__ w, const float* __restrict__ bmap,
+                                                   float* __restrict__ out)
+{
+    __shared__ float tv[kC1HY * kC1HX][17];
+    const int tilesx = (W + kC1TX - 1) / kC1TX, tilesy = (H + kC1TY - 1) / kC1TY;
+    int blk = blockIdx.x;
+    const int tx = blk % tilesx;
+    blk /= tilesx;
+    const int ty = blk % tilesy, b = blk / tilesy;
+    const int y0 = ty * kC1TY, x0 = tx * kC1TX;
+    const int tap = threadIdx.x & 15, slice = threadIdx.x >> 4;
+    float wr[kC1Ci];
+#pragma unroll
+    for (int c = 0; c < kC1Ci; c += 4) {
+        const f4 v = *reinterpret_cast<const f4*>(w + tap * kC1Ci + c);
+        wr[c] = v[0];
+        wr[c + 1] = v[1];
+        wr[c + 2] = v[2];
+        wr[c + 3] = v[3];
+    }
+    for (int p = slice; p < kC1HY * kC1HX; p += kWG / 16) {
+        const int yy = y0 - 1 + p / kC1HX, xx = x0 - 1 + p % kC1HX;
+        float s = 0.f;
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+            const f4* a = reinterpret_cast<const f4*>(in + (((int64_t)b * H + yy) * W + xx) * kC1Ci);
+#pragma unroll
+            for (int q = 0; q < kC1Ci / 4; ++q) {
+                const f4 v = a[q];
+                s = fmaf(v[0], wr[4 * q], s);
+                s = fmaf(v[1], wr[4 * q + 1], s);
+                s = fmaf(v[2], wr[4 * q + 2], s);
+                s = fmaf(v[3], wr[4 * q + 3], s);
+            }
+        }
+        tv[p][tap] = s;
+    }
+    __syncthreads();
+    const int H2 = 2 * H, W2 = 2 * W;
+    for (int o = threadIdx.x; o < 4 * kC1TY * kC1TX; o += kWG) {
+        const int Y = 2 * y0 + o / (2 * kC1TX), X = 2 * x0 + o % (2 * kC1TX);
+        if (Y >= H2 || X >= W2) continue;
+        float s = bmap ? bmap[Y * W2 + X] : 0.f;
+        const int ky0 = (Y + 1) & 1, kx0 = (X + 1) & 1;
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            const int ky = ky0 + 2 * dy, y = (Y + 1 - ky) / 2;
+            if (y < 0 || y >= H) continue;
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const int kx = kx0 + 2 * dx, x = (X + 1 - kx) / 2;
+                if (x < 0 || x >= W) continue;
+                s += tv[(y - y0 + 1) * kC1HX + (x - x0 + 1)][ky * 4 + kx];
+            }
+        }
+        out[((int64_t)b * H2 + Y) * W2 + X] = s;
+    }
+}
+
+// The 16 output-gradient values a pixel's taps reach: d[ky*4 + kx] =
+// dout[b][2y - 1 + ky][2x - 1 + kx] (0 outside the output grid).
+__device__ inline void conv1_window(const float* __restrict__ dout, int64_t b, int y, int x, int H2, int W2,
+                                    float (&d)[16])
+{
+#pragma unroll
+    for (int ky = 0; ky < 4; ++ky) {
+        const int Y = 2 * y - 1 + ky;
+        const bool yok = Y >= 0 && Y < H2;
+#pragma unroll
+        for (int kx = 0; kx < 4; ++kx) {
+            const int X = 2 * x - 1 + kx;
+            d[ky * 4 + kx] = (yok && X >= 0 && X < W2) ? dout[(b * H2 + Y) * W2 + X] : 0.f;
+        }
+    }
+}
+
+// din[p][c] = sum_tap window(p)[tap] w[tap][c]; thread = (pixel, 4 channels).
+__global__ __launch_bounds__(kWG) void k_conv1_din(const float* __restrict__ dout, int B, int H, int W,
+                                                   const float* __restrict__ w, float* __restrict__ din)
+{
+    __shared__ f4 sw[16][kC1Ci / 4];
+    if (threadIdx.x < 16 * kC1Ci / 4) sw[threadIdx.x / (kC1Ci / 4)][threadIdx.x % (kC1Ci / 4)] =
+        reinterpret_cast<const f4*>(w)[threadIdx.x];
+    __syncthreads();
+    const int64_t P = (int64_t)B * H * W;
+    const int64_t g = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    if (g >= P * (kC1Ci / 4)) return;
+    const int q = (int)(g % (kC1Ci / 4));
+    const int64_t p = g / (kC1Ci / 4);
+    const int x = (int)(p % W), y = (int)((p / W) % H);
+    const int64_t b = p / ((int64_t)W * H);
+    float d[16];
+    conv1_window(dout, b, y, x, 2 * H, 2 * W, d);
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const f4 wv = sw[t][q];
+        acc[0] = fmaf(d[t], wv[0], acc[0]);
+        acc[1] = fmaf(d[t], wv[1], acc[1]);
+        acc[2] = fmaf(d[t], wv[2], acc[2]);
+        acc[3] = fmaf(d[t], wv[3], acc[3]);
+    }
+    *reinterpret_cast<f4*>(din + p * kC1Ci + 4 * q) = acc;
+}
+
+// Partial weight gradients: part[wg][tap*32 + c] = sum over the workgroup's
+// pixels of window(p)[tap] in[p][c].  Thread = (pixel lane, 4 channels): 64
+// pixel lanes x 8 channel quads, 16 x 4 accumulators; the pixel lanes of a
+// wave are summed with shuffles, then the waves through LDS, in a fixed order.
+constexpr int kC1DwWG = 512;
+__global__ __launch_bounds__(kC1DwWG) void k_conv1_dw(const float* __restrict__ dout, const float* __restrict__ in,
+                                                      int B, int H, int W, int64_t per_wg,
+                                                      float* __restrict__ part)
+{
+    constexpr int NWV = kC1DwWG / 64;
+    __shared__ float red[NWV][16 * kC1Ci];
+    const int q = threadIdx.x & 7, pl = threadIdx.x >> 3, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t P = (int64_t)B * H * W;
+    const int64_t p0 = (int64_t)blockIdx.x * per_wg;
+    const int64_t p1 = p0 + per_wg < P ? p0 + per_wg : P;
+    float acc[16][4];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) acc[t][0] = acc[t][1] = acc[t][2] = acc[t][3] = 0.f;
+    for (int64_t p = p0 + pl; p < p1; p += kC1DwWG / 8) {
+        const f4 a = *reinterpret_cast<const f4*>(in + p * kC1Ci + 4 * q);
+        const int x = (int)(p % W), y = (int)((p / W) % H);
+        const int64_t b = p / ((int64_t)W * H);
+        float d[16];
+        conv1_window(dout, b, y, x, 2 * H, 2 * W, d);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            acc[t][0] = fmaf(d[t], a[0], acc[t][0]);
+            acc[t][1] = fmaf(d[t], a[1], acc[t][1]);
+            acc[t][2] = fmaf(d[t], a[2], acc[t][2]);
+            acc[t][3] = fmaf(d[t], a[3], acc[t][3]);
+        }
+    }
+    // sum the 8 pixel lanes of this wave (lane bits 3..5)
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float v = acc[t][j];
+            v += __shfl_xor(v, 8, 64);
+            v += __shfl_xor(v, 16, 64);
+            v += __shfl_xor(v, 32, 64);
+            acc[t][j] = v;
+        }
+    if (lane < 8) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) red[wave][t * kC1Ci + 4 * q + j] = acc[t][j];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 16 * kC1Ci; i += kC1DwWG) {
+        float s = red[0][i];
+#pragma unroll
+        for (int k = 1; k < NWV; ++k) s += red[k][i];
+        part[(int64_t)blockIdx.x * 16 * kC1Ci + i] = s;
+    }
+}
+
 // Four consecutive channels per thread (Co % 4 == 0, ldc / ldd % 4 == 0),
 // 32-bit index arithmetic (P * ldd < 2^31).
 template <typename T>
@@ -818,6 +996,16 @@ int colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols,
 
 }  // namespace
 
+// The direct output-layer kernels: 32 input channels, 16-byte aligned
+// tensors, 32-bit pixel coordinates.
+static bool conv1_direct(int64_t B, int64_t H, int64_t W, int64_t Ci, const float* in, const float* w,
+                         const float* din = nullptr)
+{
+    const auto al = [](const float* q) { return !q || ((uintptr_t)q & 15) == 0; };
+    return Ci == kC1Ci && al(in) && al(w) && al(din) && B >= 1 && 2 * H * 2 * W < ((int64_t)1 << 31) &&
+           B * H * W * kC1Ci < ((int64_t)1 << 40) && B * cdiv(H, kC1TY) * cdiv(W, kC1TX) < ((int64_t)1 << 31);
+}
+
 extern "C" {
 
 int sfh_nn_gemm_nt(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* B, int64_t ldb, float* C,
@@ -947,6 +1135,12 @@ int sfh_nn_conv1_fwd(sfh_ctx* ctx, const float* in, int64_t B, int64_t H, int64_
                   "sfh_nn_conv1_fwd: Ci a positive multiple of 16");
     const int64_t P = B * H * W;
     if (P == 0) return SFH_OK;
+    if (conv1_direct(B, H, W, Ci, in, w)) {
+        const int64_t nblk = B * cdiv(H, kC1TY) * cdiv(W, kC1TX);
+        hipLaunchKernelGGL(k_conv1_fwd, dim3((unsigned)nblk), dim3(kWG), 0, ctx->stream, in, (int)B, (int)H, (int)W,
+                           w, bmap, out);
+        return check_launch(ctx, "k_conv1_fwd");
+    }
     void* p = nullptr;
     if (int rc = scratch(ctx, "nn_conv1_cols", (size_t)P * 16 * sizeof(float), &p)) return rc;
     float* cols = (float*)p;
@@ -968,7 +1162,26 @@ int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B
     const int64_t P = B * H * W, Pp = cdiv(P, 16) * 16;
     if (P == 0) return SFH_OK;
     void* p = nullptr;
-    if (din || dw) {
+    if ((din || dw) && conv1_direct(B, H, W, Ci, dw ? in : nullptr, din ? w : nullptr, din)) {
+        if (din) {
+            hipLaunchKernelGGL(k_conv1_din, dim3((unsigned)cdiv(P * (kC1Ci / 4), kWG)), dim3(kWG), 0, ctx->stream,
+                               dout, (int)B, (int)H, (int)W, w, din);
+            if (int rc = check_launch(ctx, "k_conv1_din")) return rc;
+        }
+        if (dw) {
+            // one partial per workgroup (at most 256: a single-pass column sum)
+            int64_t nwg = std::min<int64_t>(256, cdiv(P, kC1DwWG / 8));
+            const int64_t per = cdiv(P, nwg);
+            nwg = cdiv(P, per);
+            if (int rc = scratch(ctx, "nn_conv1_dw", (size_t)nwg * 16 * kC1Ci * sizeof(float), &p)) return rc;
+            hipLaunchKernelGGL(k_conv1_dw, dim3((unsigned)nwg), dim3(kC1DwWG), 0, ctx->stream, dout, in, (int)B,
+                               (int)H, (int)W, per, (float*)p);
+            if (int rc = check_launch(ctx, "k_conv1_dw")) return rc;
+            if (int rc = colsum(ctx, (const float*)p, 16 * kC1Ci, nwg, 16 * kC1Ci, 1.f, 0.f, dw, 16 * kC1Ci,
+                                16 * kC1Ci, "nn_conv1_dw_sum"))
+                return rc;
+        }
+    } else if (din || dw) {
         if (int rc = scratch(ctx, "nn_conv1_cols", (size_t)P * 16 * sizeof(float), &p)) return rc;
         float* dcols = (float*)p;
         hipLaunchKernelGGL(k_im2col<float>, dim3(grid_for(P * 16)), dim3(kWG), 0, ctx->stream, dout, B, H, W,

@@ -1,9 +1,16 @@
 """GPU primitives of the surrogate pipeline against NumPy restatements
 (template <-> log-image transposes, the per-template KL behind dlnL, row
 gathers, the indexed MSE, split-K products, column sums) and the bookkeeping
-of fill_templates against exact templates built directly."""
+of fill_templates against exact templates built directly; the output layer
+against the oracle's transposed convolution."""
+import os
+import sys
+
 import numpy as np
 import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import surrogate_ref as R   # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -134,6 +141,40 @@ def test_colsum(sfh, gpu_ctx, rows, cols):
     gpu_ctx.synchronize()
     ref = X.astype(np.float64).sum(0) + 0.5 * out0
     assert np.max(np.abs(o_d.cpu().numpy() - ref)) <= 1e-5 * np.sqrt(rows) + 1e-6
+
+
+@pytest.mark.parametrize("B,H,W,Ci", [(2, 13, 37, 32), (1, 8, 32, 32), (3, 5, 7, 32), (2, 6, 9, 16)])
+def test_output_layer_against_oracle(sfh, gpu_ctx, B, H, W, Ci):
+    """The single-channel output layer (32 input channels: direct pixel-grid
+    kernels; other widths: the matrix-product form) against the FP64 oracle's
+    transposed convolution and its gradients, on grids that leave partial
+    input tiles."""
+    import torch
+    rng = np.random.default_rng(B * 1000 + H * 10 + W)
+    a = rng.normal(size=(B, H, W, Ci)).astype(np.float32)
+    w = rng.normal(size=(16, Ci)).astype(np.float32)
+    bmap = rng.normal(size=(2 * H * 2 * W,)).astype(np.float32)
+    dout = rng.normal(size=(B, 2 * H, 2 * W)).astype(np.float32)
+    a_d, w_d, bm_d, do_d = (torch.from_numpy(v).cuda() for v in (a, w, bmap, dout))
+    out_d = torch.empty((B, 2 * H, 2 * W), dtype=torch.float32, device="cuda")
+    din_d = torch.empty_like(a_d)
+    dw_d = torch.empty_like(w_d)
+    dbm_d = torch.empty_like(bm_d)
+    torch.cuda.synchronize()
+    lib, h = gpu_ctx.lib, gpu_ctx.handle
+    gpu_ctx.check(lib.sfh_nn_conv1_fwd(h, a_d.data_ptr(), B, H, W, Ci, w_d.data_ptr(), bm_d.data_ptr(), out_d.data_ptr()))
+    gpu_ctx.check(lib.sfh_nn_conv1_bwd(h, do_d.data_ptr(), a_d.data_ptr(), B, H, W, Ci, w_d.data_ptr(),
+                                       din_d.data_ptr(), dw_d.data_ptr(), dbm_d.data_ptr()))
+    gpu_ctx.synchronize()
+    a64, w64 = a.astype(np.float64), w.astype(np.float64)
+    ref = R.convT(a64, w64, 1)[..., 0] + bmap.reshape(2 * H, 2 * W)
+    rdin, rdw = R.convT_bwd(a64, w64, 1, dout.astype(np.float64)[..., None])
+    def rel(x, r):
+        return float(np.max(np.abs(x - r)) / np.max(np.abs(r)))
+    assert rel(out_d.cpu().numpy(), ref) < 1e-6
+    assert rel(din_d.cpu().numpy(), rdin) < 1e-6
+    assert rel(dw_d.cpu().numpy(), rdw) < 1e-6
+    assert rel(dbm_d.cpu().numpy(), dout.astype(np.float64).sum(0).ravel()) < 1e-6
 
 
 def test_fill_templates_bookkeeping(sfh, gpu_ctx):
