@@ -424,11 +424,13 @@ int sfh_nn_im2col(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t
 int sfh_nn_im2col_t(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t H, int64_t W, int64_t Co,
                     void* dcolsT, int64_t ldo);
 /* The same transposed convolution with one output channel (B x 2H x 2W) and a
- * per-pixel bias map; its backward gives din, dw (16 x Ci) and dbmap (any may be NULL). */
+ * per-pixel bias map, applied to act(in) (act 0: identity, 1: SiLU, in then
+ * holding pre-activations); its backward gives din (the gradient of in:
+ * through act'), dw (16 x Ci) and dbmap (any may be NULL). */
 int sfh_nn_conv1_fwd(sfh_ctx* ctx, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci, const float* w,
-                     const float* bmap, float* out);
+                     const float* bmap, int act, float* out);
 int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci,
-                     const float* w, float* din, float* dw, float* dbmap);
+                     const float* w, int act, float* din, float* dw, float* dbmap);
 /* Mean squared error over the top-left R x S of each of B outputs of H2 x W2
  * against target rows (R x S each): output b against row rows[b] (b when rows
  * is NULL).  dout (may be NULL) gets its gradient.  With loss non-NULL the

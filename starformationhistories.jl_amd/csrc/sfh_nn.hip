@@ -332,22 +332,42 @@ constexpr int kC1Ci = 32;
 constexpr int kC1TY = 8, kC1TX = 32;                       // input tile of the forward kernel
 constexpr int kC1HY = kC1TY + 2, kC1HX = kC1TX + 2;        // with a one-pixel halo
 
+__device__ inline f4 act4(f4 v, int act)
+{
+    if (act == 1) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = v[j] * sigmoid(v[j]);
+    }
+    return v;
+}
+
 // out[b][Y][X] = bmap[Y][X] + sum over the (y, x, tap) that reach (Y, X) of
-// in[b][y][x][:] . w[tap][:].  Workgroup = one input tile of one image: the 16
-// tap values of every tile (+ halo) pixel go to LDS (thread = tap x pixel
-// slice, its tap's 32 weights in registers), then every output pixel of the
-// tile's 2x-scaled footprint adds its four.
+// act(in[b][y][x][:]) . w[tap][:].  Workgroup = one input tile of one image:
+// the tile's activated pixels (+ a one-pixel halo, zeros outside the image)
+// are staged in LDS, their 16 tap values computed into LDS (thread = tap x
+// pixel slice, its tap's 32 weights in registers), then every output pixel of
+// the tile's 2x-scaled footprint adds its four.
 __global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in, int B, int H, int W,
                                                    const float* __restrict__ w, const float* __restrict__ bmap,
-                                                   float* __restrict__ out)
+                                                   int act, float* __restrict__ out)
 {
-    __shared__ float tv[kC1HY * kC1HX][17];
+    constexpr int NP = kC1HY * kC1HX, NQ = kC1Ci / 4;
+    __shared__ f4 xs[NP][NQ];
+    __shared__ float tv[NP][17];
     const int tilesx = (W + kC1TX - 1) / kC1TX, tilesy = (H + kC1TY - 1) / kC1TY;
     int blk = blockIdx.x;
     const int tx = blk % tilesx;
     blk /= tilesx;
     const int ty = blk % tilesy, b = blk / tilesy;
     const int y0 = ty * kC1TY, x0 = tx * kC1TX;
+    for (int i = threadIdx.x; i < NP * NQ; i += kWG) {
+        const int p = i / NQ, q = i % NQ;
+        const int yy = y0 - 1 + p / kC1HX, xx = x0 - 1 + p % kC1HX;
+        f4 v = {0.f, 0.f, 0.f, 0.f};
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+            v = act4(reinterpret_cast<const f4*>(in + (((int64_t)b * H + yy) * W + xx) * kC1Ci)[q], act);
+        xs[p][q] = v;
+    }
     const int tap = threadIdx.x & 15, slice = threadIdx.x >> 4;
     float wr[kC1Ci];
 #pragma unroll
@@ -358,19 +378,16 @@ __global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in,
         wr[c + 2] = v[2];
         wr[c + 3] = v[3];
     }
-    for (int p = slice; p < kC1HY * kC1HX; p += kWG / 16) {
-        const int yy = y0 - 1 + p / kC1HX, xx = x0 - 1 + p % kC1HX;
+    __syncthreads();
+    for (int p = slice; p < NP; p += kWG / 16) {
         float s = 0.f;
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-            const f4* a = reinterpret_cast<const f4*>(in + (((int64_t)b * H + yy) * W + xx) * kC1Ci);
 #pragma unroll
-            for (int q = 0; q < kC1Ci / 4; ++q) {
-                const f4 v = a[q];
-                s = fmaf(v[0], wr[4 * q], s);
-                s = fmaf(v[1], wr[4 * q + 1], s);
-                s = fmaf(v[2], wr[4 * q + 2], s);
-                s = fmaf(v[3], wr[4 * q + 3], s);
-            }
+        for (int q = 0; q < NQ; ++q) {
+            const f4 v = xs[p][q];
+            s = fmaf(v[0], wr[4 * q], s);
+            s = fmaf(v[1], wr[4 * q + 1], s);
+            s = fmaf(v[2], wr[4 * q + 2], s);
+            s = fmaf(v[3], wr[4 * q + 3], s);
         }
         tv[p][tap] = s;
     }
@@ -413,9 +430,12 @@ __device__ inline void conv1_window(const float* __restrict__ dout, int64_t b, i
     }
 }
 
-// din[p][c] = sum_tap window(p)[tap] w[tap][c]; thread = (pixel, 4 channels).
-__global__ __launch_bounds__(kWG) void k_conv1_din(const float* __restrict__ dout, int B, int H, int W,
-                                                   const float* __restrict__ w, float* __restrict__ din)
+// din[p][c] = sum_tap window(p)[tap] w[tap][c], times act'(in[p][c]) with
+// act = SiLU (din is then the gradient of the pre-activation in); thread =
+// (pixel, 4 channels).
+__global__ __launch_bounds__(kWG) void k_conv1_din(const float* __restrict__ dout, const float* __restrict__ in,
+                                                   int B, int H, int W, const float* __restrict__ w, int act,
+                                                   float* __restrict__ din)
 {
     __shared__ f4 sw[16][kC1Ci / 4];
     if (threadIdx.x < 16 * kC1Ci / 4) sw[threadIdx.x / (kC1Ci / 4)][threadIdx.x % (kC1Ci / 4)] =
@@ -439,16 +459,24 @@ __global__ __launch_bounds__(kWG) void k_conv1_din(const float* __restrict__ dou
         acc[2] = fmaf(d[t], wv[2], acc[2]);
         acc[3] = fmaf(d[t], wv[3], acc[3]);
     }
+    if (act == 1) {
+        const f4 x = *reinterpret_cast<const f4*>(in + p * kC1Ci + 4 * q);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float s = sigmoid(x[j]);
+            acc[j] *= s * (1.f + x[j] * (1.f - s));
+        }
+    }
     *reinterpret_cast<f4*>(din + p * kC1Ci + 4 * q) = acc;
 }
 
 // Partial weight gradients: part[wg][tap*32 + c] = sum over the workgroup's
-// pixels of window(p)[tap] in[p][c].  Thread = (pixel lane, 4 channels): 64
+// pixels of window(p)[tap] act(in[p][c]).  Thread = (pixel lane, 4 channels): 64
 // pixel lanes x 8 channel quads, 16 x 4 accumulators; the pixel lanes of a
 // wave are summed with shuffles, then the waves through LDS, in a fixed order.
 constexpr int kC1DwWG = 512;
 __global__ __launch_bounds__(kC1DwWG) void k_conv1_dw(const float* __restrict__ dout, const float* __restrict__ in,
-                                                      int B, int H, int W, int64_t per_wg,
+                                                      int B, int H, int W, int64_t per_wg, int act,
                                                       float* __restrict__ part)
 {
     constexpr int NWV = kC1DwWG / 64;
@@ -461,7 +489,7 @@ __global__ __launch_bounds__(kC1DwWG) void k_conv1_dw(const float* __restrict__ 
 #pragma unroll
     for (int t = 0; t < 16; ++t) acc[t][0] = acc[t][1] = acc[t][2] = acc[t][3] = 0.f;
     for (int64_t p = p0 + pl; p < p1; p += kC1DwWG / 8) {
-        const f4 a = *reinterpret_cast<const f4*>(in + p * kC1Ci + 4 * q);
+        const f4 a = act4(*reinterpret_cast<const f4*>(in + p * kC1Ci + 4 * q), act);
         const int x = (int)(p % W), y = (int)((p / W) % H);
         const int64_t b = p / ((int64_t)W * H);
         float d[16];
@@ -1127,20 +1155,35 @@ int sfh_nn_im2col_t(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64
 // The single-channel output layer as the other layers: a matrix product to
 // the 16 taps of each input pixel, then the col2im gather (with the per-pixel
 // bias map); FP32 operands on the matrix cores.
+// act(in) for the matrix-product form of the output layer (act = identity: in).
+static int conv1_activated(sfh_ctx* ctx, const float* in, int64_t P, int64_t Ci, int act, const float** out)
+{
+    *out = in;
+    if (act == 0) return SFH_OK;
+    void* p = nullptr;
+    if (int rc = scratch(ctx, "nn_conv1_act", (size_t)P * Ci * sizeof(float), &p)) return rc;
+    hipLaunchKernelGGL(k_bias_act<float>, dim3(grid_for(P * Ci)), dim3(kWG), 0, ctx->stream, in, Ci,
+                       (const float*)nullptr, P, Ci, act, (float*)nullptr, (float*)p, Ci);
+    *out = (const float*)p;
+    return check_launch(ctx, "k_bias_act");
+}
+
 int sfh_nn_conv1_fwd(sfh_ctx* ctx, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci, const float* w,
-                     const float* bmap, float* out)
+                     const float* bmap, int act, float* out)
 {
     SFH_NN_BEGIN(ctx);
-    SFH_ARG_CHECK(ctx, in && w && out && B >= 0 && H >= 1 && W >= 1 && Ci >= 16 && Ci % 16 == 0,
-                  "sfh_nn_conv1_fwd: Ci a positive multiple of 16");
+    SFH_ARG_CHECK(ctx, in && w && out && B >= 0 && H >= 1 && W >= 1 && Ci >= 16 && Ci % 16 == 0 &&
+                           (act == 0 || act == 1),
+                  "sfh_nn_conv1_fwd: Ci a positive multiple of 16, act 0 (identity) or 1 (SiLU)");
     const int64_t P = B * H * W;
     if (P == 0) return SFH_OK;
     if (conv1_direct(B, H, W, Ci, in, w)) {
         const int64_t nblk = B * cdiv(H, kC1TY) * cdiv(W, kC1TX);
         hipLaunchKernelGGL(k_conv1_fwd, dim3((unsigned)nblk), dim3(kWG), 0, ctx->stream, in, (int)B, (int)H, (int)W,
-                           w, bmap, out);
+                           w, bmap, act, out);
         return check_launch(ctx, "k_conv1_fwd");
     }
+    if (int rc = conv1_activated(ctx, in, P, Ci, act, &in)) return rc;
     void* p = nullptr;
     if (int rc = scratch(ctx, "nn_conv1_cols", (size_t)P * 16 * sizeof(float), &p)) return rc;
     float* cols = (float*)p;
@@ -1151,21 +1194,24 @@ int sfh_nn_conv1_fwd(sfh_ctx* ctx, const float* in, int64_t B, int64_t H, int64_
     return check_launch(ctx, "k_col2im");
 }
 
-// Backward: dcols = im2col(dout) (P x 16); din = dcols w (P x Ci);
-// dw = dcols^T in (16 x Ci, K = pixels, split along K); dbmap = sum over b of dout.
+// Backward (matrix-product form): dcols = im2col(dout) (P x 16); din = dcols w
+// (P x Ci), times act'(in); dw = dcols^T act(in) (16 x Ci, K = pixels, split
+// along K); dbmap = sum over b of dout.
 int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci,
-                     const float* w, float* din, float* dw, float* dbmap)
+                     const float* w, int act, float* din, float* dw, float* dbmap)
 {
     SFH_NN_BEGIN(ctx);
-    SFH_ARG_CHECK(ctx, dout && B >= 0 && H >= 1 && W >= 1 && Ci >= 16 && Ci % 16 == 0 && (!din || w) && (!dw || in),
-                  "sfh_nn_conv1_bwd: Ci a positive multiple of 16, w with din, in with dw");
+    SFH_ARG_CHECK(ctx, dout && B >= 0 && H >= 1 && W >= 1 && Ci >= 16 && Ci % 16 == 0 && (!din || w) &&
+                           (!dw || in) && (act == 0 || act == 1) && (act == 0 || !din || in),
+                  "sfh_nn_conv1_bwd: Ci a positive multiple of 16, w with din, in with dw (and with din when act "
+                  "is 1), act 0 or 1");
     const int64_t P = B * H * W, Pp = cdiv(P, 16) * 16;
     if (P == 0) return SFH_OK;
     void* p = nullptr;
-    if ((din || dw) && conv1_direct(B, H, W, Ci, dw ? in : nullptr, din ? w : nullptr, din)) {
+    if ((din || dw) && conv1_direct(B, H, W, Ci, (dw || act) ? in : nullptr, din ? w : nullptr, din)) {
         if (din) {
             hipLaunchKernelGGL(k_conv1_din, dim3((unsigned)cdiv(P * (kC1Ci / 4), kWG)), dim3(kWG), 0, ctx->stream,
-                               dout, (int)B, (int)H, (int)W, w, din);
+                               dout, in, (int)B, (int)H, (int)W, w, act, din);
             if (int rc = check_launch(ctx, "k_conv1_din")) return rc;
         }
         if (dw) {
@@ -1175,13 +1221,16 @@ int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B
             nwg = cdiv(P, per);
             if (int rc = scratch(ctx, "nn_conv1_dw", (size_t)nwg * 16 * kC1Ci * sizeof(float), &p)) return rc;
             hipLaunchKernelGGL(k_conv1_dw, dim3((unsigned)nwg), dim3(kC1DwWG), 0, ctx->stream, dout, in, (int)B,
-                               (int)H, (int)W, per, (float*)p);
+                               (int)H, (int)W, per, act, (float*)p);
             if (int rc = check_launch(ctx, "k_conv1_dw")) return rc;
             if (int rc = colsum(ctx, (const float*)p, 16 * kC1Ci, nwg, 16 * kC1Ci, 1.f, 0.f, dw, 16 * kC1Ci,
                                 16 * kC1Ci, "nn_conv1_dw_sum"))
                 return rc;
         }
     } else if (din || dw) {
+        const float* pre = in;
+        if (dw)
+            if (int rc = conv1_activated(ctx, pre, P, Ci, act, &in)) return rc;
         if (int rc = scratch(ctx, "nn_conv1_cols", (size_t)P * 16 * sizeof(float), &p)) return rc;
         float* dcols = (float*)p;
         hipLaunchKernelGGL(k_im2col<float>, dim3(grid_for(P * 16)), dim3(kWG), 0, ctx->stream, dout, B, H, W,
@@ -1194,6 +1243,11 @@ int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B
             hipLaunchKernelGGL(k_transpose<float>, dim3(1, (unsigned)cdiv(Ci, 32)), dim3(kWG), 0, ctx->stream, w, Ci,
                                (int64_t)16, Ci, (float*)q, (int64_t)16, Ci, 0);
             if (int rc = gemm(ctx, 1, dcols, 16, q, 16, din, Ci, P, Ci, 16, 1.f, 0.f)) return rc;
+            if (act) {
+                hipLaunchKernelGGL(k_act_bwd, dim3(grid_for(P * Ci)), dim3(kWG), 0, ctx->stream, (const float*)din, pre,
+                                   P * Ci, act, din);
+                if (int rc = check_launch(ctx, "k_act_bwd")) return rc;
+            }
         }
         if (dw) {
             // dw = dcols^T (16 x P) times in (P x Ci): operands dcols^T and in^T, K = Pp (zero padded)

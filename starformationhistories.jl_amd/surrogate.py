@@ -145,7 +145,6 @@ class TemplateSurrogate:
         self.a0, self.a1, self.a2 = self._buf(B * HID0, t), self._buf(B * HID1, t), self._buf(B * LAT, t)
         self.pre3, self.pre4, self.pre5 = self._buf(P2 * CH), self._buf(P3 * CH), self._buf(P4 * CH)
         self.a3, self.a4 = self._buf(P2 * CH, t), self._buf(P3 * CH, t)
-        self.a5 = self._buf(P4 * CH)
         self.out = self._buf(B * OUT_H * OUT_W)
         self.dout = self._buf(B * OUT_H * OUT_W)
         self.dA = self._buf(max(P4 * CH, B * LAT))            # gradients w.r.t. activations (FP32)
@@ -235,10 +234,9 @@ class TemplateSurrogate:
             H, W = 2 * H, 2 * W
             if outs[li] is not None:
                 self._bias_act(pres[li], CH, None, B * H * W, CH, SILU, None, outs[li], CH)
-            else:
-                self._bias_act(pres[li], CH, None, B * H * W, CH, SILU, None, self.a5, CH, dt=1)
-        self._c(self.lib.sfh_nn_conv1_fwd(self.ctx.handle, self.a5.data_ptr(), B, H, W, CH, v["w4"].data_ptr(),
-                                          v["bmap"].data_ptr(), self.out.data_ptr()))
+        # single-channel output layer on SiLU(pre5), the activation applied inside
+        self._c(self.lib.sfh_nn_conv1_fwd(self.ctx.handle, self.pre5.data_ptr(), B, H, W, CH, v["w4"].data_ptr(),
+                                          v["bmap"].data_ptr(), SILU, self.out.data_ptr()))
 
     # ---------------------------------------------------------- backward
     def backward(self, target):
@@ -258,18 +256,20 @@ class TemplateSurrogate:
         B, v, g = self.B, self.views, self.gviews
         H, W = 8 * H0, 8 * W0                                   # input grid of the last layer
         # single-channel output layer
-        self._c(self.lib.sfh_nn_conv1_bwd(self.ctx.handle, self.dout.data_ptr(), self.a5.data_ptr(), B, H, W, CH,
-                                          v["w4"].data_ptr(), self.dA.data_ptr(), g["w4"].data_ptr(),
+        self._c(self.lib.sfh_nn_conv1_bwd(self.ctx.handle, self.dout.data_ptr(), self.pre5.data_ptr(), B, H, W, CH,
+                                          v["w4"].data_ptr(), SILU, self.dA.data_ptr(), g["w4"].data_ptr(),
                                           g["bmap"].data_ptr()))
-        d = self.dA                                              # gradient w.r.t. a5
+        d = self.dA                                              # gradient w.r.t. pre5
         layers = (("K3", "c3", self.pre5, self.a4, self.pre4),
                   ("K2", "c2", self.pre4, self.a3, self.pre3),
                   ("K1", "c1", self.pre3, self.a2, self.pre2))
-        for K, cb, pre_out, a_in, pre_in in layers:
-            # d is the gradient w.r.t. act(pre_out) (output grid H x W)
+        for li, (K, cb, pre_out, a_in, pre_in) in enumerate(layers):
+            # d is the gradient w.r.t. act(pre_out) (output grid H x W); the
+            # output layer already returned the first one w.r.t. pre5 itself
             Hi, Wi = H // 2, W // 2
             Pi, Po = B * Hi * Wi, B * H * W
-            self._act_bwd(d, pre_out, Po * CH)                   # -> gradient w.r.t. pre_out
+            if li > 0:
+                self._act_bwd(d, pre_out, Po * CH)               # -> gradient w.r.t. pre_out
             self._colsum(d, CH, Po, CH, g[cb])
             self._c(self.lib.sfh_nn_im2col(self.ctx.handle, self.dt, d.data_ptr(), B, Hi, Wi, CH,
                                            self.dcols.data_ptr(), 16 * CH))

@@ -143,12 +143,13 @@ def test_colsum(sfh, gpu_ctx, rows, cols):
     assert np.max(np.abs(o_d.cpu().numpy() - ref)) <= 1e-5 * np.sqrt(rows) + 1e-6
 
 
+@pytest.mark.parametrize("act", [0, 1])
 @pytest.mark.parametrize("B,H,W,Ci", [(2, 13, 37, 32), (1, 8, 32, 32), (3, 5, 7, 32), (2, 6, 9, 16)])
-def test_output_layer_against_oracle(sfh, gpu_ctx, B, H, W, Ci):
+def test_output_layer_against_oracle(sfh, gpu_ctx, B, H, W, Ci, act):
     """The single-channel output layer (32 input channels: direct pixel-grid
-    kernels; other widths: the matrix-product form) against the FP64 oracle's
-    transposed convolution and its gradients, on grids that leave partial
-    input tiles."""
+    kernels; other widths: the matrix-product form), on act(in) with act the
+    identity or SiLU, against the FP64 oracle's transposed convolution and its
+    gradients, on grids that leave partial input tiles."""
     import torch
     rng = np.random.default_rng(B * 1000 + H * 10 + W)
     a = rng.normal(size=(B, H, W, Ci)).astype(np.float32)
@@ -162,13 +163,17 @@ def test_output_layer_against_oracle(sfh, gpu_ctx, B, H, W, Ci):
     dbm_d = torch.empty_like(bm_d)
     torch.cuda.synchronize()
     lib, h = gpu_ctx.lib, gpu_ctx.handle
-    gpu_ctx.check(lib.sfh_nn_conv1_fwd(h, a_d.data_ptr(), B, H, W, Ci, w_d.data_ptr(), bm_d.data_ptr(), out_d.data_ptr()))
-    gpu_ctx.check(lib.sfh_nn_conv1_bwd(h, do_d.data_ptr(), a_d.data_ptr(), B, H, W, Ci, w_d.data_ptr(),
+    gpu_ctx.check(lib.sfh_nn_conv1_fwd(h, a_d.data_ptr(), B, H, W, Ci, w_d.data_ptr(), bm_d.data_ptr(), act,
+                                       out_d.data_ptr()))
+    gpu_ctx.check(lib.sfh_nn_conv1_bwd(h, do_d.data_ptr(), a_d.data_ptr(), B, H, W, Ci, w_d.data_ptr(), act,
                                        din_d.data_ptr(), dw_d.data_ptr(), dbm_d.data_ptr()))
     gpu_ctx.synchronize()
-    a64, w64 = a.astype(np.float64), w.astype(np.float64)
+    x64, w64 = a.astype(np.float64), w.astype(np.float64)
+    a64 = R.silu(x64) if act else x64
     ref = R.convT(a64, w64, 1)[..., 0] + bmap.reshape(2 * H, 2 * W)
     rdin, rdw = R.convT_bwd(a64, w64, 1, dout.astype(np.float64)[..., None])
+    if act:
+        rdin = rdin * R.dsilu(x64)
     def rel(x, r):
         return float(np.max(np.abs(x - r)) / np.max(np.abs(r)))
     assert rel(out_d.cpu().numpy(), ref) < 1e-6
