@@ -415,27 +415,40 @@ __global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in,
 
 // The 16 output-gradient values a pixel's taps reach: d[ky*4 + kx] =
 // dout[b][2y - 1 + ky][2x - 1 + kx] (0 outside the output grid).
+// (32-bit offsets inside one image: the host keeps 2H x 2W below 2^31.)
 __device__ inline void conv1_window(const float* __restrict__ dout, int64_t b, int y, int x, int H2, int W2,
                                     float (&d)[16])
 {
+    const float* img = dout + b * H2 * (int64_t)W2;
+    const int base = (2 * y - 1) * W2 + (2 * x - 1);
+    bool xok[4];
+#pragma unroll
+    for (int kx = 0; kx < 4; ++kx) xok[kx] = 2 * x - 1 + kx >= 0 && 2 * x - 1 + kx < W2;
 #pragma unroll
     for (int ky = 0; ky < 4; ++ky) {
-        const int Y = 2 * y - 1 + ky;
-        const bool yok = Y >= 0 && Y < H2;
+        const bool yok = 2 * y - 1 + ky >= 0 && 2 * y - 1 + ky < H2;
 #pragma unroll
-        for (int kx = 0; kx < 4; ++kx) {
-            const int X = 2 * x - 1 + kx;
-            d[ky * 4 + kx] = (yok && X >= 0 && X < W2) ? dout[(b * H2 + Y) * W2 + X] : 0.f;
-        }
+        for (int kx = 0; kx < 4; ++kx) d[ky * 4 + kx] = (yok && xok[kx]) ? img[base + ky * W2 + kx] : 0.f;
     }
+}
+
+// n / d for n, d < 2^31 through an FP64 reciprocal (inv = 1 / d) and one
+// exact correction step.
+__device__ inline int64_t udiv31(int64_t n, int64_t d, double inv)
+{
+    int64_t q = (int64_t)((double)n * inv);
+    const int64_t r = n - q * d;
+    if (r < 0) --q;
+    else if (r >= d) ++q;
+    return q;
 }
 
 // din[p][c] = sum_tap window(p)[tap] w[tap][c], times act'(in[p][c]) with
 // act = SiLU (din is then the gradient of the pre-activation in); thread =
 // (pixel, 4 channels).
 __global__ __launch_bounds__(kWG) void k_conv1_din(const float* __restrict__ dout, const float* __restrict__ in,
-                                                   int B, int H, int W, const float* __restrict__ w, int act,
-                                                   float* __restrict__ din)
+                                                   int B, int H, int W, double inv_w, double inv_h,
+                                                   const float* __restrict__ w, int act, float* __restrict__ din)
 {
     __shared__ f4 sw[16][kC1Ci / 4];
     if (threadIdx.x < 16 * kC1Ci / 4) sw[threadIdx.x / (kC1Ci / 4)][threadIdx.x % (kC1Ci / 4)] =
@@ -446,8 +459,8 @@ __global__ __launch_bounds__(kWG) void k_conv1_din(const float* __restrict__ dou
     if (g >= P * (kC1Ci / 4)) return;
     const int q = (int)(g % (kC1Ci / 4));
     const int64_t p = g / (kC1Ci / 4);
-    const int x = (int)(p % W), y = (int)((p / W) % H);
-    const int64_t b = p / ((int64_t)W * H);
+    const int64_t r = udiv31(p, W, inv_w), b = udiv31(r, H, inv_h);
+    const int x = (int)(p - r * W), y = (int)(r - b * H);
     float d[16];
     conv1_window(dout, b, y, x, 2 * H, 2 * W, d);
     f4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -476,8 +489,8 @@ __global__ __launch_bounds__(kWG) void k_conv1_din(const float* __restrict__ dou
 // wave are summed with shuffles, then the waves through LDS, in a fixed order.
 constexpr int kC1DwWG = 512;
 __global__ __launch_bounds__(kC1DwWG) void k_conv1_dw(const float* __restrict__ dout, const float* __restrict__ in,
-                                                      int B, int H, int W, int64_t per_wg, int act,
-                                                      float* __restrict__ part)
+                                                      int B, int H, int W, double inv_w, double inv_h,
+                                                      int64_t per_wg, int act, float* __restrict__ part)
 {
     constexpr int NWV = kC1DwWG / 64;
     __shared__ float red[NWV][16 * kC1Ci];
@@ -488,10 +501,12 @@ __global__ __launch_bounds__(kC1DwWG) void k_conv1_dw(const float* __restrict__ 
     float acc[16][4];
 #pragma unroll
     for (int t = 0; t < 16; ++t) acc[t][0] = acc[t][1] = acc[t][2] = acc[t][3] = 0.f;
-    for (int64_t p = p0 + pl; p < p1; p += kC1DwWG / 8) {
+    // pixel coordinates of p, advanced by the step instead of divided each time
+    constexpr int STEP = kC1DwWG / 8;
+    int64_t rr = udiv31(p0 + pl, W, inv_w), b = udiv31(rr, H, inv_h);
+    int x = (int)(p0 + pl - rr * W), y = (int)(rr - b * H);
+    for (int64_t p = p0 + pl; p < p1; p += STEP) {
         const f4 a = act4(*reinterpret_cast<const f4*>(in + p * kC1Ci + 4 * q), act);
-        const int x = (int)(p % W), y = (int)((p / W) % H);
-        const int64_t b = p / ((int64_t)W * H);
         float d[16];
         conv1_window(dout, b, y, x, 2 * H, 2 * W, d);
 #pragma unroll
@@ -500,6 +515,14 @@ __global__ __launch_bounds__(kC1DwWG) void k_conv1_dw(const float* __restrict__ 
             acc[t][1] = fmaf(d[t], a[1], acc[t][1]);
             acc[t][2] = fmaf(d[t], a[2], acc[t][2]);
             acc[t][3] = fmaf(d[t], a[3], acc[t][3]);
+        }
+        x += STEP;
+        while (x >= W) {
+            x -= W;
+            if (++y == H) {
+                y = 0;
+                ++b;
+            }
         }
     }
     // sum the 8 pixel lanes of this wave (lane bits 3..5)
@@ -1211,7 +1234,7 @@ int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B
     if ((din || dw) && conv1_direct(B, H, W, Ci, (dw || act) ? in : nullptr, din ? w : nullptr, din)) {
         if (din) {
             hipLaunchKernelGGL(k_conv1_din, dim3((unsigned)cdiv(P * (kC1Ci / 4), kWG)), dim3(kWG), 0, ctx->stream,
-                               dout, in, (int)B, (int)H, (int)W, w, act, din);
+                               dout, in, (int)B, (int)H, (int)W, 1.0 / (double)W, 1.0 / (double)H, w, act, din);
             if (int rc = check_launch(ctx, "k_conv1_din")) return rc;
         }
         if (dw) {
@@ -1221,7 +1244,7 @@ int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B
             nwg = cdiv(P, per);
             if (int rc = scratch(ctx, "nn_conv1_dw", (size_t)nwg * 16 * kC1Ci * sizeof(float), &p)) return rc;
             hipLaunchKernelGGL(k_conv1_dw, dim3((unsigned)nwg), dim3(kC1DwWG), 0, ctx->stream, dout, in, (int)B,
-                               (int)H, (int)W, per, act, (float*)p);
+                               (int)H, (int)W, 1.0 / (double)W, 1.0 / (double)H, per, act, (float*)p);
             if (int rc = check_launch(ctx, "k_conv1_dw")) return rc;
             if (int rc = colsum(ctx, (const float*)p, 16 * kC1Ci, nwg, 16 * kC1Ci, 1.f, 0.f, dw, 16 * kC1Ci,
                                 16 * kC1Ci, "nn_conv1_dw_sum"))
