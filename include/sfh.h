@@ -423,6 +423,13 @@ int sfh_nn_im2col(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t
  * rows of ldo >= B H W, dtype, zero past B H W) with Co in {4, 8, 16, 32}. */
 int sfh_nn_im2col_t(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t H, int64_t W, int64_t Co,
                     void* dcolsT, int64_t ldo);
+/* Gradient of that transposed convolution's input: out (B Hi Wi x Ci, FP32,
+ * row stride ldo) = im2col(d) wT^T, d the FP32 output gradient (B x 2Hi x 2Wi x
+ * Co), wT (dtype, Ci rows of ldw >= 16 Co, column tap Co + co) -- the product of
+ * sfh_nn_im2col and sfh_nn_gemm_nt with the column matrix gathered as it is
+ * read instead of written (Co a multiple of 8). */
+int sfh_nn_conv_t_dinput(sfh_ctx* ctx, int dtype, const float* d, int64_t B, int64_t Hi, int64_t Wi, int64_t Co,
+                         const void* wT, int64_t ldw, int64_t Ci, float* out, int64_t ldo);
 /* The same transposed convolution with one output channel (B x 2H x 2W) and a
  * per-pixel bias map, applied to act(in) (act 0: identity, 1: SiLU, in then
  * holding pre-activations); its backward gives din (the gradient of in:

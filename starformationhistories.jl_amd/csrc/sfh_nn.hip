@@ -551,6 +551,99 @@ __global__ __launch_bounds__(kC1DwWG) void k_conv1_dw(const float* __restrict__ 
     }
 }
 
+// ------------------------------------- transposed-convolution input gradient
+
+__device__ inline void to_frag(f4 v0, f4 v1, Frag<float>& f)
+{
+    f.v[0] = v0;
+    f.v[1] = v1;
+}
+__device__ inline void to_frag(f4 v0, f4 v1, Frag<uint16_t>& f)
+{
+    f.v = u32x4{(uint32_t)to_bf16(v0[0]) | ((uint32_t)to_bf16(v0[1]) << 16),
+                (uint32_t)to_bf16(v0[2]) | ((uint32_t)to_bf16(v0[3]) << 16),
+                (uint32_t)to_bf16(v1[0]) | ((uint32_t)to_bf16(v1[1]) << 16),
+                (uint32_t)to_bf16(v1[2]) | ((uint32_t)to_bf16(v1[3]) << 16)};
+}
+
+// Gradient of a stride-2, 4x4, padding-1 transposed convolution's input:
+// out[p][ci] = sum_{tap, co} d[b][2y - 1 + ky][2x - 1 + kx][co] wT[ci][tap Co + co]
+// for input pixel p = (b, y, x), tap = ky 4 + kx.  The product of the im2col
+// rows of d (K = 16 Co) with wT, with 256 x 32 workgroup tiles (4 waves x two
+// 32-row MFMA tiles) and register double buffering as k_gemm_nt<.., 4, 1, 2, 1>,
+// except that the A operand is gathered from the FP32 gradient d as it is
+// loaded (a lane's 8 consecutive k are 8 consecutive channels of one output
+// pixel, zero outside the grid) and rounded to T, so the column matrix is
+// never written.  Rows run along grid.x, 32-column tiles along grid.y.
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d, int B, int Hi, int Wi, int Co,
+                                                     double inv_w, double inv_h, const T* __restrict__ wT,
+                                                     int64_t ldw, int64_t Ci, float* __restrict__ out, int64_t ldo)
+{
+    constexpr int TM = 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t P = (int64_t)B * Hi * Wi;
+    const int64_t m0 = (int64_t)blockIdx.x * 256 + wave * (TM * 32);
+    const int64_t n0 = (int64_t)blockIdx.y * 32;
+    const int r = lane & 31, h = lane >> 5;
+    const int H2 = 2 * Hi, W2 = 2 * Wi, K = 16 * Co;
+    const float* img[TM];
+    int y2[TM], x2[TM];
+    bool oka[TM];
+#pragma unroll
+    for (int a = 0; a < TM; ++a) {
+        const int64_t p = m0 + a * 32 + r;
+        oka[a] = p < P;
+        const int64_t pp = oka[a] ? p : 0;
+        const int64_t rr = udiv31(pp, Wi, inv_w), b = udiv31(rr, Hi, inv_h);
+        img[a] = d + b * H2 * (int64_t)W2 * Co;
+        x2[a] = 2 * (int)(pp - rr * Wi) - 1;
+        y2[a] = 2 * (int)(rr - b * Hi) - 1;
+    }
+    const int64_t col = n0 + r;
+    const bool okb = col < Ci;
+    const T* pb = wT + (okb ? col : 0) * ldw + 8 * h;
+    auto load_a = [&](int a, int k0, bool more, Frag<T>& f) {
+        const int tap = k0 / Co, co0 = k0 - tap * Co;
+        const int Y = y2[a] + (tap >> 2), X = x2[a] + (tap & 3);
+        const bool ok = more && oka[a] && Y >= 0 && Y < H2 && X >= 0 && X < W2;
+        f4 v0 = {0.f, 0.f, 0.f, 0.f}, v1 = {0.f, 0.f, 0.f, 0.f};
+        if (ok) {
+            const f4* s = reinterpret_cast<const f4*>(img[a] + ((int64_t)Y * W2 + X) * Co + co0);
+            v0 = s[0];
+            v1 = s[1];
+        }
+        to_frag(v0, v1, f);
+    };
+    f32x16 acc[TM];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[a][i] = 0.f;
+    Frag<T> fa[TM], fb, na[TM], nb;
+#pragma unroll
+    for (int a = 0; a < TM; ++a) load_a(a, 8 * h, true, fa[a]);
+    load_frag(pb, okb, fb);
+    for (int k = 0; k < K; k += 16) {
+        const bool more = k + 16 < K;
+#pragma unroll
+        for (int a = 0; a < TM; ++a) load_a(a, (more ? k + 16 : 0) + 8 * h, more, na[a]);
+        load_frag(pb + (more ? k + 16 : 0), okb && more, nb);
+#pragma unroll
+        for (int a = 0; a < TM; ++a) acc[a] = mma(fa[a], fb, acc[a]);
+#pragma unroll
+        for (int a = 0; a < TM; ++a) fa[a] = na[a];
+        fb = nb;
+    }
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int64_t row = m0 + a * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+            if (row < P && okb) out[row * ldo + col] = acc[a][i];
+        }
+}
+
 // Four consecutive channels per thread (Co % 4 == 0, ldc / ldd % 4 == 0),
 // 32-bit index arithmetic (P * ldd < 2^31).
 template <typename T>
@@ -1291,6 +1384,28 @@ int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B
                             "nn_colsum_map"))
             return rc;
     return SFH_OK;
+}
+
+int sfh_nn_conv_t_dinput(sfh_ctx* ctx, int dtype, const float* d, int64_t B, int64_t Hi, int64_t Wi, int64_t Co,
+                         const void* wT, int64_t ldw, int64_t Ci, float* out, int64_t ldo)
+{
+    SFH_NN_BEGIN(ctx);
+    const int64_t P = B * Hi * Wi;
+    SFH_ARG_CHECK(ctx, d && wT && out && (dtype == 0 || dtype == 1) && B >= 0 && Hi >= 1 && Wi >= 1 && Co >= 8 &&
+                           Co % 8 == 0 && Ci >= 1 && ldw >= 16 * Co && ldw % 8 == 0 && ldo >= Ci &&
+                           aligned16(d) && aligned16(wT) && P < ((int64_t)1 << 31) &&
+                           4 * Hi * Wi * Co < ((int64_t)1 << 31),
+                  "sfh_nn_conv_t_dinput: Co a multiple of 8, ldw >= 16 Co (a multiple of 8), ldo >= Ci, "
+                  "16-byte aligned d and wT, B Hi Wi < 2^31");
+    if (P == 0) return SFH_OK;
+    const dim3 grid((unsigned)cdiv(P, 256), (unsigned)cdiv(Ci, 32));
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_conv_dinput<uint16_t>, grid, dim3(kWG), 0, ctx->stream, d, (int)B, (int)Hi, (int)Wi,
+                           (int)Co, 1.0 / (double)Wi, 1.0 / (double)Hi, (const uint16_t*)wT, ldw, Ci, out, ldo);
+    else
+        hipLaunchKernelGGL(k_conv_dinput<float>, grid, dim3(kWG), 0, ctx->stream, d, (int)B, (int)Hi, (int)Wi,
+                           (int)Co, 1.0 / (double)Wi, 1.0 / (double)Hi, (const float*)wT, ldw, Ci, out, ldo);
+    return check_launch(ctx, "k_conv_dinput");
 }
 
 int sfh_nn_mse(sfh_ctx* ctx, const float* out, const float* target, const int64_t* rows, int64_t B, int64_t H2,

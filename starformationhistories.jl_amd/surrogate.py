@@ -149,7 +149,7 @@ class TemplateSurrogate:
         self.dout = self._buf(B * OUT_H * OUT_W)
         self.dA = self._buf(max(P4 * CH, B * LAT))            # gradients w.r.t. activations (FP32)
         self.dB = self._buf(max(P4 * CH, B * LAT))
-        self.dcols = self._buf(max(P3 * 16 * CH, B * LAT), t)     # im2col rows / low-precision copies
+        self.dcols = self._buf(B * LAT, t)                       # low-precision copies of MLP gradients
         self.dcolsT = self._buf(16 * CH * _pad16(P3) + LAT * Bp, t)
         self.actT = self._buf(CH * _pad16(P3) + HID1 * Bp, t)
         # low-precision weight copies and their transposes
@@ -271,17 +271,16 @@ class TemplateSurrogate:
             if li > 0:
                 self._act_bwd(d, pre_out, Po * CH)               # -> gradient w.r.t. pre_out
             self._colsum(d, CH, Po, CH, g[cb])
-            self._c(self.lib.sfh_nn_im2col(self.ctx.handle, self.dt, d.data_ptr(), B, Hi, Wi, CH,
-                                           self.dcols.data_ptr(), 16 * CH))
             # dK = dcols^T a_in: transposed operands (K = pixels, padded)
             Pp = _pad16(Pi)
             self._c(self.lib.sfh_nn_im2col_t(self.ctx.handle, self.dt, d.data_ptr(), B, Hi, Wi, CH,
                                              self.dcolsT.data_ptr(), Pp))
             self._transpose(pre_in, CH, Pi, CH, self.actT, Pp, act=SILU)
             self._gemm(self.dcolsT, Pp, self.actT, Pp, g[K], CH, 16 * CH, CH, Pp)
-            # d a_in = dcols K
+            # d a_in = im2col(d) K, the column matrix gathered as the product reads it
             dn = self.dB if d is self.dA else self.dA
-            self._gemm(self.dcols, 16 * CH, self.wT[K], 16 * CH, dn, CH, Pi, CH, 16 * CH)
+            self._c(self.lib.sfh_nn_conv_t_dinput(self.ctx.handle, self.dt, d.data_ptr(), B, Hi, Wi, CH,
+                                                  self.wT[K].data_ptr(), 16 * CH, CH, dn.data_ptr(), CH))
             d = dn
             H, W = Hi, Wi
         # MLP; d is the gradient w.r.t. a2 (B x LAT)

@@ -182,6 +182,32 @@ def test_output_layer_against_oracle(sfh, gpu_ctx, B, H, W, Ci, act):
     assert rel(dbm_d.cpu().numpy(), dout.astype(np.float64).sum(0).ravel()) < 1e-6
 
 
+@pytest.mark.parametrize("dtype", [0, 1])
+@pytest.mark.parametrize("B,Hi,Wi,Co,Ci", [(2, 13, 19, 32, 32), (1, 5, 7, 16, 48), (3, 26, 38, 32, 32)])
+def test_conv_t_dinput_equals_im2col_product(sfh, gpu_ctx, dtype, B, Hi, Wi, Co, Ci):
+    """The input gradient with the column matrix gathered as it is read is the
+    product of the written column matrix (sfh_nn_im2col) with the weights, to
+    the bit: same rounded operands, same MFMA order."""
+    import torch
+    rng = np.random.default_rng(B * 100 + Hi + Wi)
+    tdt = torch.bfloat16 if dtype == 0 else torch.float32
+    d = torch.from_numpy(rng.normal(size=(B, 2 * Hi, 2 * Wi, Co)).astype(np.float32)).cuda()
+    wT = torch.from_numpy(rng.normal(size=(Ci, 16 * Co)).astype(np.float32)).cuda().to(tdt)
+    P = B * Hi * Wi
+    cols = torch.empty((P, 16 * Co), dtype=tdt, device="cuda")
+    ref = torch.empty((P, Ci), dtype=torch.float32, device="cuda")
+    out = torch.full((P, Ci), float("nan"), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    lib, h = gpu_ctx.lib, gpu_ctx.handle
+    gpu_ctx.check(lib.sfh_nn_im2col(h, dtype, d.data_ptr(), B, Hi, Wi, Co, cols.data_ptr(), 16 * Co))
+    gpu_ctx.check(lib.sfh_nn_gemm_nt(h, dtype, cols.data_ptr(), 16 * Co, wT.data_ptr(), 16 * Co, ref.data_ptr(), Ci,
+                                     P, Ci, 16 * Co, 1.0, 0.0))
+    gpu_ctx.check(lib.sfh_nn_conv_t_dinput(h, dtype, d.data_ptr(), B, Hi, Wi, Co, wT.data_ptr(), 16 * Co, Ci,
+                                           out.data_ptr(), Ci))
+    gpu_ctx.synchronize()
+    assert torch.equal(out, ref)
+
+
 def test_fill_templates_bookkeeping(sfh, gpu_ctx):
     """An untrained network fails every check (all templates exact); an
     infinite threshold keeps the network everywhere off the lattice."""
