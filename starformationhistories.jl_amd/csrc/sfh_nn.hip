@@ -75,6 +75,13 @@ __device__ inline void load_frag<float>(const float* __restrict__ p, bool ok, Fr
     f.v[1] = ok ? *reinterpret_cast<const f4*>(p + 4) : f4{0.f, 0.f, 0.f, 0.f};
 }
 
+__device__ inline void load_frag_lds(const uint16_t* p, Frag<uint16_t>& f) { f.v = *reinterpret_cast<const u32x4*>(p); }
+__device__ inline void load_frag_lds(const float* p, Frag<float>& f)
+{
+    f.v[0] = *reinterpret_cast<const f4*>(p);
+    f.v[1] = *reinterpret_cast<const f4*>(p + 4);
+}
+
 __device__ inline f32x16 mma(const Frag<uint16_t>& a, const Frag<uint16_t>& b, f32x16 c)
 {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a.v), __builtin_bit_cast(bf16x8, b.v),
@@ -642,6 +649,94 @@ __global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d
             const int64_t row = m0 + a * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
             if (row < P && okb) out[row * ldo + col] = acc[a][i];
         }
+}
+
+// out[0..n) = v (a kernel, so that the call can sit in a captured graph)
+__global__ __launch_bounds__(kWG) void k_fill(float* __restrict__ out, int64_t n, float v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) out[i] = v;
+}
+
+// ------------------------------------ transposed-convolution weight gradient
+
+// Partial weight gradients of a stride-2, 4x4, padding-1 transposed
+// convolution with 32 input and 32 output channels:
+//   part[s][tap 32 + co][ci] = sum over pixels p of split s of
+//                              d[b][2y - 1 + ky][2x - 1 + kx][co] act(pre[p][ci])
+// (tap = ky 4 + kx).  Workgroup = (ky, split); its 4 waves are kx = 0..3 and
+// each owns one 32 x 32 tile.  Per chunk of 64 pixels the activated inputs and
+// the four taps' gradient windows are staged in LDS channel-major ([channel]
+// [pixel], rows padded so a lane's 16-byte fragment reads are conflict-free),
+// rounded to T; then every wave runs four 32x32x16 MFMAs.  Operands are read
+// from the tensors directly: no transposed column matrix, no transposed input.
+constexpr int kDwC = 32, kDwChunk = 64;
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_conv_dweight(const float* __restrict__ d, const float* __restrict__ pre,
+                                                      int act, int B, int Hi, int Wi, double inv_w, double inv_h,
+                                                      int64_t per, float* __restrict__ part)
+{
+    constexpr int PADL = 16 / sizeof(T);            // 16 bytes of padding per row
+    constexpr int LD = kDwChunk + PADL;
+    __shared__ __align__(16) T sA[kDwC][LD];       // act(pre)[ci][pixel]
+    __shared__ __align__(16) T sD[4][kDwC][LD];    // d window of tap (ky, kx)[co][pixel]
+    __shared__ int64_t simg[kDwChunk];             // per chunk pixel: offset of its image in d, or -1
+    __shared__ int sy2[kDwChunk], sx2[kDwChunk];
+    const int ky = blockIdx.x;
+    const int64_t P = (int64_t)B * Hi * Wi;
+    const int64_t p0 = (int64_t)blockIdx.y * per;
+    const int64_t p1 = p0 + per < P ? p0 + per : P;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 31, h = lane >> 5;
+    const int H2 = 2 * Hi, W2 = 2 * Wi;
+    f32x16 acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+    for (int64_t c0 = p0; c0 < p1; c0 += kDwChunk) {
+        __syncthreads();                           // the previous chunk's fragments are read
+        if (threadIdx.x < kDwChunk) {
+            const int64_t p = c0 + threadIdx.x;
+            if (p < p1) {
+                const int64_t rr = udiv31(p, Wi, inv_w), b = udiv31(rr, Hi, inv_h);
+                simg[threadIdx.x] = b * H2 * (int64_t)W2 * kDwC;
+                sx2[threadIdx.x] = 2 * (int)(p - rr * Wi) - 1;
+                sy2[threadIdx.x] = 2 * (int)(rr - b * Hi) - 1;
+            } else {
+                simg[threadIdx.x] = -1;
+            }
+        }
+        __syncthreads();
+        // activated inputs: 64 pixels x 8 channel quads
+        for (int i = threadIdx.x; i < kDwChunk * 8; i += kWG) {
+            const int pl = i >> 3, q = i & 7;
+            f4 v = {0.f, 0.f, 0.f, 0.f};
+            if (simg[pl] >= 0) v = act4(*reinterpret_cast<const f4*>(pre + (c0 + pl) * kDwC + 4 * q), act);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sA[4 * q + j][pl] = from_f32<T>(v[j]);
+        }
+        // gradient windows of the four taps of this ky
+        for (int i = threadIdx.x; i < 4 * kDwChunk * 8; i += kWG) {
+            const int kx = i / (kDwChunk * 8), pl = (i >> 3) % kDwChunk, q = i & 7;
+            f4 v = {0.f, 0.f, 0.f, 0.f};
+            if (simg[pl] >= 0) {
+                const int Y = sy2[pl] + ky, X = sx2[pl] + kx;
+                if (Y >= 0 && Y < H2 && X >= 0 && X < W2)
+                    v = *reinterpret_cast<const f4*>(d + simg[pl] + ((int64_t)Y * W2 + X) * kDwC + 4 * q);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sD[kx][4 * q + j][pl] = from_f32<T>(v[j]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kDwChunk / 16; ++ks) {
+            Frag<T> fa, fb;
+            load_frag_lds(&sD[wave][r][ks * 16 + 8 * h], fa);
+            load_frag_lds(&sA[r][ks * 16 + 8 * h], fb);
+            acc = mma(fa, fb, acc);
+        }
+    }
+    float* o = part + (int64_t)blockIdx.y * (16 * kDwC * kDwC) + (int64_t)(ky * 4 + wave) * kDwC * kDwC;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) o[(8 * (i >> 2) + 4 * h + (i & 3)) * kDwC + r] = acc[i];
 }
 
 // Four consecutive channels per thread (Co % 4 == 0, ldc / ldd % 4 == 0),
@@ -1406,6 +1501,37 @@ int sfh_nn_conv_t_dinput(sfh_ctx* ctx, int dtype, const float* d, int64_t B, int
         hipLaunchKernelGGL(k_conv_dinput<float>, grid, dim3(kWG), 0, ctx->stream, d, (int)B, (int)Hi, (int)Wi,
                            (int)Co, 1.0 / (double)Wi, 1.0 / (double)Hi, (const float*)wT, ldw, Ci, out, ldo);
     return check_launch(ctx, "k_conv_dinput");
+}
+
+int sfh_nn_conv_t_dweight(sfh_ctx* ctx, int dtype, const float* d, const float* pre, int act, int64_t B, int64_t Hi,
+                          int64_t Wi, int64_t C, float* dK)
+{
+    SFH_NN_BEGIN(ctx);
+    const int64_t P = B * Hi * Wi;
+    SFH_ARG_CHECK(ctx, d && pre && dK && (dtype == 0 || dtype == 1) && (act == 0 || act == 1) && C == kDwC &&
+                           B >= 0 && Hi >= 1 && Wi >= 1 && aligned16(d) && aligned16(pre) &&
+                           P < ((int64_t)1 << 31) && 4 * Hi * Wi * C < ((int64_t)1 << 31),
+                  "sfh_nn_conv_t_dweight: 32 channels, act 0 or 1, 16-byte aligned d and pre, B Hi Wi < 2^31");
+    constexpr int64_t NOUT = 16 * kDwC * kDwC;
+    if (P == 0) {
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(NOUT)), dim3(kWG), 0, ctx->stream, dK, NOUT, 0.f);
+        return check_launch(ctx, "k_fill");
+    }
+    // pixel splits: one partial tile set per split, at most 256 (a single-pass column sum)
+    int64_t nsplit = std::min<int64_t>(256, cdiv(P, kDwChunk));
+    const int64_t per = cdiv(cdiv(P, nsplit), kDwChunk) * kDwChunk;
+    nsplit = cdiv(P, per);
+    void* p = nullptr;
+    if (int rc = scratch(ctx, "nn_conv_dw", (size_t)nsplit * NOUT * sizeof(float), &p)) return rc;
+    const dim3 grid(4, (unsigned)nsplit);
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_conv_dweight<uint16_t>, grid, dim3(kWG), 0, ctx->stream, d, pre, act, (int)B, (int)Hi,
+                           (int)Wi, 1.0 / (double)Wi, 1.0 / (double)Hi, per, (float*)p);
+    else
+        hipLaunchKernelGGL(k_conv_dweight<float>, grid, dim3(kWG), 0, ctx->stream, d, pre, act, (int)B, (int)Hi,
+                           (int)Wi, 1.0 / (double)Wi, 1.0 / (double)Hi, per, (float*)p);
+    if (int rc = check_launch(ctx, "k_conv_dweight")) return rc;
+    return colsum(ctx, (const float*)p, NOUT, nsplit, NOUT, 1.f, 0.f, dK, NOUT, NOUT, "nn_conv_dw_sum");
 }
 
 int sfh_nn_mse(sfh_ctx* ctx, const float* out, const float* target, const int64_t* rows, int64_t B, int64_t H2,

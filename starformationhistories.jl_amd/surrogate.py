@@ -150,8 +150,8 @@ class TemplateSurrogate:
         self.dA = self._buf(max(P4 * CH, B * LAT))            # gradients w.r.t. activations (FP32)
         self.dB = self._buf(max(P4 * CH, B * LAT))
         self.dcols = self._buf(B * LAT, t)                       # low-precision copies of MLP gradients
-        self.dcolsT = self._buf(16 * CH * _pad16(P3) + LAT * Bp, t)
-        self.actT = self._buf(CH * _pad16(P3) + HID1 * Bp, t)
+        self.dcolsT = self._buf(LAT * Bp, t)                     # transposed MLP gradients
+        self.actT = self._buf(HID1 * Bp, t)                       # transposed MLP activations
         # low-precision weight copies and their transposes
         self.w = {k: self._buf(self.views[k].numel(), t) for k in ("W0", "W1", "W2", "K1", "K2", "K3")}
         self.wT = {k: self._buf(self.views[k].numel(), t) for k in ("W1", "W2", "K1", "K2", "K3")}
@@ -271,12 +271,9 @@ class TemplateSurrogate:
             if li > 0:
                 self._act_bwd(d, pre_out, Po * CH)               # -> gradient w.r.t. pre_out
             self._colsum(d, CH, Po, CH, g[cb])
-            # dK = dcols^T a_in: transposed operands (K = pixels, padded)
-            Pp = _pad16(Pi)
-            self._c(self.lib.sfh_nn_im2col_t(self.ctx.handle, self.dt, d.data_ptr(), B, Hi, Wi, CH,
-                                             self.dcolsT.data_ptr(), Pp))
-            self._transpose(pre_in, CH, Pi, CH, self.actT, Pp, act=SILU)
-            self._gemm(self.dcolsT, Pp, self.actT, Pp, g[K], CH, 16 * CH, CH, Pp)
+            # dK = im2col(d)^T SiLU(pre_in), both operands gathered into LDS by the kernel
+            self._c(self.lib.sfh_nn_conv_t_dweight(self.ctx.handle, self.dt, d.data_ptr(), pre_in.data_ptr(), SILU,
+                                                   B, Hi, Wi, CH, g[K].data_ptr()))
             # d a_in = im2col(d) K, the column matrix gathered as the product reads it
             dn = self.dB if d is self.dA else self.dA
             self._c(self.lib.sfh_nn_conv_t_dinput(self.ctx.handle, self.dt, d.data_ptr(), B, Hi, Wi, CH,

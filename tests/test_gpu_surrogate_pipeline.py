@@ -208,6 +208,44 @@ def test_conv_t_dinput_equals_im2col_product(sfh, gpu_ctx, dtype, B, Hi, Wi, Co,
     assert torch.equal(out, ref)
 
 
+@pytest.mark.parametrize("act", [0, 1])
+@pytest.mark.parametrize("B,Hi,Wi", [(2, 13, 19), (1, 5, 7), (3, 26, 38)])
+def test_conv_t_dweight(sfh, gpu_ctx, B, Hi, Wi, act):
+    """The decoder weight gradient with its operands gathered and transposed in
+    LDS: FP32 operands against the oracle's convT_bwd (FP64); BF16 operands
+    against the written transposed column matrix times the transposed
+    activations (same rounded operands, other summation order)."""
+    import torch
+    C = 32
+    rng = np.random.default_rng(B * 100 + Hi + Wi + act)
+    d = rng.normal(size=(B, 2 * Hi, 2 * Wi, C)).astype(np.float32)
+    pre = rng.normal(size=(B, Hi, Wi, C)).astype(np.float32)
+    d_d, pre_d = torch.from_numpy(d).cuda(), torch.from_numpy(pre).cuda()
+    out = {dt: torch.empty((16 * C, C), dtype=torch.float32, device="cuda") for dt in (0, 1)}
+    P, Pp = B * Hi * Wi, (B * Hi * Wi + 15) // 16 * 16
+    colsT = torch.zeros((16 * C, Pp), dtype=torch.bfloat16, device="cuda")
+    actT = torch.zeros((C, Pp), dtype=torch.bfloat16, device="cuda")
+    ref0 = torch.empty((16 * C, C), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    lib, h = gpu_ctx.lib, gpu_ctx.handle
+    for dt in (0, 1):
+        gpu_ctx.check(lib.sfh_nn_conv_t_dweight(h, dt, d_d.data_ptr(), pre_d.data_ptr(), act, B, Hi, Wi, C,
+                                                out[dt].data_ptr()))
+    gpu_ctx.check(lib.sfh_nn_im2col_t(h, 0, d_d.data_ptr(), B, Hi, Wi, C, colsT.data_ptr(), Pp))
+    gpu_ctx.check(lib.sfh_nn_transpose(h, 0, pre_d.data_ptr(), C, P, C, actT.data_ptr(), Pp, C, act))
+    gpu_ctx.check(lib.sfh_nn_gemm_nt(h, 0, colsT.data_ptr(), Pp, actT.data_ptr(), Pp, ref0.data_ptr(), C, 16 * C, C,
+                                     Pp, 1.0, 0.0))
+    gpu_ctx.synchronize()
+    x64 = pre.astype(np.float64)
+    a64 = R.silu(x64) if act else x64
+    _, rdk = R.convT_bwd(a64, np.zeros((16 * C, C)), C, d.astype(np.float64))
+    def rel(x, r):
+        return float(np.max(np.abs(x - r)) / np.max(np.abs(r)))
+    assert rel(out[1].cpu().numpy(), rdk) < 2e-5
+    assert rel(out[0].cpu().numpy(), ref0.cpu().numpy().astype(np.float64)) < 1e-4
+    assert rel(out[0].cpu().numpy(), rdk) < 2e-2
+
+
 def test_fill_templates_bookkeeping(sfh, gpu_ctx):
     """An untrained network fails every check (all templates exact); an
     infinite threshold keeps the network everywhere off the lattice."""
