@@ -430,6 +430,14 @@ int sfh_nn_im2col_t(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64
  * read instead of written (Co a multiple of 8). */
 int sfh_nn_conv_t_dinput(sfh_ctx* ctx, int dtype, const float* d, int64_t B, int64_t Hi, int64_t Wi, int64_t Co,
                          const void* wT, int64_t ldw, int64_t Ci, float* out, int64_t ldo);
+/* Forward of that transposed convolution with 32 input and 32 output channels:
+ * pre (B x 2Hi x 2Wi x 32, FP32) = convolution of in (dtype, B Hi Wi x 32) with
+ * w (dtype, 16 x 32 rows of 32: row tap 32 + co) plus bias[co] (may be NULL),
+ * and aout (dtype, may be NULL) = SiLU(pre).  The products of sfh_nn_gemm_nt
+ * and sfh_nn_col2im (and the SiLU of sfh_nn_bias_act) in one kernel, one output
+ * parity class per launch plane, the im2col operand gathered as it is read. */
+int sfh_nn_conv_t_fwd(sfh_ctx* ctx, int dtype, const void* in, int64_t B, int64_t Hi, int64_t Wi, int64_t C,
+                      const void* w, const float* bias, float* pre, void* aout);
 /* Weight gradient of that transposed convolution with 32 input and 32 output
  * channels: dK (16 x 32 rows of 32, row tap 32 + co, FP32) = sum over input
  * pixels p of im2col(d)[p][tap 32 + co] act(pre[p][ci]) (act 0: identity, 1:

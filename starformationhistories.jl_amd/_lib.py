@@ -146,6 +146,7 @@ SIGNATURES = [
     ("sfh_nn_col2im", _int, [_vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp]),
     ("sfh_nn_im2col", _int, [_vp, _int, _vp, _i64, _i64, _i64, _i64, _vp, _i64]),
     ("sfh_nn_im2col_t", _int, [_vp, _int, _vp, _i64, _i64, _i64, _i64, _vp, _i64]),
+    ("sfh_nn_conv_t_fwd", _int, [_vp, _int, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp]),
     ("sfh_nn_conv_t_dweight", _int, [_vp, _int, _vp, _vp, _int, _i64, _i64, _i64, _i64, _vp]),
     ("sfh_nn_conv_t_dinput", _int, [_vp, _int, _vp, _i64, _i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64]),
     ("sfh_nn_conv1_fwd", _int, [_vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _int, _vp]),

@@ -657,6 +657,95 @@ __global__ __launch_bounds__(kWG) void k_fill(float* __restrict__ out, int64_t n
     for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) out[i] = v;
 }
 
+// ----------------------------------------- transposed-convolution forward
+
+// Forward of a stride-2, 4x4, padding-1 transposed convolution with 32 input
+// and 32 output channels, by output parity class (py, px) = blockIdx.z: the
+// outputs (b, 2y + py, 2x + px) for every input pixel (b, y, x) receive the
+// four taps ky = ky0(py) + {0, 2}, kx = kx0(px) + {0, 2} from input pixels
+// y + (py + 1 - ky) / 2, x + (px + 1 - kx) / 2 (zero outside), so each class is
+// a product with M = input pixels, N = 32, K = 4 taps x 32 input channels: the
+// A rows gathered from the input as they are loaded, the B rows from the
+// tap-major weights w[tap 32 + co][ci].  Epilogue: pre = acc + bias[co]
+// (FP32) and, when aout is given, aout = SiLU(pre) rounded to T.  256 x 32
+// workgroup tiles as k_conv_dinput.
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_conv_fwd(const T* __restrict__ in, int B, int Hi, int Wi, double inv_w,
+                                                  double inv_h, const T* __restrict__ w, const float* __restrict__ bias,
+                                                  float* __restrict__ pre, T* __restrict__ aout)
+{
+    constexpr int TM = 2, C = 32, K = 4 * C;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int py = blockIdx.z >> 1, px = blockIdx.z & 1;
+    const int ky0 = (py + 1) & 1, kx0 = (px + 1) & 1;
+    const int64_t P = (int64_t)B * Hi * Wi;
+    const int64_t m0 = (int64_t)blockIdx.x * 256 + wave * (TM * 32);
+    const int r = lane & 31, h = lane >> 5;
+    const T* img[TM];
+    int yy[TM], xx[TM];
+    bool oka[TM];
+#pragma unroll
+    for (int a = 0; a < TM; ++a) {
+        const int64_t p = m0 + a * 32 + r;
+        oka[a] = p < P;
+        const int64_t pp = oka[a] ? p : 0;
+        const int64_t rr = udiv31(pp, Wi, inv_w), b = udiv31(rr, Hi, inv_h);
+        img[a] = in + b * Hi * (int64_t)Wi * C;
+        xx[a] = (int)(pp - rr * Wi);
+        yy[a] = (int)(rr - b * Hi);
+    }
+    // k = j 32 + ci with j = 2 dy + dx: tap (ky0 + 2 dy, kx0 + 2 dx)
+    auto load_a = [&](int a, int k0, Frag<T>& f) {
+        const int j = k0 >> 5, ci0 = k0 & 31;
+        const int ky = ky0 + 2 * (j >> 1), kx = kx0 + 2 * (j & 1);
+        const int y = yy[a] + (py + 1 - ky) / 2, x = xx[a] + (px + 1 - kx) / 2;
+        const bool ok = oka[a] && y >= 0 && y < Hi && x >= 0 && x < Wi;
+        load_frag(img[a] + ((int64_t)(ok ? y : 0) * Wi + (ok ? x : 0)) * C + ci0, ok, f);
+    };
+    auto load_b = [&](int k0, Frag<T>& f) {
+        const int j = k0 >> 5, ci0 = k0 & 31;
+        const int tap = (ky0 + 2 * (j >> 1)) * 4 + kx0 + 2 * (j & 1);
+        load_frag(w + (int64_t)(tap * C + r) * C + ci0, true, f);
+    };
+    f32x16 acc[TM];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[a][i] = 0.f;
+    Frag<T> fa[TM], fb, na[TM], nb;
+#pragma unroll
+    for (int a = 0; a < TM; ++a) load_a(a, 8 * h, fa[a]);
+    load_b(8 * h, fb);
+#pragma unroll
+    for (int k = 0; k < K; k += 16) {
+        if (k + 16 < K) {
+#pragma unroll
+            for (int a = 0; a < TM; ++a) load_a(a, k + 16 + 8 * h, na[a]);
+            load_b(k + 16 + 8 * h, nb);
+        }
+#pragma unroll
+        for (int a = 0; a < TM; ++a) acc[a] = mma(fa[a], fb, acc[a]);
+#pragma unroll
+        for (int a = 0; a < TM; ++a) fa[a] = na[a];
+        fb = nb;
+    }
+    const float bv = bias ? bias[r] : 0.f;
+    const int H2 = 2 * Hi, W2 = 2 * Wi;
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int64_t row = m0 + a * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+            if (row >= P) continue;
+            const int64_t rr = udiv31(row, Wi, inv_w), b = udiv31(rr, Hi, inv_h);
+            const int x = (int)(row - rr * Wi), y = (int)(rr - b * Hi);
+            const int64_t q = ((b * H2 + 2 * y + py) * (int64_t)W2 + 2 * x + px) * C + r;
+            const float v = acc[a][i] + bv;
+            pre[q] = v;
+            if (aout) aout[q] = from_f32<T>(v * sigmoid(v));
+        }
+}
+
 // ------------------------------------ transposed-convolution weight gradient
 
 // Partial weight gradients of a stride-2, 4x4, padding-1 transposed
@@ -1501,6 +1590,27 @@ int sfh_nn_conv_t_dinput(sfh_ctx* ctx, int dtype, const float* d, int64_t B, int
         hipLaunchKernelGGL(k_conv_dinput<float>, grid, dim3(kWG), 0, ctx->stream, d, (int)B, (int)Hi, (int)Wi,
                            (int)Co, 1.0 / (double)Wi, 1.0 / (double)Hi, (const float*)wT, ldw, Ci, out, ldo);
     return check_launch(ctx, "k_conv_dinput");
+}
+
+int sfh_nn_conv_t_fwd(sfh_ctx* ctx, int dtype, const void* in, int64_t B, int64_t Hi, int64_t Wi, int64_t C,
+                      const void* w, const float* bias, float* pre, void* aout)
+{
+    SFH_NN_BEGIN(ctx);
+    const int64_t P = B * Hi * Wi;
+    SFH_ARG_CHECK(ctx, in && w && pre && (dtype == 0 || dtype == 1) && C == 32 && B >= 0 && Hi >= 1 && Wi >= 1 &&
+                           aligned16(in) && aligned16(w) && P < ((int64_t)1 << 31) &&
+                           4 * P * C < ((int64_t)1 << 40),
+                  "sfh_nn_conv_t_fwd: 32 channels, 16-byte aligned in and w, B Hi Wi < 2^31");
+    if (P == 0) return SFH_OK;
+    const dim3 grid((unsigned)cdiv(P, 256), 1, 4);
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_conv_fwd<uint16_t>, grid, dim3(kWG), 0, ctx->stream, (const uint16_t*)in, (int)B,
+                           (int)Hi, (int)Wi, 1.0 / (double)Wi, 1.0 / (double)Hi, (const uint16_t*)w, bias, pre,
+                           (uint16_t*)aout);
+    else
+        hipLaunchKernelGGL(k_conv_fwd<float>, grid, dim3(kWG), 0, ctx->stream, (const float*)in, (int)B, (int)Hi,
+                           (int)Wi, 1.0 / (double)Wi, 1.0 / (double)Hi, (const float*)w, bias, pre, (float*)aout);
+    return check_launch(ctx, "k_conv_fwd");
 }
 
 int sfh_nn_conv_t_dweight(sfh_ctx* ctx, int dtype, const float* d, const float* pre, int act, int64_t B, int64_t Hi,

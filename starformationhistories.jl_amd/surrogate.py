@@ -136,11 +136,11 @@ class TemplateSurrogate:
         self.max_batch = B
         f, t = self.torch.float32, self.tdt
         Bp = _pad16(B)
-        P1, P2, P3, P4 = B * H0 * W0, B * 4 * H0 * W0, B * 16 * H0 * W0, B * 64 * H0 * W0
+        P2, P3, P4 = B * 4 * H0 * W0, B * 16 * H0 * W0, B * 64 * H0 * W0
         self.bp = Bp
         self.x0f = self._buf(B * 16)
         self.x0 = self._buf(B * 16, t)
-        self.z = self._buf(max(B * LAT, P1 * 16 * CH, P2 * 16 * CH, P3 * 16 * CH))   # GEMM outputs (FP32)
+        self.z = self._buf(B * max(LAT, HID0, HID1))             # MLP product outputs (FP32)
         self.pre0, self.pre1, self.pre2 = self._buf(B * HID0), self._buf(B * HID1), self._buf(B * LAT)
         self.a0, self.a1, self.a2 = self._buf(B * HID0, t), self._buf(B * HID1, t), self._buf(B * LAT, t)
         self.pre3, self.pre4, self.pre5 = self._buf(P2 * CH), self._buf(P3 * CH), self._buf(P4 * CH)
@@ -227,13 +227,11 @@ class TemplateSurrogate:
         pres = (self.pre3, self.pre4, self.pre5)
         outs = (self.a3, self.a4, None)
         for li, (K, cb) in enumerate((("K1", "c1"), ("K2", "c2"), ("K3", "c3"))):
-            P = B * H * W
-            self._gemm(ins[li], CH, self.w[K], CH, self.z, 16 * CH, P, 16 * CH, CH)
-            self._c(self.lib.sfh_nn_col2im(self.ctx.handle, self.z.data_ptr(), 16 * CH, B, H, W, CH,
-                                           v[cb].data_ptr(), pres[li].data_ptr()))
+            # pre-activation (+ bias) and, but for the last layer, its SiLU in one kernel
+            self._c(self.lib.sfh_nn_conv_t_fwd(self.ctx.handle, self.dt, ins[li].data_ptr(), B, H, W, CH,
+                                               self.w[K].data_ptr(), v[cb].data_ptr(), pres[li].data_ptr(),
+                                               outs[li].data_ptr() if outs[li] is not None else None))
             H, W = 2 * H, 2 * W
-            if outs[li] is not None:
-                self._bias_act(pres[li], CH, None, B * H * W, CH, SILU, None, outs[li], CH)
         # single-channel output layer on SiLU(pre5), the activation applied inside
         self._c(self.lib.sfh_nn_conv1_fwd(self.ctx.handle, self.pre5.data_ptr(), B, H, W, CH, v["w4"].data_ptr(),
                                           v["bmap"].data_ptr(), SILU, self.out.data_ptr()))

@@ -246,6 +246,49 @@ def test_conv_t_dweight(sfh, gpu_ctx, B, Hi, Wi, act):
     assert rel(out[0].cpu().numpy(), rdk) < 2e-2
 
 
+@pytest.mark.parametrize("dtype", [0, 1])
+@pytest.mark.parametrize("B,Hi,Wi", [(2, 13, 19), (1, 5, 7), (3, 26, 38)])
+def test_conv_t_fwd_against_products(sfh, gpu_ctx, dtype, B, Hi, Wi):
+    """The decoder forward by output parity class (gathered operands, bias and
+    SiLU in the epilogue) against the column product, col2im and bias/SiLU
+    kernels it replaces (same rounded operands), and the FP32 form against the
+    oracle's convT."""
+    import torch
+    C = 32
+    rng = np.random.default_rng(B * 10 + Hi + Wi)
+    tdt = torch.bfloat16 if dtype == 0 else torch.float32
+    x = rng.normal(size=(B, Hi, Wi, C)).astype(np.float32)
+    w = (0.2 * rng.normal(size=(16 * C, C))).astype(np.float32)
+    bias = rng.normal(size=C).astype(np.float32)
+    x_d = torch.from_numpy(x).cuda().to(tdt)
+    w_d = torch.from_numpy(w).cuda().to(tdt)
+    b_d = torch.from_numpy(bias).cuda()
+    P, H2, W2 = B * Hi * Wi, 2 * Hi, 2 * Wi
+    z = torch.empty((P, 16 * C), dtype=torch.float32, device="cuda")
+    pre_ref = torch.empty((B, H2, W2, C), dtype=torch.float32, device="cuda")
+    a_ref = torch.empty((B, H2, W2, C), dtype=tdt, device="cuda")
+    pre = torch.full((B, H2, W2, C), float("nan"), dtype=torch.float32, device="cuda")
+    a = torch.empty((B, H2, W2, C), dtype=tdt, device="cuda")
+    torch.cuda.synchronize()
+    lib, h = gpu_ctx.lib, gpu_ctx.handle
+    gpu_ctx.check(lib.sfh_nn_gemm_nt(h, dtype, x_d.data_ptr(), C, w_d.data_ptr(), C, z.data_ptr(), 16 * C, P, 16 * C,
+                                     C, 1.0, 0.0))
+    gpu_ctx.check(lib.sfh_nn_col2im(h, z.data_ptr(), 16 * C, B, Hi, Wi, C, b_d.data_ptr(), pre_ref.data_ptr()))
+    gpu_ctx.check(lib.sfh_nn_bias_act(h, dtype, pre_ref.data_ptr(), C, None, B * H2 * W2, C, 1, None,
+                                      a_ref.data_ptr(), C))
+    gpu_ctx.check(lib.sfh_nn_conv_t_fwd(h, dtype, x_d.data_ptr(), B, Hi, Wi, C, w_d.data_ptr(), b_d.data_ptr(),
+                                        pre.data_ptr(), a.data_ptr()))
+    gpu_ctx.synchronize()
+    def rel(u, v):
+        u, v = np.asarray(u, dtype=np.float64), np.asarray(v, dtype=np.float64)
+        return float(np.max(np.abs(u - v)) / np.max(np.abs(v)))
+    assert rel(pre.cpu().numpy(), pre_ref.cpu().numpy()) < 1e-5
+    assert rel(a.float().cpu().numpy(), a_ref.float().cpu().numpy()) < (1e-2 if dtype == 0 else 1e-5)
+    if dtype == 1:
+        ref = R.convT(x.astype(np.float64), w.astype(np.float64), C) + bias
+        assert rel(pre.cpu().numpy(), ref) < 1e-5
+
+
 def test_fill_templates_bookkeeping(sfh, gpu_ctx):
     """An untrained network fails every check (all templates exact); an
     infinite threshold keeps the network everywhere off the lattice."""
