@@ -427,7 +427,7 @@ int sfh_nn_im2col_t(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64
  * row stride ldo) = im2col(d) wT^T, d the FP32 output gradient (B x 2Hi x 2Wi x
  * Co), wT (dtype, Ci rows of ldw >= 16 Co, column tap Co + co) -- the product of
  * sfh_nn_im2col and sfh_nn_gemm_nt with the column matrix gathered as it is
- * read instead of written (Co a multiple of 8). */
+ * read instead of written (Co 8, 16 or 32). */
 int sfh_nn_conv_t_dinput(sfh_ctx* ctx, int dtype, const float* d, int64_t B, int64_t Hi, int64_t Wi, int64_t Co,
                          const void* wT, int64_t ldw, int64_t Ci, float* out, int64_t ldo);
 /* Forward of that transposed convolution with 32 input and 32 output channels:

@@ -582,8 +582,8 @@ __device__ inline void to_frag(f4 v0, f4 v1, Frag<uint16_t>& f)
 // loaded (a lane's 8 consecutive k are 8 consecutive channels of one output
 // pixel, zero outside the grid) and rounded to T, so the column matrix is
 // never written.  Rows run along grid.x, 32-column tiles along grid.y.
-template <typename T>
-__global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d, int B, int Hi, int Wi, int Co,
+template <typename T, int Co>
+__global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d, int B, int Hi, int Wi,
                                                      double inv_w, double inv_h, const T* __restrict__ wT,
                                                      int64_t ldw, int64_t Ci, float* __restrict__ out, int64_t ldo)
 {
@@ -593,7 +593,8 @@ __global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d
     const int64_t m0 = (int64_t)blockIdx.x * 256 + wave * (TM * 32);
     const int64_t n0 = (int64_t)blockIdx.y * 32;
     const int r = lane & 31, h = lane >> 5;
-    const int H2 = 2 * Hi, W2 = 2 * Wi, K = 16 * Co;
+    const int H2 = 2 * Hi, W2 = 2 * Wi;
+    constexpr int K = 16 * Co, NS = K / 16;
     const float* img[TM];
     int y2[TM], x2[TM];
     bool oka[TM];
@@ -627,20 +628,24 @@ __global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d
     for (int a = 0; a < TM; ++a)
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[a][i] = 0.f;
-    Frag<T> fa[TM], fb, na[TM], nb;
+    // a ring of three fragment sets: the loads of step s + 2 are issued
+    // before the products of step s
+    Frag<T> fa[3][TM], fb[3];
 #pragma unroll
-    for (int a = 0; a < TM; ++a) load_a(a, 8 * h, true, fa[a]);
-    load_frag(pb, okb, fb);
-    for (int k = 0; k < K; k += 16) {
-        const bool more = k + 16 < K;
+    for (int s = 0; s < 2 && s < NS; ++s) {
 #pragma unroll
-        for (int a = 0; a < TM; ++a) load_a(a, (more ? k + 16 : 0) + 8 * h, more, na[a]);
-        load_frag(pb + (more ? k + 16 : 0), okb && more, nb);
+        for (int a = 0; a < TM; ++a) load_a(a, 16 * s + 8 * h, true, fa[s][a]);
+        load_frag(pb + 16 * s, okb, fb[s]);
+    }
 #pragma unroll
-        for (int a = 0; a < TM; ++a) acc[a] = mma(fa[a], fb, acc[a]);
+    for (int s = 0; s < NS; ++s) {
+        if (s + 2 < NS) {
 #pragma unroll
-        for (int a = 0; a < TM; ++a) fa[a] = na[a];
-        fb = nb;
+            for (int a = 0; a < TM; ++a) load_a(a, 16 * (s + 2) + 8 * h, true, fa[(s + 2) % 3][a]);
+            load_frag(pb + 16 * (s + 2), okb, fb[(s + 2) % 3]);
+        }
+#pragma unroll
+        for (int a = 0; a < TM; ++a) acc[a] = mma(fa[s % 3][a], fb[s % 3], acc[a]);
     }
 #pragma unroll
     for (int a = 0; a < TM; ++a)
@@ -712,22 +717,25 @@ __global__ __launch_bounds__(kWG) void k_conv_fwd(const T* __restrict__ in, int 
     for (int a = 0; a < TM; ++a)
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[a][i] = 0.f;
-    Frag<T> fa[TM], fb, na[TM], nb;
+    // a ring of three fragment sets: the loads of step s + 2 are issued
+    // before the products of step s
+    constexpr int NS = K / 16;
+    Frag<T> fa[3][TM], fb[3];
 #pragma unroll
-    for (int a = 0; a < TM; ++a) load_a(a, 8 * h, fa[a]);
-    load_b(8 * h, fb);
+    for (int s = 0; s < 2; ++s) {
 #pragma unroll
-    for (int k = 0; k < K; k += 16) {
-        if (k + 16 < K) {
+        for (int a = 0; a < TM; ++a) load_a(a, 16 * s + 8 * h, fa[s][a]);
+        load_b(16 * s + 8 * h, fb[s]);
+    }
 #pragma unroll
-            for (int a = 0; a < TM; ++a) load_a(a, k + 16 + 8 * h, na[a]);
-            load_b(k + 16 + 8 * h, nb);
+    for (int s = 0; s < NS; ++s) {
+        if (s + 2 < NS) {
+#pragma unroll
+            for (int a = 0; a < TM; ++a) load_a(a, 16 * (s + 2) + 8 * h, fa[(s + 2) % 3][a]);
+            load_b(16 * (s + 2) + 8 * h, fb[(s + 2) % 3]);
         }
 #pragma unroll
-        for (int a = 0; a < TM; ++a) acc[a] = mma(fa[a], fb, acc[a]);
-#pragma unroll
-        for (int a = 0; a < TM; ++a) fa[a] = na[a];
-        fb = nb;
+        for (int a = 0; a < TM; ++a) acc[a] = mma(fa[s % 3][a], fb[s % 3], acc[a]);
     }
     const float bv = bias ? bias[r] : 0.f;
     const int H2 = 2 * Hi, W2 = 2 * Wi;
@@ -1575,20 +1583,24 @@ int sfh_nn_conv_t_dinput(sfh_ctx* ctx, int dtype, const float* d, int64_t B, int
 {
     SFH_NN_BEGIN(ctx);
     const int64_t P = B * Hi * Wi;
-    SFH_ARG_CHECK(ctx, d && wT && out && (dtype == 0 || dtype == 1) && B >= 0 && Hi >= 1 && Wi >= 1 && Co >= 8 &&
-                           Co % 8 == 0 && Ci >= 1 && ldw >= 16 * Co && ldw % 8 == 0 && ldo >= Ci &&
+    SFH_ARG_CHECK(ctx, d && wT && out && (dtype == 0 || dtype == 1) && B >= 0 && Hi >= 1 && Wi >= 1 &&
+                           (Co == 8 || Co == 16 || Co == 32) && Ci >= 1 && ldw >= 16 * Co && ldw % 8 == 0 && ldo >= Ci &&
                            aligned16(d) && aligned16(wT) && P < ((int64_t)1 << 31) &&
                            4 * Hi * Wi * Co < ((int64_t)1 << 31),
-                  "sfh_nn_conv_t_dinput: Co a multiple of 8, ldw >= 16 Co (a multiple of 8), ldo >= Ci, "
+                  "sfh_nn_conv_t_dinput: Co 8, 16 or 32, ldw >= 16 Co (a multiple of 8), ldo >= Ci, "
                   "16-byte aligned d and wT, B Hi Wi < 2^31");
     if (P == 0) return SFH_OK;
     const dim3 grid((unsigned)cdiv(P, 256), (unsigned)cdiv(Ci, 32));
-    if (dtype == 0)
-        hipLaunchKernelGGL(k_conv_dinput<uint16_t>, grid, dim3(kWG), 0, ctx->stream, d, (int)B, (int)Hi, (int)Wi,
-                           (int)Co, 1.0 / (double)Wi, 1.0 / (double)Hi, (const uint16_t*)wT, ldw, Ci, out, ldo);
-    else
-        hipLaunchKernelGGL(k_conv_dinput<float>, grid, dim3(kWG), 0, ctx->stream, d, (int)B, (int)Hi, (int)Wi,
-                           (int)Co, 1.0 / (double)Wi, 1.0 / (double)Hi, (const float*)wT, ldw, Ci, out, ldo);
+    const double iw = 1.0 / (double)Wi, ih = 1.0 / (double)Hi;
+#define SFH_DIN(TT, CC)                                                                                           \
+    hipLaunchKernelGGL((k_conv_dinput<TT, CC>), grid, dim3(kWG), 0, ctx->stream, d, (int)B, (int)Hi, (int)Wi, iw, ih, \
+                       (const TT*)wT, ldw, Ci, out, ldo)
+    if (dtype == 0) {
+        if (Co == 8) SFH_DIN(uint16_t, 8); else if (Co == 16) SFH_DIN(uint16_t, 16); else SFH_DIN(uint16_t, 32);
+    } else {
+        if (Co == 8) SFH_DIN(float, 8); else if (Co == 16) SFH_DIN(float, 16); else SFH_DIN(float, 32);
+    }
+#undef SFH_DIN
     return check_launch(ctx, "k_conv_dinput");
 }
 
