@@ -764,7 +764,8 @@ __global__ __launch_bounds__(kWG) void k_conv_fwd(const T* __restrict__ in, int 
 // each owns one 32 x 32 tile.  Per chunk of 64 pixels the activated inputs and
 // the four taps' gradient windows are staged in LDS channel-major ([channel]
 // [pixel], rows padded so a lane's 16-byte fragment reads are conflict-free),
-// rounded to T; then every wave runs four 32x32x16 MFMAs.  Operands are read
+// rounded to T; then every wave runs four 32x32x16 MFMAs while the next
+// chunk's loads are in flight in registers.  Operands are read
 // from the tensors directly: no transposed column matrix, no transposed input.
 constexpr int kDwC = 32, kDwChunk = 64;
 template <typename T>
@@ -776,8 +777,6 @@ __global__ __launch_bounds__(kWG) void k_conv_dweight(const float* __restrict__ 
     constexpr int LD = kDwChunk + PADL;
     __shared__ __align__(16) T sA[kDwC][LD];       // act(pre)[ci][pixel]
     __shared__ __align__(16) T sD[4][kDwC][LD];    // d window of tap (ky, kx)[co][pixel]
-    __shared__ int64_t simg[kDwChunk];             // per chunk pixel: offset of its image in d, or -1
-    __shared__ int sy2[kDwChunk], sx2[kDwChunk];
     const int ky = blockIdx.x;
     const int64_t P = (int64_t)B * Hi * Wi;
     const int64_t p0 = (int64_t)blockIdx.y * per;
@@ -785,44 +784,49 @@ __global__ __launch_bounds__(kWG) void k_conv_dweight(const float* __restrict__ 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 31, h = lane >> 5;
     const int H2 = 2 * Hi, W2 = 2 * Wi;
+    // staging role of this thread in every chunk: channel quad q of pixel
+    // slots pq and pq + 32, for the activations and the four taps' windows
+    const int q = threadIdx.x & 7, pq = threadIdx.x >> 3;
+    f4 va[2], vd[2][4];
+    auto fetch = [&](int64_t c0) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int64_t p = c0 + pq + 32 * s;
+            const bool ok = p < p1;
+            const int64_t pp = ok ? p : p0;
+            const int64_t rr = udiv31(pp, Wi, inv_w), b = udiv31(rr, Hi, inv_h);
+            const int x2 = 2 * (int)(pp - rr * Wi) - 1, y2 = 2 * (int)(rr - b * Hi) - 1;
+            const float* img = d + b * H2 * (int64_t)W2 * kDwC;
+            va[s] = ok ? act4(*reinterpret_cast<const f4*>(pre + pp * kDwC + 4 * q), act) : f4{0.f, 0.f, 0.f, 0.f};
+            const int Y = y2 + ky;
+            const bool yok = ok && Y >= 0 && Y < H2;
+#pragma unroll
+            for (int kx = 0; kx < 4; ++kx) {
+                const int X = x2 + kx;
+                vd[s][kx] = (yok && X >= 0 && X < W2)
+                                ? *reinterpret_cast<const f4*>(img + ((int64_t)Y * W2 + X) * kDwC + 4 * q)
+                                : f4{0.f, 0.f, 0.f, 0.f};
+            }
+        }
+    };
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+    if (p0 < p1) fetch(p0);
     for (int64_t c0 = p0; c0 < p1; c0 += kDwChunk) {
         __syncthreads();                           // the previous chunk's fragments are read
-        if (threadIdx.x < kDwChunk) {
-            const int64_t p = c0 + threadIdx.x;
-            if (p < p1) {
-                const int64_t rr = udiv31(p, Wi, inv_w), b = udiv31(rr, Hi, inv_h);
-                simg[threadIdx.x] = b * H2 * (int64_t)W2 * kDwC;
-                sx2[threadIdx.x] = 2 * (int)(p - rr * Wi) - 1;
-                sy2[threadIdx.x] = 2 * (int)(rr - b * Hi) - 1;
-            } else {
-                simg[threadIdx.x] = -1;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int pl = pq + 32 * s;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                sA[4 * q + j][pl] = from_f32<T>(va[s][j]);
+#pragma unroll
+                for (int kx = 0; kx < 4; ++kx) sD[kx][4 * q + j][pl] = from_f32<T>(vd[s][kx][j]);
             }
         }
         __syncthreads();
-        // activated inputs: 64 pixels x 8 channel quads
-        for (int i = threadIdx.x; i < kDwChunk * 8; i += kWG) {
-            const int pl = i >> 3, q = i & 7;
-            f4 v = {0.f, 0.f, 0.f, 0.f};
-            if (simg[pl] >= 0) v = act4(*reinterpret_cast<const f4*>(pre + (c0 + pl) * kDwC + 4 * q), act);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) sA[4 * q + j][pl] = from_f32<T>(v[j]);
-        }
-        // gradient windows of the four taps of this ky
-        for (int i = threadIdx.x; i < 4 * kDwChunk * 8; i += kWG) {
-            const int kx = i / (kDwChunk * 8), pl = (i >> 3) % kDwChunk, q = i & 7;
-            f4 v = {0.f, 0.f, 0.f, 0.f};
-            if (simg[pl] >= 0) {
-                const int Y = sy2[pl] + ky, X = sx2[pl] + kx;
-                if (Y >= 0 && Y < H2 && X >= 0 && X < W2)
-                    v = *reinterpret_cast<const f4*>(d + simg[pl] + ((int64_t)Y * W2 + X) * kDwC + 4 * q);
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) sD[kx][4 * q + j][pl] = from_f32<T>(v[j]);
-        }
-        __syncthreads();
+        if (c0 + kDwChunk < p1) fetch(c0 + kDwChunk);     // in flight during this chunk's products
 #pragma unroll
         for (int ks = 0; ks < kDwChunk / 16; ++ks) {
             Frag<T> fa, fb;
