@@ -349,32 +349,38 @@ __device__ inline f4 act4(f4 v, int act)
 }
 
 // out[b][Y][X] = bmap[Y][X] + sum over the (y, x, tap) that reach (Y, X) of
-// act(in[b][y][x][:]) . w[tap][:].  Workgroup = one input tile of one image:
-// the tile's activated pixels (+ a one-pixel halo, zeros outside the image)
-// are staged in LDS, their 16 tap values computed into LDS (thread = tap x
-// pixel slice, its tap's 32 weights in registers), then every output pixel of
-// the tile's 2x-scaled footprint adds its four.
+// act(in[b][y][x][:]) . w[tap][:].  A workgroup walks input tiles (one image's
+// kC1TY x kC1TX pixels) t = blockIdx.x, + gridDim.x, ...: the tile's pixels
+// (+ a one-pixel halo, zeros outside the image) are staged activated in LDS,
+// their 16 tap values computed into LDS (thread = tap x pixel slice, its tap's
+// 32 weights in registers), then every output pixel of the tile's 2x-scaled
+// footprint adds its four; the next tile's inputs are loaded into registers
+// while the current one is computed.
 __global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in, int B, int H, int W,
                                                    const float* __restrict__ w, const float* __restrict__ bmap,
-                                                   int act, float* __restrict__ out)
+                                                   int act, int64_t ntiles, float* __restrict__ out)
 {
     constexpr int NP = kC1HY * kC1HX, NQ = kC1Ci / 4;
+    constexpr int NPER = (NP * NQ + kWG - 1) / kWG;
     __shared__ f4 xs[NP][NQ];
     __shared__ float tv[NP][17];
     const int tilesx = (W + kC1TX - 1) / kC1TX, tilesy = (H + kC1TY - 1) / kC1TY;
-    int blk = blockIdx.x;
-    const int tx = blk % tilesx;
-    blk /= tilesx;
-    const int ty = blk % tilesy, b = blk / tilesy;
-    const int y0 = ty * kC1TY, x0 = tx * kC1TX;
-    for (int i = threadIdx.x; i < NP * NQ; i += kWG) {
-        const int p = i / NQ, q = i % NQ;
-        const int yy = y0 - 1 + p / kC1HX, xx = x0 - 1 + p % kC1HX;
-        f4 v = {0.f, 0.f, 0.f, 0.f};
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-            v = act4(reinterpret_cast<const f4*>(in + (((int64_t)b * H + yy) * W + xx) * kC1Ci)[q], act);
-        xs[p][q] = v;
-    }
+    const int H2 = 2 * H, W2 = 2 * W;
+    f4 raw[NPER];
+    auto fetch = [&](int64_t t) {
+        const int tx = (int)(t % tilesx), ty = (int)((t / tilesx) % tilesy);
+        const int64_t b = t / ((int64_t)tilesx * tilesy);
+        const int y0 = ty * kC1TY, x0 = tx * kC1TX;
+#pragma unroll
+        for (int j = 0; j < NPER; ++j) {
+            const int i = threadIdx.x + kWG * j;
+            const int p = i / NQ, q = i % NQ;
+            const int yy = y0 - 1 + p / kC1HX, xx = x0 - 1 + p % kC1HX;
+            raw[j] = (i < NP * NQ && yy >= 0 && yy < H && xx >= 0 && xx < W)
+                         ? reinterpret_cast<const f4*>(in + ((b * H + yy) * (int64_t)W + xx) * kC1Ci)[q]
+                         : f4{0.f, 0.f, 0.f, 0.f};
+        }
+    };
     const int tap = threadIdx.x & 15, slice = threadIdx.x >> 4;
     float wr[kC1Ci];
 #pragma unroll
@@ -385,38 +391,50 @@ __global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in,
         wr[c + 2] = v[2];
         wr[c + 3] = v[3];
     }
-    __syncthreads();
-    for (int p = slice; p < NP; p += kWG / 16) {
-        float s = 0.f;
+    if ((int64_t)blockIdx.x < ntiles) fetch(blockIdx.x);
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int tx = (int)(t % tilesx), ty = (int)((t / tilesx) % tilesy);
+        const int64_t b = t / ((int64_t)tilesx * tilesy);
+        const int y0 = ty * kC1TY, x0 = tx * kC1TX;
+        __syncthreads();                           // the previous tile's LDS reads are done
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const f4 v = xs[p][q];
-            s = fmaf(v[0], wr[4 * q], s);
-            s = fmaf(v[1], wr[4 * q + 1], s);
-            s = fmaf(v[2], wr[4 * q + 2], s);
-            s = fmaf(v[3], wr[4 * q + 3], s);
+        for (int j = 0; j < NPER; ++j) {
+            const int i = threadIdx.x + kWG * j;
+            if (i < NP * NQ) xs[i / NQ][i % NQ] = act4(raw[j], act);   // act(0) = 0 keeps the halo zero
         }
-        tv[p][tap] = s;
-    }
-    __syncthreads();
-    const int H2 = 2 * H, W2 = 2 * W;
-    for (int o = threadIdx.x; o < 4 * kC1TY * kC1TX; o += kWG) {
-        const int Y = 2 * y0 + o / (2 * kC1TX), X = 2 * x0 + o % (2 * kC1TX);
-        if (Y >= H2 || X >= W2) continue;
-        float s = bmap ? bmap[Y * W2 + X] : 0.f;
-        const int ky0 = (Y + 1) & 1, kx0 = (X + 1) & 1;
+        __syncthreads();
+        if (t + gridDim.x < ntiles) fetch(t + gridDim.x);
+        for (int p = slice; p < NP; p += kWG / 16) {
+            float s = 0.f;
 #pragma unroll
-        for (int dy = 0; dy < 2; ++dy) {
-            const int ky = ky0 + 2 * dy, y = (Y + 1 - ky) / 2;
-            if (y < 0 || y >= H) continue;
-#pragma unroll
-            for (int dx = 0; dx < 2; ++dx) {
-                const int kx = kx0 + 2 * dx, x = (X + 1 - kx) / 2;
-                if (x < 0 || x >= W) continue;
-                s += tv[(y - y0 + 1) * kC1HX + (x - x0 + 1)][ky * 4 + kx];
+            for (int q = 0; q < NQ; ++q) {
+                const f4 v = xs[p][q];
+                s = fmaf(v[0], wr[4 * q], s);
+                s = fmaf(v[1], wr[4 * q + 1], s);
+                s = fmaf(v[2], wr[4 * q + 2], s);
+                s = fmaf(v[3], wr[4 * q + 3], s);
             }
+            tv[p][tap] = s;
         }
-        out[((int64_t)b * H2 + Y) * W2 + X] = s;
+        __syncthreads();
+        for (int o = threadIdx.x; o < 4 * kC1TY * kC1TX; o += kWG) {
+            const int Y = 2 * y0 + o / (2 * kC1TX), X = 2 * x0 + o % (2 * kC1TX);
+            if (Y >= H2 || X >= W2) continue;
+            float s = bmap ? bmap[Y * W2 + X] : 0.f;
+            const int ky0 = (Y + 1) & 1, kx0 = (X + 1) & 1;
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy) {
+                const int ky = ky0 + 2 * dy, y = (Y + 1 - ky) / 2;
+                if (y < 0 || y >= H) continue;
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int kx = kx0 + 2 * dx, x = (X + 1 - kx) / 2;
+                    if (x < 0 || x >= W) continue;
+                    s += tv[(y - y0 + 1) * kC1HX + (x - x0 + 1)][ky * 4 + kx];
+                }
+            }
+            out[(b * H2 + Y) * (int64_t)W2 + X] = s;
+        }
     }
 }
 
@@ -1490,9 +1508,13 @@ int sfh_nn_conv1_fwd(sfh_ctx* ctx, const float* in, int64_t B, int64_t H, int64_
     const int64_t P = B * H * W;
     if (P == 0) return SFH_OK;
     if (conv1_direct(B, H, W, Ci, in, w)) {
-        const int64_t nblk = B * cdiv(H, kC1TY) * cdiv(W, kC1TX);
-        hipLaunchKernelGGL(k_conv1_fwd, dim3((unsigned)nblk), dim3(kWG), 0, ctx->stream, in, (int)B, (int)H, (int)W,
-                           w, bmap, act, out);
+        // two workgroups per CU (their LDS), each walking tiles
+        const int64_t ntiles = B * cdiv(H, kC1TY) * cdiv(W, kC1TX);
+        int cus = 0;
+        SFH_HIP_TRY(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        const int64_t grid = std::min<int64_t>(ntiles, 2 * (int64_t)std::max(cus, 1));
+        hipLaunchKernelGGL(k_conv1_fwd, dim3((unsigned)grid), dim3(kWG), 0, ctx->stream, in, (int)B, (int)H, (int)W,
+                           w, bmap, act, ntiles, out);
         return check_launch(ctx, "k_conv1_fwd");
     }
     if (int rc = conv1_activated(ctx, in, P, Ci, act, &in)) return rc;

@@ -144,12 +144,14 @@ def test_colsum(sfh, gpu_ctx, rows, cols):
 
 
 @pytest.mark.parametrize("act", [0, 1])
-@pytest.mark.parametrize("B,H,W,Ci", [(2, 13, 37, 32), (1, 8, 32, 32), (3, 5, 7, 32), (2, 6, 9, 16)])
+@pytest.mark.parametrize("B,H,W,Ci", [(2, 13, 37, 32), (1, 8, 32, 32), (3, 5, 7, 32), (2, 6, 9, 16),
+                                      (16, 104, 152, 32)])
 def test_output_layer_against_oracle(sfh, gpu_ctx, B, H, W, Ci, act):
     """The single-channel output layer (32 input channels: direct pixel-grid
     kernels; other widths: the matrix-product form), on act(in) with act the
     identity or SiLU, against the FP64 oracle's transposed convolution and its
-    gradients, on grids that leave partial input tiles."""
+    gradients, on grids that leave partial input tiles and, at 16 x 104 x 152,
+    more tiles than workgroups (each workgroup walks several)."""
     import torch
     rng = np.random.default_rng(B * 1000 + H * 10 + W)
     a = rng.normal(size=(B, H, W, Ci)).astype(np.float32)
