@@ -526,21 +526,13 @@ __global__ __launch_bounds__(kC1DwWG) void k_conv1_dw(const float* __restrict__ 
     float acc[16][4];
 #pragma unroll
     for (int t = 0; t < 16; ++t) acc[t][0] = acc[t][1] = acc[t][2] = acc[t][3] = 0.f;
-    // pixel coordinates of p, advanced by the step instead of divided each time
+    // pixel coordinates of the pixel being loaded, advanced by the step
+    // instead of divided each time; each pixel's input quad and gradient
+    // window are loaded one step ahead of the products that use them
     constexpr int STEP = kC1DwWG / 8;
     int64_t rr = udiv31(p0 + pl, W, inv_w), b = udiv31(rr, H, inv_h);
     int x = (int)(p0 + pl - rr * W), y = (int)(rr - b * H);
-    for (int64_t p = p0 + pl; p < p1; p += STEP) {
-        const f4 a = act4(*reinterpret_cast<const f4*>(in + p * kC1Ci + 4 * q), act);
-        float d[16];
-        conv1_window(dout, b, y, x, 2 * H, 2 * W, d);
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            acc[t][0] = fmaf(d[t], a[0], acc[t][0]);
-            acc[t][1] = fmaf(d[t], a[1], acc[t][1]);
-            acc[t][2] = fmaf(d[t], a[2], acc[t][2]);
-            acc[t][3] = fmaf(d[t], a[3], acc[t][3]);
-        }
+    auto advance = [&]() {
         x += STEP;
         while (x >= W) {
             x -= W;
@@ -549,6 +541,33 @@ __global__ __launch_bounds__(kC1DwWG) void k_conv1_dw(const float* __restrict__ 
                 ++b;
             }
         }
+    };
+    f4 a = {0.f, 0.f, 0.f, 0.f};
+    float d[16];
+    if (p0 + pl < p1) {
+        a = *reinterpret_cast<const f4*>(in + (p0 + pl) * kC1Ci + 4 * q);
+        conv1_window(dout, b, y, x, 2 * H, 2 * W, d);
+    }
+    for (int64_t p = p0 + pl; p < p1; p += STEP) {
+        f4 an = {0.f, 0.f, 0.f, 0.f};
+        float dn[16];
+        const bool more = p + STEP < p1;
+        if (more) {
+            advance();
+            an = *reinterpret_cast<const f4*>(in + (p + STEP) * kC1Ci + 4 * q);
+            conv1_window(dout, b, y, x, 2 * H, 2 * W, dn);
+        }
+        const f4 av = act4(a, act);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            acc[t][0] = fmaf(d[t], av[0], acc[t][0]);
+            acc[t][1] = fmaf(d[t], av[1], acc[t][1]);
+            acc[t][2] = fmaf(d[t], av[2], acc[t][2]);
+            acc[t][3] = fmaf(d[t], av[3], acc[t][3]);
+        }
+        a = an;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) d[t] = more ? dn[t] : 0.f;
     }
     // sum the 8 pixel lanes of this wave (lane bits 3..5)
 #pragma unroll
