@@ -310,14 +310,14 @@ def surrogate_bench(sfh, torch, ctx, O, args, ncpu):
 
 def binaries_bench(sfh, ctx, args):
     """Unresolved-binary templates (RandomBinaryPairs(0.3)) of the 700
-    config-1 isochrones, downsampled 4x as partial_cmd_smooth's binary branch
+    2000-point deposition-benchmark isochrones (200 x 300 grid), downsampled 4x as partial_cmd_smooth's binary branch
     does: all pair histograms in one launch and all templates in one
     deposition (binary_hess_batch), against binary_hess one template at a
     time on a sample."""
     from sfh_amd import synth
-    edges = synth.cfg1_edges()
+    edges = synth.edges_200x300()
     imf = synth.Kroupa2001(0.08, 120.0)
-    pts = [synth.template_points(s, 2000, edges, 120.0)[0] for s in synth.cfg1_specs()]
+    pts = [synth.template_points(s, 2000, edges, 120.0)[0] for s in synth.specs_70x10()]
     isos = [(np.asarray(sp.new_mini)[::4], [np.asarray(m)[::4] for m in sp.new_iso_mags]) for sp in pts]
     kw = dict(normalize_value=1.0, mean_mass=imf.mean(), ctx=ctx)
     model = sfh.RandomBinaryPairs(0.3)
@@ -348,10 +348,10 @@ def binaries_bench(sfh, ctx, args):
 
 
 def mock_bench(sfh, ctx, args):
-    """The mock-observation path: a population of one config-1 isochrone
+    """The mock-observation path: a population of one 2000-point isochrone
     (logAge 9, [M/H] -1, 2000 points) with a Kroupa IMF and random binary
     pairs (fraction 0.3) sampled to a birth mass on the GPU, observed with
-    the config-1 error and completeness model, and binned on the 200 x 300
+    the deposition benchmark's error and completeness model, and binned on the 200 x 300
     grid; kernel times from HIP events.  The NumPy restatement of the same
     sampling rule times the host on a bounded sample (one thread)."""
     import importlib
@@ -359,7 +359,7 @@ def mock_bench(sfh, ctx, args):
     m, (B, V) = synth.isochrone(9.0, -1.0, 2000, 7, mmax_imf=120.0)
     g = np.stack([B, V], axis=1)
     imf = mock.Kroupa2001(0.08, 120.0)
-    edges = synth.cfg1_edges()
+    edges = synth.edges_200x300()
     limit = args.mock_mass
     kw = dict(dist_mod=24.0, binary_model=sfh.RandomBinaryPairs(0.3), catalog=True, ctx=ctx)
     sfh.generate_stars_mass(m, g, ["B", "V"], 1e4, imf, seed=1, **kw).close()          # warm-up
@@ -450,7 +450,7 @@ def _fill_bench(S, torch, ctx, O, args, ncpu, net, la, mh, nstars, exact_rate):
     if not args.no_cpu and nstars == args.surrogate_stars:
         # partial_cmd_smooth on the host: the same host preparation, then the C
         # FP64 restatement of bin_cmd_smooth on all host cores; bounded sample
-        edges = synth.cfg1_edges()
+        edges = synth.edges_200x300()
         ex_, ey_ = (O.Range(e.first, e.step, e.length) for e in edges)
         rng = np.random.default_rng(4)
         cpu_s, k = 0.0, 0
@@ -500,9 +500,9 @@ def main():
     res = {}
 
     # ---------------------------------------------------- templates (deposition)
-    edges = synth.cfg1_edges()
+    edges = synth.edges_200x300()
     nbins = (edges[0].length - 1) * (edges[1].length - 1)
-    specs = synth.cfg1_specs()
+    specs = synth.specs_70x10()
     t0 = time.time()
     pc = [synth.template_points(s, 2000, edges, 120.0, correlation=0.6) for s in specs]
     prep_s = time.time() - t0

@@ -423,6 +423,16 @@ int sfh_nn_im2col(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t
  * rows of ldo >= B H W, dtype, zero past B H W) with Co in {4, 8, 16, 32}. */
 int sfh_nn_im2col_t(sfh_ctx* ctx, int dtype, const float* dout, int64_t B, int64_t H, int64_t W, int64_t Co,
                     void* dcolsT, int64_t ldo);
+/* Low-precision copies of weight matrices, one launch for up to 8: for each,
+ * dst (row-major, rows x cols) = src (FP32, row-major) and, when dstT is not
+ * NULL, dstT (cols x rows) = its transpose, both rounded to dtype. */
+typedef struct sfh_nn_wcopy {
+    const float* src;
+    int64_t rows, cols;
+    void* dst;
+    void* dstT;
+} sfh_nn_wcopy;
+int sfh_nn_weight_copies(sfh_ctx* ctx, int dtype, int64_t n, const sfh_nn_wcopy* mats);
 /* Gradient of that transposed convolution's input: out (B Hi Wi x Ci, FP32,
  * row stride ldo) = im2col(d) wT^T, d the FP32 output gradient (B x 2Hi x 2Wi x
  * Co), wT (dtype, Ci rows of ldw >= 16 Co, column tap Co + co) -- the product of

@@ -61,6 +61,11 @@ class sfh_fit_results(C.Structure):
 
 SFH_FIT_LOG = 0
 SFH_FIT_SQRT = 1
+class sfh_nn_wcopy(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("rows", C.c_int64), ("cols", C.c_int64), ("dst", C.c_void_p),
+                ("dstT", C.c_void_p)]
+
+
 class sfh_imf(C.Structure):
     _fields_ = [("nseg", C.c_int32), ("breaks", C.c_void_p), ("slopes", C.c_void_p), ("coefs", C.c_void_p)]
 
@@ -142,6 +147,7 @@ SIGNATURES = [
     ("sfh_nn_bias_act", _int, [_vp, _int, _vp, _i64, _vp, _i64, _i64, _int, _vp, _vp, _i64]),
     ("sfh_nn_act_bwd", _int, [_vp, _vp, _vp, _i64, _int, _vp]),
     ("sfh_nn_transpose", _int, [_vp, _int, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _int]),
+    ("sfh_nn_weight_copies", _int, [_vp, _int, _i64, _vp]),
     ("sfh_nn_colsum", _int, [_vp, _vp, _i64, _i64, _i64, C.c_float, _vp]),
     ("sfh_nn_col2im", _int, [_vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp]),
     ("sfh_nn_im2col", _int, [_vp, _int, _vp, _i64, _i64, _i64, _i64, _vp, _i64]),

@@ -234,6 +234,50 @@ __global__ __launch_bounds__(kWG) void k_transpose(const float* __restrict__ in,
     }
 }
 
+// Low-precision copies of up to kMaxWCopies weight matrices in one launch:
+// per 32 x 32 tile of src (rows x cols, row-major FP32), dst gets the tile in
+// place (row stride cols) and dstT (may be null) its transpose (row stride
+// rows), both rounded to T.  Workgroup w takes tile w - first[m] of the
+// matrix m whose tile range holds it.
+constexpr int kMaxWCopies = 8;
+struct WCopies {
+    int n;
+    int64_t first[kMaxWCopies + 1];       // tile prefix sums
+    const float* src[kMaxWCopies];
+    int64_t rows[kMaxWCopies], cols[kMaxWCopies];
+    void* dst[kMaxWCopies];
+    void* dstT[kMaxWCopies];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_weight_copies(WCopies wc)
+{
+    __shared__ float tile[32][33];
+    const int64_t g = blockIdx.x;
+    int m = 0;
+    while (m + 1 < wc.n && g >= wc.first[m + 1]) ++m;
+    const int64_t rows = wc.rows[m], cols = wc.cols[m];
+    const int64_t tcols = (cols + 31) / 32, t = g - wc.first[m];
+    const int64_t r0 = (t / tcols) * 32, c0 = (t % tcols) * 32;
+    const float* src = wc.src[m];
+    T* dst = (T*)wc.dst[m];
+    T* dstT = (T*)wc.dstT[m];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t r = r0 + j, c = c0 + tx;
+        const bool ok = r < rows && c < cols;
+        const float v = ok ? src[r * cols + c] : 0.f;
+        if (ok) dst[r * cols + c] = from_f32<T>(v);
+        tile[j][tx] = v;
+    }
+    if (!dstT) return;
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t c = c0 + j, r = r0 + tx;
+        if (c < cols && r < rows) dstT[c * rows + r] = from_f32<T>(tile[tx][j]);
+    }
+}
+
 // column sums of a rows x cols matrix: per-chunk partials then a fixed-order sum
 // Column sums in two deterministic passes.  Workgroup (x, y): columns
 // 64 x .. 64 x + 63, rows of chunk y; thread (g, c) = (t / 64, t % 64) adds
@@ -1432,6 +1476,35 @@ int sfh_nn_transpose(sfh_ctx* ctx, int dtype, const float* in, int64_t ldi, int6
         hipLaunchKernelGGL(k_transpose<float>, grid, dim3(kWG), 0, ctx->stream, in, ldi, rows, cols, (float*)out, ldo,
                            ocols, act);
     return check_launch(ctx, "k_transpose");
+}
+
+int sfh_nn_weight_copies(sfh_ctx* ctx, int dtype, int64_t n, const sfh_nn_wcopy* mats)
+{
+    SFH_NN_BEGIN(ctx);
+    SFH_ARG_CHECK(ctx, (dtype == 0 || dtype == 1) && n >= 0 && n <= kMaxWCopies && (n == 0 || mats),
+                  "sfh_nn_weight_copies: dtype 0 or 1, at most %d matrices", kMaxWCopies);
+    WCopies wc{};
+    wc.n = (int)n;
+    int64_t tiles = 0;
+    for (int64_t m = 0; m < n; ++m) {
+        SFH_ARG_CHECK(ctx, mats[m].src && mats[m].dst && mats[m].rows >= 1 && mats[m].cols >= 1,
+                      "sfh_nn_weight_copies: matrix %lld needs src, dst and a positive shape", (long long)m);
+        wc.first[m] = tiles;
+        wc.src[m] = mats[m].src;
+        wc.rows[m] = mats[m].rows;
+        wc.cols[m] = mats[m].cols;
+        wc.dst[m] = mats[m].dst;
+        wc.dstT[m] = mats[m].dstT;
+        tiles += cdiv(mats[m].rows, 32) * cdiv(mats[m].cols, 32);
+    }
+    wc.first[n] = tiles;
+    if (tiles == 0) return SFH_OK;
+    SFH_ARG_CHECK(ctx, tiles < ((int64_t)1 << 31), "sfh_nn_weight_copies: too many tiles");
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_weight_copies<uint16_t>, dim3((unsigned)tiles), dim3(kWG), 0, ctx->stream, wc);
+    else
+        hipLaunchKernelGGL(k_weight_copies<float>, dim3((unsigned)tiles), dim3(kWG), 0, ctx->stream, wc);
+    return check_launch(ctx, "k_weight_copies");
 }
 
 int sfh_nn_colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols, float beta, float* out)

@@ -33,9 +33,10 @@ from __future__ import annotations
 import ctypes as C
 import numpy as np
 
+from . import _lib
 from .core import Context, default_context
 
-# parameter ranges of config 4 (scaled to [-1, 1] on input)
+# training box of the surrogate: logAge, [M/H], dmod, error scale (scaled to [-1, 1] on input)
 PARAM_LO = np.array([6.6, -2.5, 24.0, 0.5])
 PARAM_HI = np.array([10.1, 0.3, 30.0, 2.0])
 
@@ -181,14 +182,15 @@ class TemplateSurrogate:
         self._c(self.lib.sfh_nn_colsum(self.ctx.handle, X.data_ptr(), ld, rows, cols, 0.0, out.data_ptr()))
 
     def _refresh_weights(self):
-        """Low-precision copies (and transposes) of the master weights."""
+        """Low-precision copies (and transposes) of the master weights, one launch."""
         v = self.views
-        for k in ("W0", "W1", "W2", "K1", "K2", "K3"):
+        names = ("W0", "W1", "W2", "K1", "K2", "K3")
+        mats = (_lib.sfh_nn_wcopy * len(names))()
+        for i, k in enumerate(names):
             r, c = v[k].shape
-            self._bias_act(v[k], c, None, r, c, IDENT, None, self.w[k], c)
-        for k in ("W1", "W2", "K1", "K2", "K3"):
-            r, c = v[k].shape
-            self._transpose(v[k], c, r, c, self.wT[k], r)
+            mats[i] = _lib.sfh_nn_wcopy(v[k].data_ptr(), r, c, self.w[k].data_ptr(),
+                                        self.wT[k].data_ptr() if k in self.wT else None)
+        self._c(self.lib.sfh_nn_weight_copies(self.ctx.handle, self.dt, len(names), mats))
 
     # ---------------------------------------------------------- forward
     def forward(self, params, raw=False):
@@ -483,7 +485,7 @@ def exact_templates(params, edges=None, npoints=2000, correlation=0.6, mmax_imf=
     from . import synth
     from .templates import PointBatch, deposit_batch
     ctx = ctx or default_context()
-    edges = edges or synth.cfg1_edges()
+    edges = edges or synth.edges_200x300()
     p = np.atleast_2d(np.asarray(params, dtype=np.float64))
     n = p.shape[0]
     nbins = (edges[0].length - 1) * (edges[1].length - 1)

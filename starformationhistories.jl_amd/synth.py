@@ -116,21 +116,21 @@ def template_grid(log_ages, mhs, base_seed):
     return specs
 
 
-def cfg0_edges():
+def edges_75x100():
     """75 x 100 bins (76 / 101 edges) over colour -0.5..2.5, mag 18..30."""
     return (edges_range(-0.5, 2.5, 76), edges_range(18.0, 30.0, 101))
 
 
-def cfg1_edges():
+def edges_200x300():
     """200 x 300 bins (201 / 301 edges), 60,000 bins."""
     return (edges_range(-0.5, 2.5, 201), edges_range(18.0, 30.0, 301))
 
 
-def cfg0_specs(seed=20261202):
+def specs_10x5(seed=20261202):
     return template_grid(np.linspace(8.0, 10.1, 10), np.linspace(-2.0, 0.0, 5), seed)
 
 
-def cfg1_specs(seed=20261202, n_age=70, n_mh=10):
+def specs_70x10(seed=20261202, n_age=70, n_mh=10):
     return template_grid(6.6 + 0.05 * np.arange(n_age), np.linspace(-2.5, 0.0, n_mh), seed + 1)
 
 

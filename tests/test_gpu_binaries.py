@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 def _prep(sfh, y_index=1, cidx=(0, 1)):
     from sfh_amd import synth
-    edges = synth.cfg0_edges()
+    edges = synth.edges_75x100()
     imf = synth.Kroupa2001(0.08, 2.0)
     m, mags = synth.isochrone(9.2, -0.8, 500, 21, mmax_imf=2.0)
     sp = sfh.partial_cmd_smooth_points(m, mags, [synth.photerr] * 2, y_index, cidx, imf,
@@ -96,7 +96,7 @@ def test_binary_hess_batch_matches_single_templates(sfh, gpu_ctx):
     isochrones' magnitudes; FP32 templates), including an isochrone entirely
     off the grid (an empty template)."""
     from sfh_amd import synth
-    edges = synth.cfg1_edges()
+    edges = synth.edges_200x300()
     imf = synth.Kroupa2001(0.08, 120.0)
     isos = []
     for la, mh, dm in [(8.0, -1.0, 24.0), (9.0, -0.5, 25.0), (10.0, -2.0, 26.0), (9.5, -1.5, 45.0), (7.2, 0.0, 27.0)]:

@@ -178,7 +178,7 @@ def test_bin_cmd_matches_oracle(sfh, gpu_ctx):
     from sfh_amd import mock, synth
     rng = np.random.default_rng(4)
     x, y, w = rng.uniform(-1, 3, 400_000), rng.uniform(17, 31, 400_000), rng.uniform(0, 3, 400_000)
-    ex, ey = synth.cfg1_edges()
+    ex, ey = synth.edges_200x300()
     h = sfh.bin_cmd(x, y, edges=(ex, ey))
     ref = M.bin_cmd(x, y, ex.first, ex.step, ex.length - 1, ey.first, ey.step, ey.length - 1)
     assert np.array_equal(h.weights, ref) and h.closed == "left"
@@ -203,7 +203,7 @@ def test_monte_carlo_population_matches_template(sfh, gpu_ctx):
     from sfh_amd import mock, synth
     m, g = _synth_iso(9.0, -1.0)
     dmod, limit = 24.0, 3e5
-    edges = synth.cfg1_edges()
+    edges = synth.edges_200x300()
     imf = mock.Kroupa2001(0.08, 120.0)
     cat = sfh.generate_stars_mass(m, g, ["B", "V"], limit, imf, dist_mod=dmod, seed=31,
                                   binary_model=sfh.NoBinaries(), catalog=True)

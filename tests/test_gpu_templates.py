@@ -22,9 +22,9 @@ def test_golden_template(sfh, gpu_ctx):
 
 
 @pytest.mark.parametrize("y_index,cidx", [(1, (0, 1)), (0, (0, 1))])
-def test_partial_cmd_smooth_cfg0(sfh, O, gpu_ctx, y_index, cidx):
+def test_partial_cmd_smooth_75x100(sfh, O, gpu_ctx, y_index, cidx):
     from sfh_amd import synth
-    edges = synth.cfg0_edges()
+    edges = synth.edges_75x100()
     imf = synth.Kroupa2001(0.08, 2.0)
     m, mags = synth.isochrone(9.2, -0.8, 500, 21, mmax_imf=2.0)
     h = sfh.partial_cmd_smooth(m, mags, [synth.photerr] * 2, y_index, cidx, imf, [synth.completeness] * 2,
@@ -38,7 +38,7 @@ def test_partial_cmd_smooth_cfg0(sfh, O, gpu_ctx, y_index, cidx):
 
 def test_asymmetric_kernel(sfh, O, gpu_ctx):
     from sfh_amd import synth
-    edges = synth.cfg0_edges()
+    edges = synth.edges_75x100()
     sp, _ = synth.template_points(synth.TemplateSpec(9.8, -1.5, 4), 500, edges, 2.0)
     ex, ey = oracle_range(O, edges[0]), oracle_range(O, edges[1])
     ref, _ = O.bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, 0, sp.weights, ex, ey, nthreads=8)
@@ -46,11 +46,11 @@ def test_asymmetric_kernel(sfh, O, gpu_ctx):
     assert max_norm_err(h.weights, ref) < TOL
 
 
-def test_batch_cfg1_shape(sfh, O, gpu_ctx):
+def test_batch_200x300_shape(sfh, O, gpu_ctx):
     """A few templates at the 200 x 300 / 2000-point config, one launch."""
     from sfh_amd import synth
-    edges = synth.cfg1_edges()
-    specs = synth.cfg1_specs()[::97][:6]
+    edges = synth.edges_200x300()
+    specs = synth.specs_70x10()[::97][:6]
     pts = [synth.template_points(s, 2000, edges, 120.0)[0] for s in specs]
     batch = sfh.PointBatch(pts, edges)
     out, pairs = sfh.deposit_batch(batch, return_pairs=True)
@@ -67,7 +67,7 @@ def test_batch_cfg1_shape(sfh, O, gpu_ctx):
 
 def test_batch_real_valued_covariance_and_empty(sfh, O, gpu_ctx):
     from sfh_amd import synth
-    edges = synth.cfg0_edges()
+    edges = synth.edges_75x100()
     specs = [synth.TemplateSpec(8.5, -1.0, 1), synth.TemplateSpec(9.9, -0.2, 2)]
     pc = [synth.template_points(s, 500, edges, 2.0, correlation=0.6) for s in specs]
     pts = [p for p, _ in pc]
@@ -86,8 +86,8 @@ def test_batch_real_valued_covariance_and_empty(sfh, O, gpu_ctx):
 
 def test_deposited_templates_feed_likelihood(sfh, O, gpu_ctx):
     from sfh_amd import synth
-    edges = synth.cfg0_edges()
-    pts = [synth.template_points(s, 500, edges, 2.0, normalize_value=1e3)[0] for s in synth.cfg0_specs()[:8]]
+    edges = synth.edges_75x100()
+    pts = [synth.template_points(s, 500, edges, 2.0, normalize_value=1e3)[0] for s in synth.specs_10x5()[:8]]
     out = sfh.deposit_batch(sfh.PointBatch(pts, edges))
     nbins = out.shape[1]
     tpl = sfh.templates_from_device(out, nbins)

@@ -39,7 +39,7 @@ def test_emulation_matches_golden(emu, O):
 @pytest.mark.parametrize("cov_mult", [1, -1, 0])
 def test_emulation_synthetic_template(sfh, emu, O, cov_mult):
     from sfh_amd import synth
-    edges = synth.cfg0_edges()
+    edges = synth.edges_75x100()
     sp, _ = synth.template_points(synth.TemplateSpec(9.3, -1.2, 42), 500, edges, 2.0)
     ex, ey = oracle_range(O, edges[0]), oracle_range(O, edges[1])
     ref, pr = O.bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, cov_mult, sp.weights, ex, ey, nthreads=8)
@@ -50,7 +50,7 @@ def test_emulation_synthetic_template(sfh, emu, O, cov_mult):
 
 def test_emulation_real_valued_covariance(sfh, emu, O):
     from sfh_amd import synth
-    edges = synth.cfg0_edges()
+    edges = synth.edges_75x100()
     sp, cm = synth.template_points(synth.TemplateSpec(8.4, -0.3, 5), 500, edges, 2.0, correlation=0.6)
     ex, ey = oracle_range(O, edges[0]), oracle_range(O, edges[1])
     ref, _ = O.bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, 1, sp.weights, ex, ey, cmult=cm, nthreads=8)
@@ -62,7 +62,7 @@ def test_host_prep_matches_oracle_prep(sfh, O):
     """The product's vectorised partial_cmd_smooth preparation equals the
     oracle's loop restatement."""
     from sfh_amd import synth
-    edges = synth.cfg0_edges()
+    edges = synth.edges_75x100()
     imf = synth.Kroupa2001(0.08, 2.0)
     m, mags = synth.isochrone(9.0, -1.0, 500, 17, mmax_imf=2.0)
     for y_index, cidx in ((1, (0, 1)), (0, (0, 1))):
