@@ -411,6 +411,14 @@ int sfh_nn_transpose(sfh_ctx* ctx, int dtype, const float* in, int64_t ldi, int6
                      int64_t ldo, int64_t ocols, int act);
 /* out[c] = sum_r X[r, c] (+ beta * out[c]), fixed summation order. */
 int sfh_nn_colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols, float beta, float* out);
+/* A dense layer's output gradient d (rows x cols FP32, row stride cols)
+ * prepared for its two products in one pass: d' = d * act'(pre), gb[c] =
+ * sum_r d'[r, c] (the summation order of sfh_nn_colsum), dl (rows x cols,
+ * dtype) = d' and dT (cols x ldt, dtype, 16-byte aligned) = d'^T with zeros
+ * for rows <= r < ldt (ldt a multiple of 8): sfh_nn_act_bwd, sfh_nn_colsum,
+ * sfh_nn_bias_act and sfh_nn_transpose of the same d in one launch. */
+int sfh_nn_dense_bwd_prep(sfh_ctx* ctx, int dtype, const float* d, const float* pre, int64_t rows, int64_t cols,
+                          int act, float* gb, void* dl, void* dT, int64_t ldt);
 /* Stride-2 4x4 transposed convolution (padding 1), the two gathers around
  * its matrix products: col2im sums the 4 taps landing on each of the
  * B x 2H x 2W x Co outputs from cols (B*H*W rows, column (ky*4+kx)*Co+co) and

@@ -147,6 +147,7 @@ SIGNATURES = [
     ("sfh_nn_bias_act", _int, [_vp, _int, _vp, _i64, _vp, _i64, _i64, _int, _vp, _vp, _i64]),
     ("sfh_nn_act_bwd", _int, [_vp, _vp, _vp, _i64, _int, _vp]),
     ("sfh_nn_transpose", _int, [_vp, _int, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _int]),
+    ("sfh_nn_dense_bwd_prep", _int, [_vp, _int, _vp, _vp, _i64, _i64, _int, _vp, _vp, _vp, _i64]),
     ("sfh_nn_weight_copies", _int, [_vp, _int, _i64, _vp]),
     ("sfh_nn_colsum", _int, [_vp, _vp, _i64, _i64, _i64, C.c_float, _vp]),
     ("sfh_nn_col2im", _int, [_vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp]),

@@ -75,6 +75,18 @@ __device__ inline void load_frag<float>(const float* __restrict__ p, bool ok, Fr
     f.v[1] = ok ? *reinterpret_cast<const f4*>(p + 4) : f4{0.f, 0.f, 0.f, 0.f};
 }
 
+// Unmasked loads for operands whose out-of-range rows are clamped to a valid
+// row: their products land only in accumulator entries that are never stored.
+__device__ inline void load_frag_raw(const uint16_t* __restrict__ p, Frag<uint16_t>& f)
+{
+    f.v = *reinterpret_cast<const u32x4*>(p);
+}
+__device__ inline void load_frag_raw(const float* __restrict__ p, Frag<float>& f)
+{
+    f.v[0] = *reinterpret_cast<const f4*>(p);
+    f.v[1] = *reinterpret_cast<const f4*>(p + 4);
+}
+
 __device__ inline void load_frag_lds(const uint16_t* p, Frag<uint16_t>& f) { f.v = *reinterpret_cast<const u32x4*>(p); }
 __device__ inline void load_frag_lds(const float* p, Frag<float>& f)
 {
@@ -140,26 +152,49 @@ __global__ __launch_bounds__(kWG) void k_gemm_nt(const T* __restrict__ A, int64_
         okb[b] = col < N;
         pb[b] = B + (okb[b] ? col : 0) * ldb + 8 * h;
     }
-    Frag<T> fa[TM], fb[TN], na[TM], nb[TN];
+    // ring of D K-steps in flight: slot s holds step k0 + 16 s and is refilled
+    // with step k0 + 16 (s + D) once its products are issued.  The loads are
+    // unmasked (no branch around them, so they stay in flight across
+    // iterations): rows and columns past M and N read the clamped row 0 into
+    // accumulator entries that are never stored, and a step past K (reading
+    // step 0 again) is never multiplied.  The main loop runs whole groups of D
+    // steps without branches; the last K / 16 mod D steps follow it.
+    constexpr int D = sizeof(T) == 2 ? 4 : 2;
+    Frag<T> fa[D][TM], fb[D][TN];
 #pragma unroll
-    for (int a = 0; a < TM; ++a) load_frag(pa[a], oka[a] && K > 0, fa[a]);
+    for (int s = 0; s < D; ++s) {
+        const int64_t ks = 16 * s < K ? 16 * s : 0;
 #pragma unroll
-    for (int b = 0; b < TN; ++b) load_frag(pb[b], okb[b] && K > 0, fb[b]);
-    for (int64_t k = 0; k < K; k += 16) {
-        const bool more = k + 16 < K;
+        for (int a = 0; a < TM; ++a) load_frag_raw(pa[a] + ks, fa[s][a]);
 #pragma unroll
-        for (int a = 0; a < TM; ++a) load_frag(pa[a] + (more ? k + 16 : 0), oka[a] && more, na[a]);
-#pragma unroll
-        for (int b = 0; b < TN; ++b) load_frag(pb[b] + (more ? k + 16 : 0), okb[b] && more, nb[b]);
-#pragma unroll
-        for (int a = 0; a < TM; ++a)
-#pragma unroll
-            for (int b = 0; b < TN; ++b) acc[a][b] = mma(fa[a], fb[b], acc[a][b]);
-#pragma unroll
-        for (int a = 0; a < TM; ++a) fa[a] = na[a];
-#pragma unroll
-        for (int b = 0; b < TN; ++b) fb[b] = nb[b];
+        for (int b = 0; b < TN; ++b) load_frag_raw(pb[b] + ks, fb[s][b]);
+        __builtin_amdgcn_sched_barrier(0);
     }
+    const int64_t nsteps = K / 16, kfull = nsteps / D * (16 * D);
+    for (int64_t k0 = 0; k0 < kfull; k0 += 16 * D) {
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TN; ++b) acc[a][b] = mma(fa[s][a], fb[s][b], acc[a][b]);
+            const int64_t kn = k0 + 16 * (s + D) < K ? k0 + 16 * (s + D) : 0;
+#pragma unroll
+            for (int a = 0; a < TM; ++a) load_frag_raw(pa[a] + kn, fa[s][a]);
+#pragma unroll
+            for (int b = 0; b < TN; ++b) load_frag_raw(pb[b] + kn, fb[s][b]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    const int rem = (int)(nsteps - kfull / 16);
+#pragma unroll
+    for (int s = 0; s < D - 1; ++s)
+        if (s < rem) {
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TN; ++b) acc[a][b] = mma(fa[s][a], fb[s][b], acc[a][b]);
+        }
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
@@ -654,12 +689,77 @@ __device__ inline void to_frag(f4 v0, f4 v1, Frag<uint16_t>& f)
                 (uint32_t)to_bf16(v1[2]) | ((uint32_t)to_bf16(v1[3]) << 16)};
 }
 
+// ------------------------------------- dense-layer gradient preparation
+
+__device__ inline void store8(uint16_t* p, f4 v0, f4 v1)
+{
+    Frag<uint16_t> f;
+    to_frag(v0, v1, f);
+    *reinterpret_cast<u32x4*>(p) = f.v;
+}
+__device__ inline void store8(float* p, f4 v0, f4 v1)
+{
+    reinterpret_cast<f4*>(p)[0] = v0;
+    reinterpret_cast<f4*>(p)[1] = v1;
+}
+
+// What a small batch needs of one dense layer's output gradient d (rows x
+// cols FP32, row stride cols) before its two products, in one pass with one
+// thread per column: d' = d act'(pre), the bias gradient gb[c] = sum_r d'[r][c]
+// (four interleaved row sums added as ((s0 + s1) + s2) + s3, the order of
+// k_colsum_final), the copy dl[r cols + c] = d'[r][c] and the transpose
+// dT[c ldt + r] (zeros for rows <= r < ldt; ldt a multiple of 8, 8 rows per
+// store), both rounded to T.  Rows go in blocks of 32 whose loads are all
+// issued before the first is used (rows past the last read the last row and
+// are replaced by zeros); one-wave workgroups spread the columns over the CUs.
+constexpr int kPrepWG = 64;
+template <typename T>
+__global__ __launch_bounds__(kPrepWG) void k_dense_bwd_prep(const float* __restrict__ d,
+                                                            const float* __restrict__ pre, int64_t rows, int64_t cols,
+                                                            int act, float* __restrict__ gb, T* __restrict__ dl,
+                                                            T* __restrict__ dT, int64_t ldt)
+{
+    const int64_t c = (int64_t)blockIdx.x * kPrepWG + threadIdx.x;
+    if (c >= cols) return;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t r0 = 0; r0 < ldt; r0 += 32) {
+        float x[32], p[32];
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            const int64_t r = r0 + j < rows ? r0 + j : rows - 1;
+            x[j] = rows > 0 ? d[r * cols + c] : 0.f;
+            p[j] = rows > 0 && act == 1 ? pre[r * cols + c] : 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            const int64_t r = r0 + j;
+            float v = 0.f;
+            if (r < rows) {
+                v = x[j];
+                if (act == 1) {
+                    const float sg = sigmoid(p[j]);
+                    v *= sg * (1.f + p[j] * (1.f - sg));
+                }
+                s[j & 3] += v;                // r0 is a multiple of 32, so r % 4 == j % 4
+                dl[r * cols + c] = from_f32<T>(v);
+            }
+            x[j] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (r0 + 8 * q < ldt)
+                store8(dT + c * ldt + r0 + 8 * q, f4{x[8 * q], x[8 * q + 1], x[8 * q + 2], x[8 * q + 3]},
+                       f4{x[8 * q + 4], x[8 * q + 5], x[8 * q + 6], x[8 * q + 7]});
+    }
+    gb[c] = ((s[0] + s[1]) + s[2]) + s[3];
+}
+
 // Gradient of a stride-2, 4x4, padding-1 transposed convolution's input:
 // out[p][ci] = sum_{tap, co} d[b][2y - 1 + ky][2x - 1 + kx][co] wT[ci][tap Co + co]
 // for input pixel p = (b, y, x), tap = ky 4 + kx.  The product of the im2col
 // rows of d (K = 16 Co) with wT, with 256 x 32 workgroup tiles (4 waves x two
-// 32-row MFMA tiles) and register double buffering as k_gemm_nt<.., 4, 1, 2, 1>,
-// except that the A operand is gathered from the FP32 gradient d as it is
+// 32-row MFMA tiles) as k_gemm_nt<.., 4, 1, 2, 1> and a register ring of three
+// K-steps, except that the A operand is gathered from the FP32 gradient d as it is
 // loaded (a lane's 8 consecutive k are 8 consecutive channels of one output
 // pixel, zero outside the grid) and rounded to T, so the column matrix is
 // never written.  Rows run along grid.x, 32-column tiles along grid.y.
@@ -1476,6 +1576,24 @@ int sfh_nn_transpose(sfh_ctx* ctx, int dtype, const float* in, int64_t ldi, int6
         hipLaunchKernelGGL(k_transpose<float>, grid, dim3(kWG), 0, ctx->stream, in, ldi, rows, cols, (float*)out, ldo,
                            ocols, act);
     return check_launch(ctx, "k_transpose");
+}
+
+int sfh_nn_dense_bwd_prep(sfh_ctx* ctx, int dtype, const float* d, const float* pre, int64_t rows, int64_t cols,
+                          int act, float* gb, void* dl, void* dT, int64_t ldt)
+{
+    SFH_NN_BEGIN(ctx);
+    SFH_ARG_CHECK(ctx, d && gb && dl && dT && (act == 0 || (act == 1 && pre)) && rows >= 0 && cols >= 0 &&
+                           ldt >= rows && ldt % 8 == 0 && ((uintptr_t)dT & 15u) == 0,
+                  "sfh_nn_dense_bwd_prep: bad arguments");
+    if (cols == 0) return SFH_OK;
+    const dim3 grid((unsigned)cdiv(cols, kPrepWG));
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_dense_bwd_prep<uint16_t>, grid, dim3(kPrepWG), 0, ctx->stream, d, pre, rows, cols, act,
+                           gb, (uint16_t*)dl, (uint16_t*)dT, ldt);
+    else
+        hipLaunchKernelGGL(k_dense_bwd_prep<float>, grid, dim3(kPrepWG), 0, ctx->stream, d, pre, rows, cols, act,
+                           gb, (float*)dl, (float*)dT, ldt);
+    return check_launch(ctx, "k_dense_bwd_prep");
 }
 
 int sfh_nn_weight_copies(sfh_ctx* ctx, int dtype, int64_t n, const sfh_nn_wcopy* mats)

@@ -286,10 +286,12 @@ class TemplateSurrogate:
                ("W0", "b0", self.pre0, None, 16, HID0))
         Bp = self.bp
         for Wn, bn, pre_out, pre_in, nin, nout in mlp:
-            self._act_bwd(d, pre_out, B * nout)
-            self._colsum(d, nout, B, nout, g[bn])
+            # gradient w.r.t. pre_out, its column sums (the bias gradient) and
+            # its low-precision copy and transpose, in one launch
+            self._c(self.lib.sfh_nn_dense_bwd_prep(self.ctx.handle, self.dt, d.data_ptr(), pre_out.data_ptr(), B, nout,
+                                                   SILU, g[bn].data_ptr(), self.dcols.data_ptr(),
+                                                   self.dcolsT.data_ptr(), Bp))
             # dW = d^T a_in
-            self._transpose(d, nout, B, nout, self.dcolsT, Bp)
             if pre_in is not None:
                 self._transpose(pre_in, nin, B, nin, self.actT, Bp, act=SILU)
             else:
@@ -298,7 +300,6 @@ class TemplateSurrogate:
             if pre_in is None:
                 break
             # d a_in = d W  (low-precision copy of d times the transposed weights)
-            self._bias_act(d, nout, None, B, nout, IDENT, None, self.dcols, nout)
             dn = self.dB if d is self.dA else self.dA
             self._gemm(self.dcols, nout, self.wT[Wn], nout, dn, nin, B, nin, nout)
             d = dn

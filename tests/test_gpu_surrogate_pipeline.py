@@ -108,7 +108,7 @@ def test_gather_and_indexed_mse(sfh, gpu_ctx):
 
 @pytest.mark.parametrize("dtype", [0, 1])
 @pytest.mark.parametrize("M,N,K", [(200, 40, 20000), (16, 32, 50016), (300, 130, 64), (1000, 24, 48), (777, 32, 2048),
-                                   (70000, 16, 32)])
+                                   (70000, 16, 32), (33, 200, 16), (129, 257, 80)])
 def test_gemm_split_k(sfh, gpu_ctx, dtype, M, N, K):
     import torch
     rng = np.random.default_rng(M + N + K)
@@ -390,3 +390,45 @@ def test_captured_training_equals_eager(sfh, gpu_ctx):
     assert s0 == s1 == 45 and len(h0) == len(h1) == 9
     assert h0 == h1
     assert np.array_equal(p0, p1)
+
+
+@pytest.mark.parametrize("dtype", [0, 1])
+@pytest.mark.parametrize("rows,cols,act", [(32, 7904, 1), (5, 300, 1), (40, 16, 0), (32, 512, 1)])
+def test_dense_bwd_prep_equals_separate_kernels(sfh, gpu_ctx, dtype, rows, cols, act):
+    """The one-pass preparation of a dense layer's output gradient equals, bit
+    for bit, the four kernels it replaces (activation derivative, column sums,
+    low-precision copy, padded transpose), and the NumPy restatement."""
+    import torch
+    rng = np.random.default_rng(rows * cols + act)
+    d = rng.normal(size=(rows, cols)).astype(np.float32)
+    pre = rng.normal(scale=2.0, size=(rows, cols)).astype(np.float32)
+    ldt = (rows + 15) // 16 * 16
+    tdt = torch.bfloat16 if dtype == 0 else torch.float32
+    d_d, pre_d = torch.from_numpy(d).cuda(), torch.from_numpy(pre).cuda()
+    gb = torch.full((cols,), np.nan, dtype=torch.float32, device="cuda")
+    dl = torch.full((rows, cols), np.nan, dtype=tdt, device="cuda")
+    dT = torch.full((cols, ldt), np.nan, dtype=tdt, device="cuda")
+    d2 = d_d.clone()
+    gb2 = torch.empty_like(gb)
+    dl2 = torch.empty_like(dl)
+    dT2 = torch.empty_like(dT)
+    torch.cuda.synchronize()
+    lib, h = gpu_ctx.lib, gpu_ctx.handle
+    gpu_ctx.check(lib.sfh_nn_dense_bwd_prep(h, dtype, d_d.data_ptr(), pre_d.data_ptr(), rows, cols, act,
+                                            gb.data_ptr(), dl.data_ptr(), dT.data_ptr(), ldt))
+    gpu_ctx.check(lib.sfh_nn_act_bwd(h, d2.data_ptr(), pre_d.data_ptr(), rows * cols, act, d2.data_ptr()))
+    gpu_ctx.check(lib.sfh_nn_colsum(h, d2.data_ptr(), cols, rows, cols, 0.0, gb2.data_ptr()))
+    gpu_ctx.check(lib.sfh_nn_bias_act(h, dtype, d2.data_ptr(), cols, None, rows, cols, 0, None, dl2.data_ptr(),
+                                      cols))
+    gpu_ctx.check(lib.sfh_nn_transpose(h, dtype, d2.data_ptr(), cols, rows, cols, dT2.data_ptr(), ldt, cols, 0))
+    gpu_ctx.synchronize()
+    assert torch.equal(gb, gb2)
+    assert torch.equal(dl, dl2)
+    assert torch.equal(dT, dT2)
+    x = pre.astype(np.float64)
+    s = 1.0 / (1.0 + np.exp(-x))
+    ref = d * (s * (1.0 + x * (1.0 - s))) if act else d.astype(np.float64)
+    assert np.allclose(gb.cpu().numpy(), ref.sum(axis=0), rtol=1e-5, atol=1e-5)
+    assert np.all(dT.float().cpu().numpy()[:, rows:] == 0.0)
+    tol = 1e-2 if dtype == 0 else 1e-6
+    assert np.allclose(dT.float().cpu().numpy()[:, :rows], ref.T, rtol=tol, atol=tol)
