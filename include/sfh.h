@@ -406,6 +406,11 @@ int sfh_nn_bias_act(sfh_ctx* ctx, int dtype, const float* X, int64_t ldx, const 
                     int64_t cols, int act, float* pre, void* Y, int64_t ldy);
 /* dX = dY * act'(pre) elementwise (dX may alias dY). */
 int sfh_nn_act_bwd(sfh_ctx* ctx, const float* dY, const float* pre, int64_t n, int act, float* dX);
+/* The same over P pixels of 32 channels (16-byte aligned), with the channel
+ * sums dsum[c] = sum_p dX[p, c] (a convolution's bias gradient) from the same
+ * pass, added in an order fixed by P. */
+int sfh_nn_act_bwd_chansum(sfh_ctx* ctx, const float* dY, const float* pre, int64_t P, int act, float* dX,
+                           float* dsum);
 /* out (ocols x ldo, dtype) = act(in)^T (in: rows x cols FP32, ldi), zero padded. */
 int sfh_nn_transpose(sfh_ctx* ctx, int dtype, const float* in, int64_t ldi, int64_t rows, int64_t cols, void* out,
                      int64_t ldo, int64_t ocols, int act);
@@ -467,11 +472,12 @@ int sfh_nn_conv_t_dweight(sfh_ctx* ctx, int dtype, const float* d, const float* 
 /* The same transposed convolution with one output channel (B x 2H x 2W) and a
  * per-pixel bias map, applied to act(in) (act 0: identity, 1: SiLU, in then
  * holding pre-activations); its backward gives din (the gradient of in:
- * through act'), dw (16 x Ci) and dbmap (any may be NULL). */
+ * through act'), dsum (Ci: the channel sums of din, the previous layer's bias
+ * gradient; needs din), dw (16 x Ci) and dbmap (any may be NULL). */
 int sfh_nn_conv1_fwd(sfh_ctx* ctx, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci, const float* w,
                      const float* bmap, int act, float* out);
 int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci,
-                     const float* w, int act, float* din, float* dw, float* dbmap);
+                     const float* w, int act, float* din, float* dsum, float* dw, float* dbmap);
 /* Mean squared error over the top-left R x S of each of B outputs of H2 x W2
  * against target rows (R x S each): output b against row rows[b] (b when rows
  * is NULL).  dout (may be NULL) gets its gradient.  With loss non-NULL the

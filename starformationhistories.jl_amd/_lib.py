@@ -146,6 +146,7 @@ SIGNATURES = [
     ("sfh_nn_gemm_nt", _int, [_vp, _int, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _i64, C.c_float, C.c_float]),
     ("sfh_nn_bias_act", _int, [_vp, _int, _vp, _i64, _vp, _i64, _i64, _int, _vp, _vp, _i64]),
     ("sfh_nn_act_bwd", _int, [_vp, _vp, _vp, _i64, _int, _vp]),
+    ("sfh_nn_act_bwd_chansum", _int, [_vp, _vp, _vp, _i64, _int, _vp, _vp]),
     ("sfh_nn_transpose", _int, [_vp, _int, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _int]),
     ("sfh_nn_dense_bwd_prep", _int, [_vp, _int, _vp, _vp, _i64, _i64, _int, _vp, _vp, _vp, _i64]),
     ("sfh_nn_weight_copies", _int, [_vp, _int, _i64, _vp]),
@@ -157,7 +158,7 @@ SIGNATURES = [
     ("sfh_nn_conv_t_dweight", _int, [_vp, _int, _vp, _vp, _int, _i64, _i64, _i64, _i64, _vp]),
     ("sfh_nn_conv_t_dinput", _int, [_vp, _int, _vp, _i64, _i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64]),
     ("sfh_nn_conv1_fwd", _int, [_vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _int, _vp]),
-    ("sfh_nn_conv1_bwd", _int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _int, _vp, _vp, _vp]),
+    ("sfh_nn_conv1_bwd", _int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _int, _vp, _vp, _vp, _vp]),
     ("sfh_nn_mse", _int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, C.POINTER(C.c_float), _vp]),
     ("sfh_nn_template_loss", _int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, C.c_float, C.c_float,
                                     C.c_float, C.c_float, C.c_float, _vp, _vp]),

@@ -547,44 +547,120 @@ __device__ inline int64_t udiv31(int64_t n, int64_t d, double inv)
     return q;
 }
 
+// Per-workgroup channel sums of 32-channel pixels: thread t holds f4 sums of
+// channel quad t % 8 (a grid-stride loop whose stride is a multiple of 8
+// quads keeps each thread on one quad); the workgroup's 32 pixel slots are
+// added through LDS in slot order into part[wg][c].
+__device__ inline void chan_part32(f4 s, f4* red, float* __restrict__ part)
+{
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        const int q = threadIdx.x >> 2, j = threadIdx.x & 3;
+        float t = 0.f;
+        for (int slot = 0; slot < kWG / 8; ++slot) t += red[slot * 8 + q][j];
+        part[(int64_t)blockIdx.x * 32 + threadIdx.x] = t;
+    }
+}
+
 // din[p][c] = sum_tap window(p)[tap] w[tap][c], times act'(in[p][c]) with
 // act = SiLU (din is then the gradient of the pre-activation in); thread =
-// (pixel, 4 channels).
+// (pixel, 4 channels), grid-stride over the pixels.  With part non-null, also
+// the workgroup's channel sums of din (chan_part32).
 __global__ __launch_bounds__(kWG) void k_conv1_din(const float* __restrict__ dout, const float* __restrict__ in,
                                                    int B, int H, int W, double inv_w, double inv_h,
-                                                   const float* __restrict__ w, int act, float* __restrict__ din)
+                                                   const float* __restrict__ w, int act, float* __restrict__ din,
+                                                   float* __restrict__ part)
 {
     __shared__ f4 sw[16][kC1Ci / 4];
+    __shared__ f4 red[kWG];
     if (threadIdx.x < 16 * kC1Ci / 4) sw[threadIdx.x / (kC1Ci / 4)][threadIdx.x % (kC1Ci / 4)] =
         reinterpret_cast<const f4*>(w)[threadIdx.x];
     __syncthreads();
-    const int64_t P = (int64_t)B * H * W;
-    const int64_t g = (int64_t)blockIdx.x * kWG + threadIdx.x;
-    if (g >= P * (kC1Ci / 4)) return;
-    const int q = (int)(g % (kC1Ci / 4));
-    const int64_t p = g / (kC1Ci / 4);
-    const int64_t r = udiv31(p, W, inv_w), b = udiv31(r, H, inv_h);
-    const int x = (int)(p - r * W), y = (int)(r - b * H);
-    float d[16];
-    conv1_window(dout, b, y, x, 2 * H, 2 * W, d);
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int64_t P = (int64_t)B * H * W, nq = P * (kC1Ci / 4);
+    const int q = (int)(threadIdx.x % (kC1Ci / 4));
+    f4 sum = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t g = (int64_t)blockIdx.x * kWG + threadIdx.x; g < nq; g += (int64_t)gridDim.x * kWG) {
+        // keeps the weight reads from LDS inside the loop: hoisted, the 16
+        // quads would hold 64 registers and halve the waves in flight
+        asm volatile("" ::: "memory");
+        const int64_t p = g / (kC1Ci / 4);
+        const int64_t r = udiv31(p, W, inv_w), b = udiv31(r, H, inv_h);
+        const int x = (int)(p - r * W), y = (int)(r - b * H);
+        float d[16];
+        conv1_window(dout, b, y, x, 2 * H, 2 * W, d);
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        const f4 wv = sw[t][q];
-        acc[0] = fmaf(d[t], wv[0], acc[0]);
-        acc[1] = fmaf(d[t], wv[1], acc[1]);
-        acc[2] = fmaf(d[t], wv[2], acc[2]);
-        acc[3] = fmaf(d[t], wv[3], acc[3]);
-    }
-    if (act == 1) {
-        const f4 x = *reinterpret_cast<const f4*>(in + p * kC1Ci + 4 * q);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float s = sigmoid(x[j]);
-            acc[j] *= s * (1.f + x[j] * (1.f - s));
+        for (int t = 0; t < 16; ++t) {
+            const f4 wv = sw[t][q];
+            acc[0] = fmaf(d[t], wv[0], acc[0]);
+            acc[1] = fmaf(d[t], wv[1], acc[1]);
+            acc[2] = fmaf(d[t], wv[2], acc[2]);
+            acc[3] = fmaf(d[t], wv[3], acc[3]);
         }
+        if (act == 1) {
+            const f4 xv = *reinterpret_cast<const f4*>(in + p * kC1Ci + 4 * q);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float sg = sigmoid(xv[j]);
+                acc[j] *= sg * (1.f + xv[j] * (1.f - sg));
+            }
+        }
+        *reinterpret_cast<f4*>(din + p * kC1Ci + 4 * q) = acc;
+        sum += acc;
     }
-    *reinterpret_cast<f4*>(din + p * kC1Ci + 4 * q) = acc;
+    if (part) chan_part32(sum, red, part);
+}
+
+// dX = dY act'(pre) over P pixels of 32 channels (dX may alias dY), with the
+// workgroup's channel sums of dX (chan_part32); thread = (pixel, 4 channels).
+__global__ __launch_bounds__(kWG) void k_act_bwd_chan32(const float* dY, const float* __restrict__ pre, int64_t P,
+                                                        int act, float* dX, float* __restrict__ part)
+{
+    __shared__ f4 red[kWG];
+    f4 sum = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t g = (int64_t)blockIdx.x * kWG + threadIdx.x; g < P * 8; g += (int64_t)gridDim.x * kWG) {
+        f4 d = reinterpret_cast<const f4*>(dY)[g];
+        if (act == 1) {
+            const f4 x = reinterpret_cast<const f4*>(pre)[g];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float sg = sigmoid(x[j]);
+                d[j] *= sg * (1.f + x[j] * (1.f - sg));
+            }
+        }
+        reinterpret_cast<f4*>(dX)[g] = d;
+        sum += d;
+    }
+    chan_part32(sum, red, part);
+}
+
+// out[c] = sum over nparts (at most kChanMaxParts) of part[w][c] (32
+// channels): 32 groups of 32 threads take parts w = g, g + 32, ..., all
+// loaded before the first is added, then the groups are added in order.
+constexpr int kChanFinWG = 1024, kChanMaxParts = 2048;
+__global__ __launch_bounds__(kChanFinWG) void k_chan_final32(const float* __restrict__ part, int64_t nparts,
+                                                             float* __restrict__ out)
+{
+    __shared__ float red[32][33];
+    const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
+    constexpr int kPer = kChanMaxParts / 32;
+    float v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int64_t w = g + 32 * k;
+        v[k] = w < nparts ? part[w * 32 + c] : 0.f;
+    }
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) t += v[k];
+    red[g][c] = t;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        float u = 0.f;
+        for (int k = 0; k < 32; ++k) u += red[k][threadIdx.x];
+        out[threadIdx.x] = u;
+    }
 }
 
 // Partial weight gradients: part[wg][tap*32 + c] = sum over the workgroup's
@@ -1420,6 +1496,12 @@ __global__ __launch_bounds__(kWG) void k_adam(float* __restrict__ p, const float
 }
 
 inline unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>(cdiv(n, kWG), 8192); }
+// workgroups of the channel-sum kernels over P pixels of 32 channels: a fixed
+// function of P, so the partials are always added in the same order
+inline int64_t chan_grid(int64_t P)
+{
+    return std::max<int64_t>(1, std::min<int64_t>(cdiv(P * 8, kWG), kChanMaxParts));
+}
 
 }  // namespace nn
 }  // namespace sfh
@@ -1440,8 +1522,11 @@ int colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols,
            int64_t N, int64_t ldo, const char* name);
 
 // C = alpha A B^T + beta C.  Products with few output tiles and a long K (the
-// weight gradients, K = pixels) are split along K so that the launch fills the
-// chip: about 1024 workgroups, slices of at least 256, summed in order.
+// weight gradients, K = pixels: K >= 1024, slices of at least 256; a small
+// batch's dense layers, M <= 64: K >= 256, slices of at least 64) are split
+// along K so that the launch fills the chip: about 1024 workgroups, summed in
+// order.  A workgroup's K-steps run one after another, so a few tiles with a
+// long K would leave the launch bound by the latency of its loads.
 int gemm(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* B, int64_t ldb, float* C, int64_t ldc,
          int64_t M, int64_t N, int64_t K, float alpha, float beta)
 {
@@ -1450,8 +1535,9 @@ int gemm(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* B, int
     const int64_t tm = narrow ? 256 : 128, tn = narrow ? 32 : 128;
     const int64_t tiles = cdiv(N, tn) * cdiv(M, tm);
     int64_t splits = 1, kchunk = std::max<int64_t>(K, 16);
-    if (tiles < 512 && K >= 1024) {
-        splits = std::min<int64_t>(std::max<int64_t>(1, 1024 / tiles), K / 256);
+    const int64_t min_slice = M <= 64 ? 64 : 256;
+    if (tiles < 512 && K >= 4 * min_slice) {
+        splits = std::min<int64_t>(std::max<int64_t>(1, 1024 / tiles), K / min_slice);
         kchunk = cdiv(cdiv(K, splits), 16) * 16;
         splits = cdiv(K, kchunk);
     }
@@ -1576,6 +1662,23 @@ int sfh_nn_transpose(sfh_ctx* ctx, int dtype, const float* in, int64_t ldi, int6
         hipLaunchKernelGGL(k_transpose<float>, grid, dim3(kWG), 0, ctx->stream, in, ldi, rows, cols, (float*)out, ldo,
                            ocols, act);
     return check_launch(ctx, "k_transpose");
+}
+
+int sfh_nn_act_bwd_chansum(sfh_ctx* ctx, const float* dY, const float* pre, int64_t P, int act, float* dX,
+                           float* dsum)
+{
+    SFH_NN_BEGIN(ctx);
+    SFH_ARG_CHECK(ctx, dY && dX && dsum && P >= 0 && (act == 0 || (act == 1 && pre)) && aligned16(dY) &&
+                           aligned16(dX) && (!pre || aligned16(pre)),
+                  "sfh_nn_act_bwd_chansum: bad arguments");
+    const int64_t nwg = chan_grid(P);
+    void* cp = nullptr;
+    if (int rc = scratch(ctx, "nn_chan_part", (size_t)nwg * 32 * sizeof(float), &cp)) return rc;
+    hipLaunchKernelGGL(k_act_bwd_chan32, dim3((unsigned)nwg), dim3(kWG), 0, ctx->stream, dY, pre, P, act, dX,
+                       (float*)cp);
+    if (int rc = check_launch(ctx, "k_act_bwd_chan32")) return rc;
+    hipLaunchKernelGGL(k_chan_final32, dim3(1), dim3(kChanFinWG), 0, ctx->stream, (const float*)cp, nwg, dsum);
+    return check_launch(ctx, "k_chan_final32");
 }
 
 int sfh_nn_dense_bwd_prep(sfh_ctx* ctx, int dtype, const float* d, const float* pre, int64_t rows, int64_t cols,
@@ -1742,11 +1845,11 @@ int sfh_nn_conv1_fwd(sfh_ctx* ctx, const float* in, int64_t B, int64_t H, int64_
 // (P x Ci), times act'(in); dw = dcols^T act(in) (16 x Ci, K = pixels, split
 // along K); dbmap = sum over b of dout.
 int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B, int64_t H, int64_t W, int64_t Ci,
-                     const float* w, int act, float* din, float* dw, float* dbmap)
+                     const float* w, int act, float* din, float* dsum, float* dw, float* dbmap)
 {
     SFH_NN_BEGIN(ctx);
     SFH_ARG_CHECK(ctx, dout && B >= 0 && H >= 1 && W >= 1 && Ci >= 16 && Ci % 16 == 0 && (!din || w) &&
-                           (!dw || in) && (act == 0 || act == 1) && (act == 0 || !din || in),
+                           (!dw || in) && (act == 0 || act == 1) && (act == 0 || !din || in) && (!dsum || din),
                   "sfh_nn_conv1_bwd: Ci a positive multiple of 16, w with din, in with dw (and with din when act "
                   "is 1), act 0 or 1");
     const int64_t P = B * H * W, Pp = cdiv(P, 16) * 16;
@@ -1754,9 +1857,18 @@ int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B
     void* p = nullptr;
     if ((din || dw) && conv1_direct(B, H, W, Ci, (dw || act) ? in : nullptr, din ? w : nullptr, din)) {
         if (din) {
-            hipLaunchKernelGGL(k_conv1_din, dim3((unsigned)cdiv(P * (kC1Ci / 4), kWG)), dim3(kWG), 0, ctx->stream,
-                               dout, in, (int)B, (int)H, (int)W, 1.0 / (double)W, 1.0 / (double)H, w, act, din);
+            const int64_t nwg = chan_grid(P);
+            void* cp = nullptr;
+            if (dsum)
+                if (int rc = scratch(ctx, "nn_chan_part", (size_t)nwg * 32 * sizeof(float), &cp)) return rc;
+            hipLaunchKernelGGL(k_conv1_din, dim3((unsigned)nwg), dim3(kWG), 0, ctx->stream, dout, in, (int)B, (int)H,
+                               (int)W, 1.0 / (double)W, 1.0 / (double)H, w, act, din, (float*)cp);
             if (int rc = check_launch(ctx, "k_conv1_din")) return rc;
+            if (dsum) {
+                hipLaunchKernelGGL(k_chan_final32, dim3(1), dim3(kChanFinWG), 0, ctx->stream, (const float*)cp, nwg,
+                                   dsum);
+                if (int rc = check_launch(ctx, "k_chan_final32")) return rc;
+            }
         }
         if (dw) {
             // one partial per workgroup (at most 256: a single-pass column sum)
@@ -1792,6 +1904,8 @@ int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B
                                    P * Ci, act, din);
                 if (int rc = check_launch(ctx, "k_act_bwd")) return rc;
             }
+            if (dsum)
+                if (int rc = colsum(ctx, din, Ci, P, Ci, 1.f, 0.f, dsum, Ci, Ci, "nn_conv1_dsum")) return rc;
         }
         if (dw) {
             // dw = dcols^T (16 x P) times in (P x Ci): operands dcols^T and in^T, K = Pp (zero padded)

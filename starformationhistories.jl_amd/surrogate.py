@@ -175,12 +175,6 @@ class TemplateSurrogate:
         self._c(self.lib.sfh_nn_transpose(self.ctx.handle, self.dt, X.data_ptr(), ldi, rows, cols, out.data_ptr(),
                                           ldo, cols, act))
 
-    def _act_bwd(self, d, pre, n):
-        self._c(self.lib.sfh_nn_act_bwd(self.ctx.handle, d.data_ptr(), pre.data_ptr(), n, SILU, d.data_ptr()))
-
-    def _colsum(self, X, ld, rows, cols, out):
-        self._c(self.lib.sfh_nn_colsum(self.ctx.handle, X.data_ptr(), ld, rows, cols, 0.0, out.data_ptr()))
-
     def _refresh_weights(self):
         """Low-precision copies (and transposes) of the master weights, one launch."""
         v = self.views
@@ -256,9 +250,10 @@ class TemplateSurrogate:
         B, v, g = self.B, self.views, self.gviews
         H, W = 8 * H0, 8 * W0                                   # input grid of the last layer
         # single-channel output layer
+        # (with the channel sums of its input gradient: the last decoder layer's bias gradient)
         self._c(self.lib.sfh_nn_conv1_bwd(self.ctx.handle, self.dout.data_ptr(), self.pre5.data_ptr(), B, H, W, CH,
-                                          v["w4"].data_ptr(), SILU, self.dA.data_ptr(), g["w4"].data_ptr(),
-                                          g["bmap"].data_ptr()))
+                                          v["w4"].data_ptr(), SILU, self.dA.data_ptr(), g["c3"].data_ptr(),
+                                          g["w4"].data_ptr(), g["bmap"].data_ptr()))
         d = self.dA                                              # gradient w.r.t. pre5
         layers = (("K3", "c3", self.pre5, self.a4, self.pre4),
                   ("K2", "c2", self.pre4, self.a3, self.pre3),
@@ -269,8 +264,9 @@ class TemplateSurrogate:
             Hi, Wi = H // 2, W // 2
             Pi, Po = B * Hi * Wi, B * H * W
             if li > 0:
-                self._act_bwd(d, pre_out, Po * CH)               # -> gradient w.r.t. pre_out
-            self._colsum(d, CH, Po, CH, g[cb])
+                # -> gradient w.r.t. pre_out, and its channel sums (the bias gradient)
+                self._c(self.lib.sfh_nn_act_bwd_chansum(self.ctx.handle, d.data_ptr(), pre_out.data_ptr(), Po, SILU,
+                                                        d.data_ptr(), g[cb].data_ptr()))
             # dK = im2col(d)^T SiLU(pre_in), both operands gathered into LDS by the kernel
             self._c(self.lib.sfh_nn_conv_t_dweight(self.ctx.handle, self.dt, d.data_ptr(), pre_in.data_ptr(), SILU,
                                                    B, Hi, Wi, CH, g[K].data_ptr()))

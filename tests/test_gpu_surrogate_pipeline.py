@@ -163,12 +163,13 @@ def test_output_layer_against_oracle(sfh, gpu_ctx, B, H, W, Ci, act):
     din_d = torch.empty_like(a_d)
     dw_d = torch.empty_like(w_d)
     dbm_d = torch.empty_like(bm_d)
+    dsum_d = torch.full((Ci,), np.nan, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
     lib, h = gpu_ctx.lib, gpu_ctx.handle
     gpu_ctx.check(lib.sfh_nn_conv1_fwd(h, a_d.data_ptr(), B, H, W, Ci, w_d.data_ptr(), bm_d.data_ptr(), act,
                                        out_d.data_ptr()))
     gpu_ctx.check(lib.sfh_nn_conv1_bwd(h, do_d.data_ptr(), a_d.data_ptr(), B, H, W, Ci, w_d.data_ptr(), act,
-                                       din_d.data_ptr(), dw_d.data_ptr(), dbm_d.data_ptr()))
+                                       din_d.data_ptr(), dsum_d.data_ptr(), dw_d.data_ptr(), dbm_d.data_ptr()))
     gpu_ctx.synchronize()
     x64, w64 = a.astype(np.float64), w.astype(np.float64)
     a64 = R.silu(x64) if act else x64
@@ -182,14 +183,18 @@ def test_output_layer_against_oracle(sfh, gpu_ctx, B, H, W, Ci, act):
     assert rel(din_d.cpu().numpy(), rdin) < 1e-6
     assert rel(dw_d.cpu().numpy(), rdw) < 1e-6
     assert rel(dbm_d.cpu().numpy(), dout.astype(np.float64).sum(0).ravel()) < 1e-6
+    # channel sums of din (the previous layer's bias gradient) from the same pass
+    rsum = rdin.reshape(-1, Ci).sum(0)
+    assert np.all(np.abs(dsum_d.cpu().numpy() - rsum) <= 1e-5 * np.abs(rdin).reshape(-1, Ci).sum(0) + 1e-6)
 
 
 @pytest.mark.parametrize("dtype", [0, 1])
-@pytest.mark.parametrize("B,Hi,Wi,Co,Ci", [(2, 13, 19, 32, 32), (1, 5, 7, 16, 48), (3, 26, 38, 32, 32)])
+@pytest.mark.parametrize("B,Hi,Wi,Co,Ci", [(2, 13, 19, 32, 32), (2, 5, 7, 16, 48), (3, 26, 38, 32, 32)])
 def test_conv_t_dinput_equals_im2col_product(sfh, gpu_ctx, dtype, B, Hi, Wi, Co, Ci):
     """The input gradient with the column matrix gathered as it is read is the
     product of the written column matrix (sfh_nn_im2col) with the weights, to
-    the bit: same rounded operands, same MFMA order."""
+    the bit: same rounded operands, same MFMA order (more than 64 rows and
+    K = 16 Co < 1024, so the written product is not split along K)."""
     import torch
     rng = np.random.default_rng(B * 100 + Hi + Wi)
     tdt = torch.bfloat16 if dtype == 0 else torch.float32
@@ -432,3 +437,27 @@ def test_dense_bwd_prep_equals_separate_kernels(sfh, gpu_ctx, dtype, rows, cols,
     assert np.all(dT.float().cpu().numpy()[:, rows:] == 0.0)
     tol = 1e-2 if dtype == 0 else 1e-6
     assert np.allclose(dT.float().cpu().numpy()[:, :rows], ref.T, rtol=tol, atol=tol)
+
+
+@pytest.mark.parametrize("act", [0, 1])
+@pytest.mark.parametrize("P", [1, 7, 4096, 32 * 52 * 76, 32 * 104 * 152])
+def test_act_bwd_chansum(sfh, gpu_ctx, P, act):
+    """Activation gradient over P pixels of 32 channels with the channel sums
+    from the same pass (more pixels than threads at the largest sizes): dX as
+    the elementwise kernel gives it, bit for bit, and the sums against FP64."""
+    import torch
+    rng = np.random.default_rng(P + act)
+    dY = torch.from_numpy(rng.normal(size=(P, 32)).astype(np.float32)).cuda()
+    pre = torch.from_numpy(rng.normal(scale=2.0, size=(P, 32)).astype(np.float32)).cuda()
+    dX = torch.empty_like(dY)
+    dX2 = torch.empty_like(dY)
+    dsum = torch.full((32,), np.nan, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    lib, h = gpu_ctx.lib, gpu_ctx.handle
+    gpu_ctx.check(lib.sfh_nn_act_bwd_chansum(h, dY.data_ptr(), pre.data_ptr(), P, act, dX.data_ptr(),
+                                             dsum.data_ptr()))
+    gpu_ctx.check(lib.sfh_nn_act_bwd(h, dY.data_ptr(), pre.data_ptr(), P * 32, act, dX2.data_ptr()))
+    gpu_ctx.synchronize()
+    assert torch.equal(dX, dX2)
+    ref = dX2.double().cpu().numpy()
+    assert np.all(np.abs(dsum.cpu().numpy() - ref.sum(0)) <= 1e-5 * np.abs(ref).sum(0) + 1e-6)
