@@ -87,6 +87,30 @@ __device__ inline void load_frag_raw(const float* __restrict__ p, Frag<float>& f
     f.v[1] = *reinterpret_cast<const f4*>(p + 4);
 }
 
+// Branch-free masked loads: a buffer resource over a tensor of nbytes
+// (< 2^31) returns zeros for an offset at or past nbytes without touching
+// memory, so a gather whose lanes fall outside the grid needs no branch (a
+// branch around a load makes the wait that follows it cover every load in
+// flight).  kNoRow is such an offset.
+constexpr uint32_t kNoRow = 0x80000000u;
+__device__ inline __amdgpu_buffer_rsrc_t tensor_rsrc(const void* p, uint32_t nbytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)nbytes, 0x00020000);
+}
+__device__ inline u32x4 buf_load16(__amdgpu_buffer_rsrc_t rs, uint32_t off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+}
+__device__ inline void load_frag_buf(__amdgpu_buffer_rsrc_t rs, uint32_t off, Frag<uint16_t>& f)
+{
+    f.v = buf_load16(rs, off);
+}
+__device__ inline void load_frag_buf(__amdgpu_buffer_rsrc_t rs, uint32_t off, Frag<float>& f)
+{
+    f.v[0] = __builtin_bit_cast(f4, buf_load16(rs, off));
+    f.v[1] = __builtin_bit_cast(f4, buf_load16(rs, off + 16));
+}
+
 __device__ inline void load_frag_lds(const uint16_t* p, Frag<uint16_t>& f) { f.v = *reinterpret_cast<const u32x4*>(p); }
 __device__ inline void load_frag_lds(const float* p, Frag<float>& f)
 {
@@ -840,8 +864,8 @@ __global__ __launch_bounds__(kPrepWG) void k_dense_bwd_prep(const float* __restr
 // pixel, zero outside the grid) and rounded to T, so the column matrix is
 // never written.  Rows run along grid.x, 32-column tiles along grid.y.
 template <typename T, int Co>
-__global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d, int B, int Hi, int Wi,
-                                                     double inv_w, double inv_h, const T* __restrict__ wT,
+__global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d, uint32_t d_bytes, int B, int Hi,
+                                                     int Wi, double inv_w, double inv_h, const T* __restrict__ wT,
                                                      int64_t ldw, int64_t Ci, float* __restrict__ out, int64_t ldo)
 {
     constexpr int TM = 2;
@@ -852,7 +876,8 @@ __global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d
     const int r = lane & 31, h = lane >> 5;
     const int H2 = 2 * Hi, W2 = 2 * Wi;
     constexpr int K = 16 * Co, NS = K / 16;
-    const float* img[TM];
+    const __amdgpu_buffer_rsrc_t rs = tensor_rsrc(d, d_bytes);
+    uint32_t img[TM];    // element offset of the lane's image
     int y2[TM], x2[TM];
     bool oka[TM];
 #pragma unroll
@@ -861,24 +886,26 @@ __global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d
         oka[a] = p < P;
         const int64_t pp = oka[a] ? p : 0;
         const int64_t rr = udiv31(pp, Wi, inv_w), b = udiv31(rr, Hi, inv_h);
-        img[a] = d + b * H2 * (int64_t)W2 * Co;
+        img[a] = (uint32_t)(b * H2 * (int64_t)W2 * Co);
         x2[a] = 2 * (int)(pp - rr * Wi) - 1;
         y2[a] = 2 * (int)(rr - b * Hi) - 1;
     }
     const int64_t col = n0 + r;
     const bool okb = col < Ci;
+    // columns past Ci read row 0 of wT into outputs that are never stored
     const T* pb = wT + (okb ? col : 0) * ldw + 8 * h;
-    auto load_a = [&](int a, int k0, bool more, Frag<T>& f) {
+    // the A operand's 8 FP32 values, rounded to T when the product reads
+    // them; outside the grid the buffer's range check returns zeros
+    struct Raw {
+        f4 lo, hi;
+    };
+    auto load_a = [&](int a, int k0, Raw& v) {
         const int tap = k0 / Co, co0 = k0 - tap * Co;
         const int Y = y2[a] + (tap >> 2), X = x2[a] + (tap & 3);
-        const bool ok = more && oka[a] && Y >= 0 && Y < H2 && X >= 0 && X < W2;
-        f4 v0 = {0.f, 0.f, 0.f, 0.f}, v1 = {0.f, 0.f, 0.f, 0.f};
-        if (ok) {
-            const f4* s = reinterpret_cast<const f4*>(img[a] + ((int64_t)Y * W2 + X) * Co + co0);
-            v0 = s[0];
-            v1 = s[1];
-        }
-        to_frag(v0, v1, f);
+        const bool ok = oka[a] && Y >= 0 && Y < H2 && X >= 0 && X < W2;
+        const uint32_t off = ok ? (img[a] + (uint32_t)((Y * W2 + X) * Co + co0)) * 4u : kNoRow;
+        v.lo = __builtin_bit_cast(f4, buf_load16(rs, off));
+        v.hi = __builtin_bit_cast(f4, buf_load16(rs, off + 16));
     };
     f32x16 acc[TM];
 #pragma unroll
@@ -887,22 +914,27 @@ __global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d
         for (int i = 0; i < 16; ++i) acc[a][i] = 0.f;
     // a ring of three fragment sets: the loads of step s + 2 are issued
     // before the products of step s
-    Frag<T> fa[3][TM], fb[3];
+    Raw fa[3][TM];
+    Frag<T> fb[3];
 #pragma unroll
     for (int s = 0; s < 2 && s < NS; ++s) {
 #pragma unroll
-        for (int a = 0; a < TM; ++a) load_a(a, 16 * s + 8 * h, true, fa[s][a]);
-        load_frag(pb + 16 * s, okb, fb[s]);
+        for (int a = 0; a < TM; ++a) load_a(a, 16 * s + 8 * h, fa[s][a]);
+        load_frag_raw(pb + 16 * s, fb[s]);
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         if (s + 2 < NS) {
 #pragma unroll
-            for (int a = 0; a < TM; ++a) load_a(a, 16 * (s + 2) + 8 * h, true, fa[(s + 2) % 3][a]);
-            load_frag(pb + 16 * (s + 2), okb, fb[(s + 2) % 3]);
+            for (int a = 0; a < TM; ++a) load_a(a, 16 * (s + 2) + 8 * h, fa[(s + 2) % 3][a]);
+            load_frag_raw(pb + 16 * (s + 2), fb[(s + 2) % 3]);
         }
 #pragma unroll
-        for (int a = 0; a < TM; ++a) acc[a] = mma(fa[s % 3][a], fb[s % 3], acc[a]);
+        for (int a = 0; a < TM; ++a) {
+            Frag<T> f;
+            to_frag(fa[s % 3][a].lo, fa[s % 3][a].hi, f);
+            acc[a] = mma(f, fb[s % 3], acc[a]);
+        }
     }
 #pragma unroll
     for (int a = 0; a < TM; ++a)
@@ -932,9 +964,10 @@ __global__ __launch_bounds__(kWG) void k_fill(float* __restrict__ out, int64_t n
 // (FP32) and, when aout is given, aout = SiLU(pre) rounded to T.  256 x 32
 // workgroup tiles as k_conv_dinput.
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_conv_fwd(const T* __restrict__ in, int B, int Hi, int Wi, double inv_w,
-                                                  double inv_h, const T* __restrict__ w, const float* __restrict__ bias,
-                                                  float* __restrict__ pre, T* __restrict__ aout)
+__global__ __launch_bounds__(kWG) void k_conv_fwd(const T* __restrict__ in, uint32_t in_bytes, int B, int Hi, int Wi,
+                                                  double inv_w, double inv_h, const T* __restrict__ w,
+                                                  const float* __restrict__ bias, float* __restrict__ pre,
+                                                  T* __restrict__ aout)
 {
     constexpr int TM = 2, C = 32, K = 4 * C;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -943,7 +976,22 @@ __global__ __launch_bounds__(kWG) void k_conv_fwd(const T* __restrict__ in, int 
     const int64_t P = (int64_t)B * Hi * Wi;
     const int64_t m0 = (int64_t)blockIdx.x * 256 + wave * (TM * 32);
     const int r = lane & 31, h = lane >> 5;
-    const T* img[TM];
+    const int H2 = 2 * Hi, W2 = 2 * Wi;
+    // output offset of each of the workgroup's 256 rows (-1 past P), one
+    // row per thread, so that the epilogue divides nothing
+    __shared__ int64_t qrow[256];
+    {
+        const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+        int64_t q = -1;
+        if (p < P) {
+            const int64_t rr = udiv31(p, Wi, inv_w), b = udiv31(rr, Hi, inv_h);
+            const int x = (int)(p - rr * Wi), y = (int)(rr - b * Hi);
+            q = ((b * H2 + 2 * y + py) * (int64_t)W2 + 2 * x + px) * C;
+        }
+        qrow[threadIdx.x] = q;
+    }
+    const __amdgpu_buffer_rsrc_t rs = tensor_rsrc(in, in_bytes);
+    uint32_t img[TM];    // element offset of the lane's image
     int yy[TM], xx[TM];
     bool oka[TM];
 #pragma unroll
@@ -952,17 +1000,19 @@ __global__ __launch_bounds__(kWG) void k_conv_fwd(const T* __restrict__ in, int 
         oka[a] = p < P;
         const int64_t pp = oka[a] ? p : 0;
         const int64_t rr = udiv31(pp, Wi, inv_w), b = udiv31(rr, Hi, inv_h);
-        img[a] = in + b * Hi * (int64_t)Wi * C;
+        img[a] = (uint32_t)(b * Hi * (int64_t)Wi * C);
         xx[a] = (int)(pp - rr * Wi);
         yy[a] = (int)(rr - b * Hi);
     }
-    // k = j 32 + ci with j = 2 dy + dx: tap (ky0 + 2 dy, kx0 + 2 dx)
+    // k = j 32 + ci with j = 2 dy + dx: tap (ky0 + 2 dy, kx0 + 2 dx); taps
+    // outside the grid read zeros through the buffer's range check
     auto load_a = [&](int a, int k0, Frag<T>& f) {
         const int j = k0 >> 5, ci0 = k0 & 31;
         const int ky = ky0 + 2 * (j >> 1), kx = kx0 + 2 * (j & 1);
         const int y = yy[a] + (py + 1 - ky) / 2, x = xx[a] + (px + 1 - kx) / 2;
         const bool ok = oka[a] && y >= 0 && y < Hi && x >= 0 && x < Wi;
-        load_frag(img[a] + ((int64_t)(ok ? y : 0) * Wi + (ok ? x : 0)) * C + ci0, ok, f);
+        const uint32_t e = img[a] + (uint32_t)((y * Wi + x) * C + ci0);
+        load_frag_buf(rs, ok ? e * (uint32_t)sizeof(T) : kNoRow, f);
     };
     auto load_b = [&](int k0, Frag<T>& f) {
         const int j = k0 >> 5, ci0 = k0 & 31;
@@ -995,16 +1045,14 @@ __global__ __launch_bounds__(kWG) void k_conv_fwd(const T* __restrict__ in, int 
         for (int a = 0; a < TM; ++a) acc[a] = mma(fa[s % 3][a], fb[s % 3], acc[a]);
     }
     const float bv = bias ? bias[r] : 0.f;
-    const int H2 = 2 * Hi, W2 = 2 * Wi;
+    __syncthreads();    // qrow
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            const int64_t row = m0 + a * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
-            if (row >= P) continue;
-            const int64_t rr = udiv31(row, Wi, inv_w), b = udiv31(rr, Hi, inv_h);
-            const int x = (int)(row - rr * Wi), y = (int)(rr - b * Hi);
-            const int64_t q = ((b * H2 + 2 * y + py) * (int64_t)W2 + 2 * x + px) * C + r;
+            const int64_t qb = qrow[wave * (TM * 32) + a * 32 + 8 * (i >> 2) + 4 * h + (i & 3)];
+            if (qb < 0) continue;
+            const int64_t q = qb + r;
             const float v = acc[a][i] + bv;
             pre[q] = v;
             if (aout) aout[q] = from_f32<T>(v * sigmoid(v));
@@ -1936,22 +1984,32 @@ int sfh_nn_conv_t_dinput(sfh_ctx* ctx, int dtype, const float* d, int64_t B, int
     SFH_ARG_CHECK(ctx, d && wT && out && (dtype == 0 || dtype == 1) && B >= 0 && Hi >= 1 && Wi >= 1 &&
                            (Co == 8 || Co == 16 || Co == 32) && Ci >= 1 && ldw >= 16 * Co && ldw % 8 == 0 && ldo >= Ci &&
                            aligned16(d) && aligned16(wT) && P < ((int64_t)1 << 31) &&
-                           4 * Hi * Wi * Co < ((int64_t)1 << 31),
+                           16 * Hi * Wi * Co < ((int64_t)1 << 31),
                   "sfh_nn_conv_t_dinput: Co 8, 16 or 32, ldw >= 16 Co (a multiple of 8), ldo >= Ci, "
-                  "16-byte aligned d and wT, B Hi Wi < 2^31");
+                  "16-byte aligned d and wT, B Hi Wi < 2^31, one gradient image under 2 GiB");
     if (P == 0) return SFH_OK;
-    const dim3 grid((unsigned)cdiv(P, 256), (unsigned)cdiv(Ci, 32));
     const double iw = 1.0 / (double)Wi, ih = 1.0 / (double)Hi;
-#define SFH_DIN(TT, CC)                                                                                           \
-    hipLaunchKernelGGL((k_conv_dinput<TT, CC>), grid, dim3(kWG), 0, ctx->stream, d, (int)B, (int)Hi, (int)Wi, iw, ih, \
-                       (const TT*)wT, ldw, Ci, out, ldo)
-    if (dtype == 0) {
-        if (Co == 8) SFH_DIN(uint16_t, 8); else if (Co == 16) SFH_DIN(uint16_t, 16); else SFH_DIN(uint16_t, 32);
-    } else {
-        if (Co == 8) SFH_DIN(float, 8); else if (Co == 16) SFH_DIN(float, 16); else SFH_DIN(float, 32);
-    }
+    // batch chunks whose gradient images span less than 2^31 bytes (the gathers'
+    // 32-bit buffer offsets); images are independent, so chunking is exact
+    const int64_t img_bytes = 16 * Hi * Wi * Co, nb = std::max<int64_t>(1, (((int64_t)1 << 31) - 1) / img_bytes);
+    for (int64_t b0 = 0; b0 < B; b0 += nb) {
+        const int64_t bn = std::min(nb, B - b0), Pc = bn * Hi * Wi;
+        const float* dc = d + b0 * (img_bytes / 4);
+        float* oc = out + b0 * Hi * Wi * ldo;
+        const uint32_t nbytes = (uint32_t)(bn * img_bytes);
+        const dim3 grid((unsigned)cdiv(Pc, 256), (unsigned)cdiv(Ci, 32));
+#define SFH_DIN(TT, CC)                                                                                               \
+    hipLaunchKernelGGL((k_conv_dinput<TT, CC>), grid, dim3(kWG), 0, ctx->stream, dc, nbytes, (int)bn, (int)Hi, (int)Wi, \
+                       iw, ih, (const TT*)wT, ldw, Ci, oc, ldo)
+        if (dtype == 0) {
+            if (Co == 8) SFH_DIN(uint16_t, 8); else if (Co == 16) SFH_DIN(uint16_t, 16); else SFH_DIN(uint16_t, 32);
+        } else {
+            if (Co == 8) SFH_DIN(float, 8); else if (Co == 16) SFH_DIN(float, 16); else SFH_DIN(float, 32);
+        }
 #undef SFH_DIN
-    return check_launch(ctx, "k_conv_dinput");
+        if (int rc = check_launch(ctx, "k_conv_dinput")) return rc;
+    }
+    return SFH_OK;
 }
 
 int sfh_nn_conv_t_fwd(sfh_ctx* ctx, int dtype, const void* in, int64_t B, int64_t Hi, int64_t Wi, int64_t C,
@@ -1961,18 +2019,30 @@ int sfh_nn_conv_t_fwd(sfh_ctx* ctx, int dtype, const void* in, int64_t B, int64_
     const int64_t P = B * Hi * Wi;
     SFH_ARG_CHECK(ctx, in && w && pre && (dtype == 0 || dtype == 1) && C == 32 && B >= 0 && Hi >= 1 && Wi >= 1 &&
                            aligned16(in) && aligned16(w) && P < ((int64_t)1 << 31) &&
-                           4 * P * C < ((int64_t)1 << 40),
-                  "sfh_nn_conv_t_fwd: 32 channels, 16-byte aligned in and w, B Hi Wi < 2^31");
+                           4 * Hi * Wi * C < ((int64_t)1 << 31),
+                  "sfh_nn_conv_t_fwd: 32 channels, 16-byte aligned in and w, B Hi Wi < 2^31, one input image "
+                  "under 2 GiB");
     if (P == 0) return SFH_OK;
-    const dim3 grid((unsigned)cdiv(P, 256), 1, 4);
-    if (dtype == 0)
-        hipLaunchKernelGGL(k_conv_fwd<uint16_t>, grid, dim3(kWG), 0, ctx->stream, (const uint16_t*)in, (int)B,
-                           (int)Hi, (int)Wi, 1.0 / (double)Wi, 1.0 / (double)Hi, (const uint16_t*)w, bias, pre,
-                           (uint16_t*)aout);
-    else
-        hipLaunchKernelGGL(k_conv_fwd<float>, grid, dim3(kWG), 0, ctx->stream, (const float*)in, (int)B, (int)Hi,
-                           (int)Wi, 1.0 / (double)Wi, 1.0 / (double)Hi, (const float*)w, bias, pre, (float*)aout);
-    return check_launch(ctx, "k_conv_fwd");
+    // batch chunks whose input images span less than 2^31 bytes (the gathers'
+    // 32-bit buffer offsets); images are independent, so chunking is exact
+    const int64_t esz = dtype == 0 ? 2 : 4, img = Hi * Wi * C;
+    const int64_t nb = std::max<int64_t>(1, (((int64_t)1 << 31) - 1) / (img * esz));
+    for (int64_t b0 = 0; b0 < B; b0 += nb) {
+        const int64_t bn = std::min(nb, B - b0), Pc = bn * Hi * Wi;
+        const uint32_t nbytes = (uint32_t)(bn * img * esz);
+        const int64_t oo = b0 * 4 * img;    // output image offset (2Hi x 2Wi x C)
+        const dim3 grid((unsigned)cdiv(Pc, 256), 1, 4);
+        if (dtype == 0)
+            hipLaunchKernelGGL(k_conv_fwd<uint16_t>, grid, dim3(kWG), 0, ctx->stream, (const uint16_t*)in + b0 * img,
+                               nbytes, (int)bn, (int)Hi, (int)Wi, 1.0 / (double)Wi, 1.0 / (double)Hi,
+                               (const uint16_t*)w, bias, pre + oo, aout ? (uint16_t*)aout + oo : nullptr);
+        else
+            hipLaunchKernelGGL(k_conv_fwd<float>, grid, dim3(kWG), 0, ctx->stream, (const float*)in + b0 * img, nbytes,
+                               (int)bn, (int)Hi, (int)Wi, 1.0 / (double)Wi, 1.0 / (double)Hi, (const float*)w, bias,
+                               pre + oo, aout ? (float*)aout + oo : nullptr);
+        if (int rc = check_launch(ctx, "k_conv_fwd")) return rc;
+    }
+    return SFH_OK;
 }
 
 int sfh_nn_conv_t_dweight(sfh_ctx* ctx, int dtype, const float* d, const float* pre, int act, int64_t B, int64_t Hi,
