@@ -470,20 +470,30 @@ __global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in,
     const int tilesx = (W + kC1TX - 1) / kC1TX, tilesy = (H + kC1TY - 1) / kC1TY;
     const int H2 = 2 * H, W2 = 2 * W;
     f4 raw[NPER];
+    // the tile's pixels (+ halo) through a buffer resource over its image:
+    // pixels outside read zeros with no branch around the loads, so these
+    // prefetches stay in flight while the current tile is computed
+    const uint32_t img_bytes = (uint32_t)(H * W * kC1Ci * 4);
     auto fetch = [&](int64_t t) {
         const int tx = (int)(t % tilesx), ty = (int)((t / tilesx) % tilesy);
         const int64_t b = t / ((int64_t)tilesx * tilesy);
         const int y0 = ty * kC1TY, x0 = tx * kC1TX;
+        const __amdgpu_buffer_rsrc_t rs = tensor_rsrc(in + b * H * (int64_t)W * kC1Ci, img_bytes);
 #pragma unroll
         for (int j = 0; j < NPER; ++j) {
             const int i = threadIdx.x + kWG * j;
             const int p = i / NQ, q = i % NQ;
             const int yy = y0 - 1 + p / kC1HX, xx = x0 - 1 + p % kC1HX;
-            raw[j] = (i < NP * NQ && yy >= 0 && yy < H && xx >= 0 && xx < W)
-                         ? reinterpret_cast<const f4*>(in + ((b * H + yy) * (int64_t)W + xx) * kC1Ci)[q]
-                         : f4{0.f, 0.f, 0.f, 0.f};
+            const bool ok = i < NP * NQ && yy >= 0 && yy < H && xx >= 0 && xx < W;
+            raw[j] = __builtin_bit_cast(f4, buf_load16(rs, ok ? (uint32_t)(((yy * W + xx) * kC1Ci + 4 * q) * 4) : kNoRow));
         }
     };
+    // the output pixel map's values for the tile's outputs, read before the
+    // next tile's prefetch is issued so that waiting for them does not wait
+    // for the prefetch (loads retire in order)
+    constexpr int NOUT = 4 * kC1TY * kC1TX / kWG;
+    static_assert(NOUT * kWG == 4 * kC1TY * kC1TX, "whole output passes");
+    const __amdgpu_buffer_rsrc_t rb = tensor_rsrc(bmap ? bmap : in, bmap ? (uint32_t)(4 * H * W * 4) : 0u);
     const int tap = threadIdx.x & 15, slice = threadIdx.x >> 4;
     float wr[kC1Ci];
 #pragma unroll
@@ -506,6 +516,14 @@ __global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in,
             if (i < NP * NQ) xs[i / NQ][i % NQ] = act4(raw[j], act);   // act(0) = 0 keeps the halo zero
         }
         __syncthreads();
+        float bm[NOUT];
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k) {
+            const int o = threadIdx.x + kWG * k;
+            const int Y = 2 * y0 + o / (2 * kC1TX), X = 2 * x0 + o % (2 * kC1TX);
+            bm[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                  rb, Y < H2 && X < W2 ? (uint32_t)((Y * W2 + X) * 4) : kNoRow, 0, 0));
+        }
         if (t + gridDim.x < ntiles) fetch(t + gridDim.x);
         for (int p = slice; p < NP; p += kWG / 16) {
             float s = 0.f;
@@ -520,10 +538,12 @@ __global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in,
             tv[p][tap] = s;
         }
         __syncthreads();
-        for (int o = threadIdx.x; o < 4 * kC1TY * kC1TX; o += kWG) {
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k) {
+            const int o = threadIdx.x + kWG * k;
             const int Y = 2 * y0 + o / (2 * kC1TX), X = 2 * x0 + o % (2 * kC1TX);
             if (Y >= H2 || X >= W2) continue;
-            float s = bmap ? bmap[Y * W2 + X] : 0.f;
+            float s = bm[k];    // zero without a map (an empty buffer reads zeros)
             const int ky0 = (Y + 1) & 1, kx0 = (X + 1) & 1;
 #pragma unroll
             for (int dy = 0; dy < 2; ++dy) {
@@ -1657,7 +1677,8 @@ static bool conv1_direct(int64_t B, int64_t H, int64_t W, int64_t Ci, const floa
                          const float* din = nullptr)
 {
     const auto al = [](const float* q) { return !q || ((uintptr_t)q & 15) == 0; };
-    return Ci == kC1Ci && al(in) && al(w) && al(din) && B >= 1 && 2 * H * 2 * W < ((int64_t)1 << 31) &&
+    // one input image (the forward's buffer loads: 32-bit byte offsets) under 2 GiB
+    return Ci == kC1Ci && al(in) && al(w) && al(din) && B >= 1 && H * W * kC1Ci * 4 < ((int64_t)1 << 31) &&
            B * H * W * kC1Ci < ((int64_t)1 << 40) && B * cdiv(H, kC1TY) * cdiv(W, kC1TX) < ((int64_t)1 << 31);
 }
 
