@@ -882,8 +882,10 @@ __global__ __launch_bounds__(kPrepWG) void k_dense_bwd_prep(const float* __restr
 // K-steps, except that the A operand is gathered from the FP32 gradient d as it is
 // loaded (a lane's 8 consecutive k are 8 consecutive channels of one output
 // pixel, zero outside the grid) and rounded to T, so the column matrix is
-// never written.  Rows run along grid.x, 32-column tiles along grid.y.
-template <typename T, int Co>
+// never written.  Rows run along grid.x, 32-column tiles along grid.y; with
+// S > 1 slices of the taps (K / S each) along grid.z, slice z writing its
+// partial products to out + z P ldo, summed afterwards in slice order.
+template <typename T, int Co, int S>
 __global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d, uint32_t d_bytes, int B, int Hi,
                                                      int Wi, double inv_w, double inv_h, const T* __restrict__ wT,
                                                      int64_t ldw, int64_t Ci, float* __restrict__ out, int64_t ldo)
@@ -895,7 +897,10 @@ __global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d
     const int64_t n0 = (int64_t)blockIdx.y * 32;
     const int r = lane & 31, h = lane >> 5;
     const int H2 = 2 * Hi, W2 = 2 * Wi;
-    constexpr int K = 16 * Co, NS = K / 16;
+    constexpr int K = 16 * Co / S, NS = K / 16;     // this slice's K and K-steps
+    static_assert(NS >= 1 && K * S == 16 * Co, "whole K-steps per slice");
+    const int kz = (int)blockIdx.z * K;
+    if (S > 1) out += (int64_t)blockIdx.z * P * ldo;
     const __amdgpu_buffer_rsrc_t rs = tensor_rsrc(d, d_bytes);
     uint32_t img[TM];    // element offset of the lane's image
     int y2[TM], x2[TM];
@@ -939,15 +944,15 @@ __global__ __launch_bounds__(kWG) void k_conv_dinput(const float* __restrict__ d
 #pragma unroll
     for (int s = 0; s < 2 && s < NS; ++s) {
 #pragma unroll
-        for (int a = 0; a < TM; ++a) load_a(a, 16 * s + 8 * h, fa[s][a]);
-        load_frag_raw(pb + 16 * s, fb[s]);
+        for (int a = 0; a < TM; ++a) load_a(a, kz + 16 * s + 8 * h, fa[s][a]);
+        load_frag_raw(pb + kz + 16 * s, fb[s]);
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         if (s + 2 < NS) {
 #pragma unroll
-            for (int a = 0; a < TM; ++a) load_a(a, 16 * (s + 2) + 8 * h, fa[(s + 2) % 3][a]);
-            load_frag_raw(pb + 16 * (s + 2), fb[(s + 2) % 3]);
+            for (int a = 0; a < TM; ++a) load_a(a, kz + 16 * (s + 2) + 8 * h, fa[(s + 2) % 3][a]);
+            load_frag_raw(pb + kz + 16 * (s + 2), fb[(s + 2) % 3]);
         }
 #pragma unroll
         for (int a = 0; a < TM; ++a) {
@@ -2018,10 +2023,28 @@ int sfh_nn_conv_t_dinput(sfh_ctx* ctx, int dtype, const float* d, int64_t B, int
         const float* dc = d + b0 * (img_bytes / 4);
         float* oc = out + b0 * Hi * Wi * ldo;
         const uint32_t nbytes = (uint32_t)(bn * img_bytes);
-        const dim3 grid((unsigned)cdiv(Pc, 256), (unsigned)cdiv(Ci, 32));
-#define SFH_DIN(TT, CC)                                                                                               \
-    hipLaunchKernelGGL((k_conv_dinput<TT, CC>), grid, dim3(kWG), 0, ctx->stream, dc, nbytes, (int)bn, (int)Hi, (int)Wi, \
-                       iw, ih, (const TT*)wT, ldw, Ci, oc, ldo)
+        // fewer workgroups than CUs: four slices of the taps, each a quarter of
+        // the K-steps one workgroup runs in sequence, summed in slice order
+        const int64_t wgs = cdiv(Pc, 256) * cdiv(Ci, 32);
+        const int S = wgs < 256 ? 4 : 1;
+        float* dst = oc;
+        int64_t ldd = ldo;
+        if (S > 1) {
+            void* p = nullptr;
+            if (int rc = scratch(ctx, "nn_dinput_slices", (size_t)S * Pc * Ci * sizeof(float), &p)) return rc;
+            dst = (float*)p;
+            ldd = Ci;
+        }
+        const dim3 grid((unsigned)cdiv(Pc, 256), (unsigned)cdiv(Ci, 32), (unsigned)S);
+#define SFH_DIN(TT, CC)                                                                                                \
+    do {                                                                                                               \
+        if (S > 1)                                                                                                     \
+            hipLaunchKernelGGL((k_conv_dinput<TT, CC, 4>), grid, dim3(kWG), 0, ctx->stream, dc, nbytes, (int)bn,        \
+                               (int)Hi, (int)Wi, iw, ih, (const TT*)wT, ldw, Ci, dst, ldd);                             \
+        else                                                                                                           \
+            hipLaunchKernelGGL((k_conv_dinput<TT, CC, 1>), grid, dim3(kWG), 0, ctx->stream, dc, nbytes, (int)bn,        \
+                               (int)Hi, (int)Wi, iw, ih, (const TT*)wT, ldw, Ci, dst, ldd);                             \
+    } while (0)
         if (dtype == 0) {
             if (Co == 8) SFH_DIN(uint16_t, 8); else if (Co == 16) SFH_DIN(uint16_t, 16); else SFH_DIN(uint16_t, 32);
         } else {
@@ -2029,6 +2052,8 @@ int sfh_nn_conv_t_dinput(sfh_ctx* ctx, int dtype, const float* d, int64_t B, int
         }
 #undef SFH_DIN
         if (int rc = check_launch(ctx, "k_conv_dinput")) return rc;
+        if (S > 1)
+            if (int rc = colsum(ctx, dst, Pc * Ci, S, Pc * Ci, 1.f, 0.f, oc, Ci, ldo, "nn_dinput_sum")) return rc;
     }
     return SFH_OK;
 }
