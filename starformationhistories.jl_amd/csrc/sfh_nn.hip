@@ -465,7 +465,7 @@ __global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in,
 {
     constexpr int NP = kC1HY * kC1HX, NQ = kC1Ci / 4;
     constexpr int NPER = (NP * NQ + kWG - 1) / kWG;
-    __shared__ f4 xs[NP][NQ];
+    __shared__ f4 xs[NP][NQ + 1];    // + 1: rows 16 B apart in the banks for the MFMA operand reads
     __shared__ float tv[NP][17];
     const int tilesx = (W + kC1TX - 1) / kC1TX, tilesy = (H + kC1TY - 1) / kC1TY;
     const int H2 = 2 * H, W2 = 2 * W;
@@ -494,15 +494,22 @@ __global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in,
     constexpr int NOUT = 4 * kC1TY * kC1TX / kWG;
     static_assert(NOUT * kWG == 4 * kC1TY * kC1TX, "whole output passes");
     const __amdgpu_buffer_rsrc_t rb = tensor_rsrc(bmap ? bmap : in, bmap ? (uint32_t)(4 * H * W * 4) : 0u);
-    const int tap = threadIdx.x & 15, slice = threadIdx.x >> 4;
-    float wr[kC1Ci];
+    // tap values tv[p][tap] = sum_c act(x[p][c]) w[tap][c] as 16 x 16 x 4 FP32
+    // MFMA products: rows = 16 pixels, columns = the 16 taps, k-step ks takes
+    // channel 8 g + ks from lane group g = lane / 16, so a lane reads its 8
+    // channels of a pixel as two 16-byte LDS loads and keeps its tap's 8
+    // weights in registers.
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int mt = lane & 15, mg = lane >> 4;
+    float wk[8];
+    {
+        const f4 w0 = *reinterpret_cast<const f4*>(w + mt * kC1Ci + 8 * mg);
+        const f4 w1 = *reinterpret_cast<const f4*>(w + mt * kC1Ci + 8 * mg + 4);
 #pragma unroll
-    for (int c = 0; c < kC1Ci; c += 4) {
-        const f4 v = *reinterpret_cast<const f4*>(w + tap * kC1Ci + c);
-        wr[c] = v[0];
-        wr[c + 1] = v[1];
-        wr[c + 2] = v[2];
-        wr[c + 3] = v[3];
+        for (int j = 0; j < 4; ++j) {
+            wk[j] = w0[j];
+            wk[4 + j] = w1[j];
+        }
     }
     if ((int64_t)blockIdx.x < ntiles) fetch(blockIdx.x);
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -525,17 +532,20 @@ __global__ __launch_bounds__(kWG) void k_conv1_fwd(const float* __restrict__ in,
                                                   rb, Y < H2 && X < W2 ? (uint32_t)((Y * W2 + X) * 4) : kNoRow, 0, 0));
         }
         if (t + gridDim.x < ntiles) fetch(t + gridDim.x);
-        for (int p = slice; p < NP; p += kWG / 16) {
-            float s = 0.f;
+        for (int rb = wave; rb < (NP + 15) / 16; rb += kWG / 64) {
+            const int p = rb * 16 + mt, pr = p < NP ? p : NP - 1;    // rows past NP: results unused
+            const f4 x0 = xs[pr][2 * mg], x1 = xs[pr][2 * mg + 1];
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const f4 v = xs[p][q];
-                s = fmaf(v[0], wr[4 * q], s);
-                s = fmaf(v[1], wr[4 * q + 1], s);
-                s = fmaf(v[2], wr[4 * q + 2], s);
-                s = fmaf(v[3], wr[4 * q + 3], s);
+            for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x0[j], wk[j], acc, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x1[j], wk[4 + j], acc, 0, 0, 0);
+            // lane holds column (tap) mt of rows 4 mg .. 4 mg + 3
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int pd = rb * 16 + 4 * mg + i;
+                if (pd < NP) tv[pd][mt] = acc[i];
             }
-            tv[p][tap] = s;
         }
         __syncthreads();
 #pragma unroll
