@@ -718,30 +718,44 @@ __global__ __launch_bounds__(kChanFinWG) void k_chan_final32(const float* __rest
 }
 
 // Partial weight gradients: part[wg][tap*32 + c] = sum over the workgroup's
-// pixels of window(p)[tap] act(in[p][c]).  Thread = (pixel lane, 4 channels): 64
-// pixel lanes x 8 channel quads, 16 x 4 accumulators; the pixel lanes of a
-// wave are summed with shuffles, then the waves through LDS, in a fixed order.
+// pixels of window(p)[tap] act(in[p][c]), as 16 x 16 x 4 FP32 MFMA products
+// over groups of 4 pixels: lane l takes tap (A rows) and channel (B columns)
+// l % 16 and pixel l / 16 of a group, two products per group for channels
+// 0-15 and 16-31; wave w takes the groups w, w + NWV, ... of the workgroup's
+// pixel range, U groups per step with the next step's loads in flight.  The
+// gradient window comes through a buffer resource over dout (dout_bytes <
+// 2^31): taps off the output grid and pixels past the range read zeros with
+// no branch; their activations are read from a clamped pixel and multiply
+// those zeros.  The waves' 16 x 32 tiles are added through LDS in wave order.
 constexpr int kC1DwWG = 512;
-__global__ __launch_bounds__(kC1DwWG) void k_conv1_dw(const float* __restrict__ dout, const float* __restrict__ in,
-                                                      int B, int H, int W, double inv_w, double inv_h,
-                                                      int64_t per_wg, int act, float* __restrict__ part)
+__global__ __launch_bounds__(kC1DwWG) void k_conv1_dw(const float* __restrict__ dout, uint32_t dout_bytes,
+                                                      const float* __restrict__ in, int B, int H, int W,
+                                                      double inv_w, double inv_h, int64_t per_wg, int act,
+                                                      float* __restrict__ part)
 {
-    constexpr int NWV = kC1DwWG / 64;
+    constexpr int NWV = kC1DwWG / 64, U = 8, STEP = 4 * NWV;    // pixels between a lane's groups
     __shared__ float red[NWV][16 * kC1Ci];
-    const int q = threadIdx.x & 7, pl = threadIdx.x >> 3, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int mt = lane & 15, mk = lane >> 4;
+    const int ky = mt >> 2, kx = mt & 3;
     const int64_t P = (int64_t)B * H * W;
     const int64_t p0 = (int64_t)blockIdx.x * per_wg;
     const int64_t p1 = p0 + per_wg < P ? p0 + per_wg : P;
-    float acc[16][4];
-#pragma unroll
-    for (int t = 0; t < 16; ++t) acc[t][0] = acc[t][1] = acc[t][2] = acc[t][3] = 0.f;
-    // pixel coordinates of the pixel being loaded, advanced by the step
-    // instead of divided each time; each pixel's input quad and gradient
-    // window are loaded one step ahead of the products that use them
-    constexpr int STEP = kC1DwWG / 8;
-    int64_t rr = udiv31(p0 + pl, W, inv_w), b = udiv31(rr, H, inv_h);
-    int x = (int)(p0 + pl - rr * W), y = (int)(rr - b * H);
-    auto advance = [&]() {
+    const int H2 = 2 * H, W2 = 2 * W;
+    const __amdgpu_buffer_rsrc_t rs = tensor_rsrc(dout, dout_bytes);
+    // the next group to load: pixel p and its coordinates, advanced by STEP
+    int64_t p = p0 + 4 * wave + mk;
+    int64_t rr = udiv31(p < P ? p : P - 1, W, inv_w), b = udiv31(rr, H, inv_h);
+    int x = (int)((p < P ? p : P - 1) - rr * W), y = (int)(rr - b * H);
+    auto load_group = [&](float& a, float& v0, float& v1) {
+        const int Y = 2 * y - 1 + ky, X = 2 * x - 1 + kx;
+        const bool ok = p < p1 && Y >= 0 && Y < H2 && X >= 0 && X < W2;
+        a = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                          rs, ok ? (uint32_t)(((b * H2 + Y) * W2 + X) * 4) : kNoRow, 0, 0));
+        const int64_t pc = p < p1 ? p : p1 - 1;
+        v0 = in[pc * kC1Ci + mt];
+        v1 = in[pc * kC1Ci + 16 + mt];
+        p += STEP;
         x += STEP;
         while (x >= W) {
             x -= W;
@@ -751,49 +765,41 @@ __global__ __launch_bounds__(kC1DwWG) void k_conv1_dw(const float* __restrict__ 
             }
         }
     };
-    f4 a = {0.f, 0.f, 0.f, 0.f};
-    float d[16];
-    if (p0 + pl < p1) {
-        a = *reinterpret_cast<const f4*>(in + (p0 + pl) * kC1Ci + 4 * q);
-        conv1_window(dout, b, y, x, 2 * H, 2 * W, d);
-    }
-    for (int64_t p = p0 + pl; p < p1; p += STEP) {
-        f4 an = {0.f, 0.f, 0.f, 0.f};
-        float dn[16];
-        const bool more = p + STEP < p1;
+    const int64_t ngroups = (p1 - p0 + 3) / 4;                  // groups of 4 pixels in the range
+    const int64_t nmine = ngroups > wave ? (ngroups - wave + NWV - 1) / NWV : 0;
+    f4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    float ca[U], c0[U], c1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) load_group(ca[u], c0[u], c1[u]);
+    for (int64_t g = 0; g < nmine; g += U) {
+        float na[U], n0[U], n1[U];
+        const bool more = g + U < nmine;
         if (more) {
-            advance();
-            an = *reinterpret_cast<const f4*>(in + (p + STEP) * kC1Ci + 4 * q);
-            conv1_window(dout, b, y, x, 2 * H, 2 * W, dn);
-        }
-        const f4 av = act4(a, act);
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            acc[t][0] = fmaf(d[t], av[0], acc[t][0]);
-            acc[t][1] = fmaf(d[t], av[1], acc[t][1]);
-            acc[t][2] = fmaf(d[t], av[2], acc[t][2]);
-            acc[t][3] = fmaf(d[t], av[3], acc[t][3]);
+            for (int u = 0; u < U; ++u) load_group(na[u], n0[u], n1[u]);
         }
-        a = an;
 #pragma unroll
-        for (int t = 0; t < 16; ++t) d[t] = more ? dn[t] : 0.f;
+        for (int u = 0; u < U; ++u) {
+            // groups past this wave's last read zeros through the window (p >= p1)
+            const float s0 = act == 1 ? c0[u] * sigmoid(c0[u]) : c0[u];
+            const float s1 = act == 1 ? c1[u] * sigmoid(c1[u]) : c1[u];
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[u], s0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[u], s1, acc1, 0, 0, 0);
+        }
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ca[u] = na[u];
+                c0[u] = n0[u];
+                c1[u] = n1[u];
+            }
+        }
     }
-    // sum the 8 pixel lanes of this wave (lane bits 3..5)
+    // lane holds taps 4 mk .. 4 mk + 3 of channels mt and 16 + mt
 #pragma unroll
-    for (int t = 0; t < 16; ++t)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            float v = acc[t][j];
-            v += __shfl_xor(v, 8, 64);
-            v += __shfl_xor(v, 16, 64);
-            v += __shfl_xor(v, 32, 64);
-            acc[t][j] = v;
-        }
-    if (lane < 8) {
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) red[wave][t * kC1Ci + 4 * q + j] = acc[t][j];
+    for (int r = 0; r < 4; ++r) {
+        red[wave][(4 * mk + r) * kC1Ci + mt] = acc0[r];
+        red[wave][(4 * mk + r) * kC1Ci + 16 + mt] = acc1[r];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 16 * kC1Ci; i += kC1DwWG) {
@@ -1692,8 +1698,10 @@ static bool conv1_direct(int64_t B, int64_t H, int64_t W, int64_t Ci, const floa
                          const float* din = nullptr)
 {
     const auto al = [](const float* q) { return !q || ((uintptr_t)q & 15) == 0; };
-    // one input image (the forward's buffer loads: 32-bit byte offsets) under 2 GiB
+    // one input image (the forward's buffer loads: 32-bit byte offsets) and the
+    // whole output gradient (the weight gradient's) under 2 GiB
     return Ci == kC1Ci && al(in) && al(w) && al(din) && B >= 1 && H * W * kC1Ci * 4 < ((int64_t)1 << 31) &&
+           B * H * W * 16 < ((int64_t)1 << 31) &&
            B * H * W * kC1Ci < ((int64_t)1 << 40) && B * cdiv(H, kC1TY) * cdiv(W, kC1TX) < ((int64_t)1 << 31);
 }
 
@@ -1960,8 +1968,9 @@ int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B
             const int64_t per = cdiv(P, nwg);
             nwg = cdiv(P, per);
             if (int rc = scratch(ctx, "nn_conv1_dw", (size_t)nwg * 16 * kC1Ci * sizeof(float), &p)) return rc;
-            hipLaunchKernelGGL(k_conv1_dw, dim3((unsigned)nwg), dim3(kC1DwWG), 0, ctx->stream, dout, in, (int)B,
-                               (int)H, (int)W, 1.0 / (double)W, 1.0 / (double)H, per, act, (float*)p);
+            hipLaunchKernelGGL(k_conv1_dw, dim3((unsigned)nwg), dim3(kC1DwWG), 0, ctx->stream, dout,
+                               (uint32_t)(P * 16), in, (int)B, (int)H, (int)W, 1.0 / (double)W, 1.0 / (double)H, per,
+                               act, (float*)p);
             if (int rc = check_launch(ctx, "k_conv1_dw")) return rc;
             if (int rc = colsum(ctx, (const float*)p, 16 * kC1Ci, nwg, 16 * kC1Ci, 1.f, 0.f, dw, 16 * kC1Ci,
                                 16 * kC1Ci, "nn_conv1_dw_sum"))
