@@ -618,52 +618,95 @@ __device__ inline void chan_part32(f4 s, f4* red, float* __restrict__ part)
 }
 
 // din[p][c] = sum_tap window(p)[tap] w[tap][c], times act'(in[p][c]) with
-// act = SiLU (din is then the gradient of the pre-activation in); thread =
-// (pixel, 4 channels), grid-stride over the pixels.  With part non-null, also
-// the workgroup's channel sums of din (chan_part32).
-__global__ __launch_bounds__(kWG) void k_conv1_din(const float* __restrict__ dout, const float* __restrict__ in,
-                                                   int B, int H, int W, double inv_w, double inv_h,
-                                                   const float* __restrict__ w, int act, float* __restrict__ din,
-                                                   float* __restrict__ part)
+// act = SiLU (din is then the gradient of the pre-activation in), as 16 x 16 x 4
+// FP32 MFMA products: blocks of 16 pixels (rows) by the 32 channels (two
+// column blocks), k-step ks taking taps 4 ks .. 4 ks + 3 (lane l: pixel l % 16,
+// tap 4 ks + l / 16, i.e. kernel row ks and column l / 16; its weights held in
+// registers); the window through a buffer resource over dout (dout_bytes <
+// 2^31), so off-grid taps and pixels past P read zeros with no branch.  Waves
+// walk the blocks grid-stride.  With part non-null, also the workgroup's
+// channel sums of din: per lane over its rows, then the four lane groups with
+// shuffles and the waves through LDS, in a fixed order, into part[wg][c].
+__global__ __launch_bounds__(kWG) void k_conv1_din(const float* __restrict__ dout, uint32_t dout_bytes,
+                                                   const float* __restrict__ in, int B, int H, int W, double inv_w,
+                                                   double inv_h, const float* __restrict__ w, int act,
+                                                   float* __restrict__ din, float* __restrict__ part)
 {
-    __shared__ f4 sw[16][kC1Ci / 4];
-    __shared__ f4 red[kWG];
-    if (threadIdx.x < 16 * kC1Ci / 4) sw[threadIdx.x / (kC1Ci / 4)][threadIdx.x % (kC1Ci / 4)] =
-        reinterpret_cast<const f4*>(w)[threadIdx.x];
-    __syncthreads();
-    const int64_t P = (int64_t)B * H * W, nq = P * (kC1Ci / 4);
-    const int q = (int)(threadIdx.x % (kC1Ci / 4));
-    f4 sum = {0.f, 0.f, 0.f, 0.f};
-    for (int64_t g = (int64_t)blockIdx.x * kWG + threadIdx.x; g < nq; g += (int64_t)gridDim.x * kWG) {
-        // keeps the weight reads from LDS inside the loop: hoisted, the 16
-        // quads would hold 64 registers and halve the waves in flight
-        asm volatile("" ::: "memory");
-        const int64_t p = g / (kC1Ci / 4);
-        const int64_t r = udiv31(p, W, inv_w), b = udiv31(r, H, inv_h);
-        const int x = (int)(p - r * W), y = (int)(r - b * H);
-        float d[16];
-        conv1_window(dout, b, y, x, 2 * H, 2 * W, d);
-        f4 acc = {0.f, 0.f, 0.f, 0.f};
+    constexpr int NWV = kWG / 64;
+    __shared__ float red[NWV][kC1Ci];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int mt = lane & 15, mk = lane >> 4;
+    const int64_t P = (int64_t)B * H * W, nblk = (P + 15) / 16;
+    const int H2 = 2 * H, W2 = 2 * W;
+    const __amdgpu_buffer_rsrc_t rs = tensor_rsrc(dout, dout_bytes);
+    float wb0[4], wb1[4];
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            const f4 wv = sw[t][q];
-            acc[0] = fmaf(d[t], wv[0], acc[0]);
-            acc[1] = fmaf(d[t], wv[1], acc[1]);
-            acc[2] = fmaf(d[t], wv[2], acc[2]);
-            acc[3] = fmaf(d[t], wv[3], acc[3]);
+    for (int ks = 0; ks < 4; ++ks) {
+        wb0[ks] = w[(4 * ks + mk) * kC1Ci + mt];
+        wb1[ks] = w[(4 * ks + mk) * kC1Ci + 16 + mt];
+    }
+    float cs0 = 0.f, cs1 = 0.f;
+    for (int64_t bi = (int64_t)blockIdx.x * NWV + wave; bi < nblk; bi += (int64_t)gridDim.x * NWV) {
+        const int64_t pa = bi * 16 + mt, pq = pa < P ? pa : P - 1;
+        const int64_t rr = udiv31(pq, W, inv_w), b = udiv31(rr, H, inv_h);
+        const int x = (int)(pq - rr * W), y = (int)(rr - b * H);
+        const int X = 2 * x - 1 + mk;
+        float a[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const int Y = 2 * y - 1 + ks;
+            const bool ok = pa < P && Y >= 0 && Y < H2 && X >= 0 && X < W2;
+            a[ks] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                  rs, ok ? (uint32_t)(((b * H2 + Y) * W2 + X) * 4) : kNoRow, 0, 0));
         }
-        if (act == 1) {
-            const f4 xv = *reinterpret_cast<const f4*>(in + p * kC1Ci + 4 * q);
+        // activations of the rows this lane's results belong to: pixels 4 mk + r
+        float x0[4], x1[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float sg = sigmoid(xv[j]);
-                acc[j] *= sg * (1.f + xv[j] * (1.f - sg));
+        for (int r = 0; r < 4; ++r) {
+            const int64_t pd = bi * 16 + 4 * mk + r, pc = pd < P ? pd : P - 1;
+            x0[r] = act == 1 ? in[pc * kC1Ci + mt] : 0.f;
+            x1[r] = act == 1 ? in[pc * kC1Ci + 16 + mt] : 0.f;
+        }
+        f4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], wb0[ks], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], wb1[ks], acc1, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t pd = bi * 16 + 4 * mk + r;
+            float v0 = acc0[r], v1 = acc1[r];
+            if (act == 1) {
+                const float s0 = sigmoid(x0[r]), s1 = sigmoid(x1[r]);
+                v0 *= s0 * (1.f + x0[r] * (1.f - s0));
+                v1 *= s1 * (1.f + x1[r] * (1.f - s1));
+            }
+            if (pd < P) {
+                din[pd * kC1Ci + mt] = v0;
+                din[pd * kC1Ci + 16 + mt] = v1;
+                cs0 += v0;
+                cs1 += v1;
             }
         }
-        *reinterpret_cast<f4*>(din + p * kC1Ci + 4 * q) = acc;
-        sum += acc;
     }
-    if (part) chan_part32(sum, red, part);
+    if (part) {
+        cs0 += __shfl_xor(cs0, 16, 64);
+        cs0 += __shfl_xor(cs0, 32, 64);
+        cs1 += __shfl_xor(cs1, 16, 64);
+        cs1 += __shfl_xor(cs1, 32, 64);
+        if (mk == 0) {
+            red[wave][mt] = cs0;
+            red[wave][16 + mt] = cs1;
+        }
+        __syncthreads();
+        if (threadIdx.x < kC1Ci) {
+            float t = red[0][threadIdx.x];
+#pragma unroll
+            for (int k = 1; k < NWV; ++k) t += red[k][threadIdx.x];
+            part[(int64_t)blockIdx.x * kC1Ci + threadIdx.x] = t;
+        }
+    }
 }
 
 // dX = dY act'(pre) over P pixels of 32 channels (dX may alias dY), with the
@@ -1953,8 +1996,9 @@ int sfh_nn_conv1_bwd(sfh_ctx* ctx, const float* dout, const float* in, int64_t B
             void* cp = nullptr;
             if (dsum)
                 if (int rc = scratch(ctx, "nn_chan_part", (size_t)nwg * 32 * sizeof(float), &cp)) return rc;
-            hipLaunchKernelGGL(k_conv1_din, dim3((unsigned)nwg), dim3(kWG), 0, ctx->stream, dout, in, (int)B, (int)H,
-                               (int)W, 1.0 / (double)W, 1.0 / (double)H, w, act, din, (float*)cp);
+            hipLaunchKernelGGL(k_conv1_din, dim3((unsigned)nwg), dim3(kWG), 0, ctx->stream, dout, (uint32_t)(P * 16),
+                               in, (int)B, (int)H, (int)W, 1.0 / (double)W, 1.0 / (double)H, w, act, din,
+                               (float*)cp);
             if (int rc = check_launch(ctx, "k_conv1_din")) return rc;
             if (dsum) {
                 hipLaunchKernelGGL(k_chan_final32, dim3(1), dim3(kChanFinWG), 0, ctx->stream, (const float*)cp, nwg,
