@@ -465,8 +465,8 @@ int sfh_nn_conv_t_fwd(sfh_ctx* ctx, int dtype, const void* in, int64_t B, int64_
  * channels: dK (16 x 32 rows of 32, row tap 32 + co, FP32) = sum over input
  * pixels p of im2col(d)[p][tap 32 + co] act(pre[p][ci]) (act 0: identity, 1:
  * SiLU), d the FP32 output gradient (B x 2Hi x 2Wi x 32), pre the FP32 layer
- * input before its activation (B Hi Wi x 32); operands rounded to dtype, FP32
- * accumulation, fixed-order sums over pixel splits. */
+ * input before its activation (B Hi Wi x 32; d under 2 GiB); operands rounded
+ * to dtype, FP32 accumulation, fixed-order sums over pixel splits. */
 int sfh_nn_conv_t_dweight(sfh_ctx* ctx, int dtype, const float* d, const float* pre, int act, int64_t B, int64_t Hi,
                           int64_t Wi, int64_t C, float* dK);
 /* The same transposed convolution with one output channel (B x 2H x 2W) and a

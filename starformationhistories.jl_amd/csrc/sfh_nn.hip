@@ -1158,9 +1158,10 @@ __global__ __launch_bounds__(kWG) void k_conv_fwd(const T* __restrict__ in, uint
 // from the tensors directly: no transposed column matrix, no transposed input.
 constexpr int kDwC = 32, kDwChunk = 64;
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_conv_dweight(const float* __restrict__ d, const float* __restrict__ pre,
-                                                      int act, int B, int Hi, int Wi, double inv_w, double inv_h,
-                                                      int64_t per, float* __restrict__ part)
+__global__ __launch_bounds__(kWG) void k_conv_dweight(const float* __restrict__ d, uint32_t d_bytes,
+                                                      const float* __restrict__ pre, int act, int B, int Hi, int Wi,
+                                                      double inv_w, double inv_h, int64_t per,
+                                                      float* __restrict__ part)
 {
     constexpr int PADL = 16 / sizeof(T);            // 16 bytes of padding per row
     constexpr int LD = kDwChunk + PADL;
@@ -1176,7 +1177,13 @@ __global__ __launch_bounds__(kWG) void k_conv_dweight(const float* __restrict__ 
     // staging role of this thread in every chunk: channel quad q of pixel
     // slots pq and pq + 32, for the activations and the four taps' windows
     const int q = threadIdx.x & 7, pq = threadIdx.x >> 3;
+    // the next chunk's raw values, loaded without branches (activations from
+    // a clamped pixel, masked when staged; gradient windows through a buffer
+    // resource over d, zeros off the grid) and activated only when staged, so
+    // nothing waits for them before the current chunk's products
+    const __amdgpu_buffer_rsrc_t rs = tensor_rsrc(d, d_bytes);
     f4 va[2], vd[2][4];
+    bool vok[2];
     auto fetch = [&](int64_t c0) {
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -1185,16 +1192,15 @@ __global__ __launch_bounds__(kWG) void k_conv_dweight(const float* __restrict__ 
             const int64_t pp = ok ? p : p0;
             const int64_t rr = udiv31(pp, Wi, inv_w), b = udiv31(rr, Hi, inv_h);
             const int x2 = 2 * (int)(pp - rr * Wi) - 1, y2 = 2 * (int)(rr - b * Hi) - 1;
-            const float* img = d + b * H2 * (int64_t)W2 * kDwC;
-            va[s] = ok ? act4(*reinterpret_cast<const f4*>(pre + pp * kDwC + 4 * q), act) : f4{0.f, 0.f, 0.f, 0.f};
+            vok[s] = ok;
+            va[s] = *reinterpret_cast<const f4*>(pre + pp * kDwC + 4 * q);
             const int Y = y2 + ky;
             const bool yok = ok && Y >= 0 && Y < H2;
 #pragma unroll
             for (int kx = 0; kx < 4; ++kx) {
                 const int X = x2 + kx;
-                vd[s][kx] = (yok && X >= 0 && X < W2)
-                                ? *reinterpret_cast<const f4*>(img + ((int64_t)Y * W2 + X) * kDwC + 4 * q)
-                                : f4{0.f, 0.f, 0.f, 0.f};
+                const uint32_t off = (uint32_t)((((b * H2 + Y) * W2 + X) * kDwC + 4 * q) * 4);
+                vd[s][kx] = __builtin_bit_cast(f4, buf_load16(rs, yok && X >= 0 && X < W2 ? off : kNoRow));
             }
         }
     };
@@ -1207,9 +1213,10 @@ __global__ __launch_bounds__(kWG) void k_conv_dweight(const float* __restrict__ 
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const int pl = pq + 32 * s;
+            const f4 av = vok[s] ? act4(va[s], act) : f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                sA[4 * q + j][pl] = from_f32<T>(va[s][j]);
+                sA[4 * q + j][pl] = from_f32<T>(av[j]);
 #pragma unroll
                 for (int kx = 0; kx < 4; ++kx) sD[kx][4 * q + j][pl] = from_f32<T>(vd[s][kx][j]);
             }
@@ -2161,8 +2168,9 @@ int sfh_nn_conv_t_dweight(sfh_ctx* ctx, int dtype, const float* d, const float* 
     const int64_t P = B * Hi * Wi;
     SFH_ARG_CHECK(ctx, d && pre && dK && (dtype == 0 || dtype == 1) && (act == 0 || act == 1) && C == kDwC &&
                            B >= 0 && Hi >= 1 && Wi >= 1 && aligned16(d) && aligned16(pre) &&
-                           P < ((int64_t)1 << 31) && 4 * Hi * Wi * C < ((int64_t)1 << 31),
-                  "sfh_nn_conv_t_dweight: 32 channels, act 0 or 1, 16-byte aligned d and pre, B Hi Wi < 2^31");
+                           P < ((int64_t)1 << 31) && 16 * P * C < ((int64_t)1 << 31),
+                  "sfh_nn_conv_t_dweight: 32 channels, act 0 or 1, 16-byte aligned d and pre, the gradient d "
+                  "(B x 2Hi x 2Wi x 32 FP32) under 2 GiB");
     constexpr int64_t NOUT = 16 * kDwC * kDwC;
     if (P == 0) {
         hipLaunchKernelGGL(k_fill, dim3(grid_for(NOUT)), dim3(kWG), 0, ctx->stream, dK, NOUT, 0.f);
@@ -2176,11 +2184,11 @@ int sfh_nn_conv_t_dweight(sfh_ctx* ctx, int dtype, const float* d, const float* 
     if (int rc = scratch(ctx, "nn_conv_dw", (size_t)nsplit * NOUT * sizeof(float), &p)) return rc;
     const dim3 grid(4, (unsigned)nsplit);
     if (dtype == 0)
-        hipLaunchKernelGGL(k_conv_dweight<uint16_t>, grid, dim3(kWG), 0, ctx->stream, d, pre, act, (int)B, (int)Hi,
-                           (int)Wi, 1.0 / (double)Wi, 1.0 / (double)Hi, per, (float*)p);
+        hipLaunchKernelGGL(k_conv_dweight<uint16_t>, grid, dim3(kWG), 0, ctx->stream, d, (uint32_t)(16 * P * C), pre,
+                           act, (int)B, (int)Hi, (int)Wi, 1.0 / (double)Wi, 1.0 / (double)Hi, per, (float*)p);
     else
-        hipLaunchKernelGGL(k_conv_dweight<float>, grid, dim3(kWG), 0, ctx->stream, d, pre, act, (int)B, (int)Hi,
-                           (int)Wi, 1.0 / (double)Wi, 1.0 / (double)Hi, per, (float*)p);
+        hipLaunchKernelGGL(k_conv_dweight<float>, grid, dim3(kWG), 0, ctx->stream, d, (uint32_t)(16 * P * C), pre, act,
+                           (int)B, (int)Hi, (int)Wi, 1.0 / (double)Wi, 1.0 / (double)Hi, per, (float*)p);
     if (int rc = check_launch(ctx, "k_conv_dweight")) return rc;
     return colsum(ctx, (const float*)p, NOUT, nsplit, NOUT, 1.f, 0.f, dK, NOUT, NOUT, "nn_conv_dw_sum");
 }
