@@ -1384,9 +1384,15 @@ __global__ __launch_bounds__(kWG) void k_tl_sums(const float* __restrict__ out, 
     const int64_t tb = rows ? rows[b] : b;
     const int64_t c0 = n * blockIdx.x / kTLChunks, c1 = n * (blockIdx.x + 1) / kTLChunks;
     double ss = 0.0, se = 0.0, a = 0.0, q = 0.0;
+    // (Y, X) of element i, stepped with i instead of divided per element
+    int64_t Y = (c0 + threadIdx.x) / S, X = (c0 + threadIdx.x) % S;
     for (int64_t i = c0 + threadIdx.x; i < c1; i += kWG) {
-        const int64_t Y = i / S, X = i % S;
         const float y = out[(b * H2 + Y) * W2 + X], t = target[tb * n + i];
+        X += kWG;
+        while (X >= S) {
+            X -= S;
+            ++Y;
+        }
         const float sv = expf(fminf(y, 15.f) * sc + off), ev = expf(t * sc + off);
         ss += sv;
         se += ev;
@@ -1429,8 +1435,11 @@ __global__ __launch_bounds__(kWG) void k_tl_grad(const float* __restrict__ out, 
     const float inv_ss = (float)(1.0 / ssd), inv_se = (float)(1.0 / sed);
     const float gm = w_mse * 2.f / (float)(B * R * S), gk = w_kl * sc / (float)B;
     const float gs = (float)(2.0 * w_sum * (log(ssd) - log(sed))) * sc / (float)B;
-    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n2; i += (int64_t)gridDim.x * kWG) {
-        const int64_t Y = i / W2, X = i % W2;
+    // (Y, X) of element i, stepped with i instead of divided per element
+    const int64_t i0 = (int64_t)blockIdx.x * kWG + threadIdx.x, step = (int64_t)gridDim.x * kWG;
+    const int64_t dy = step / W2, dx = step % W2;
+    int64_t Y = i0 / W2, X = i0 % W2;
+    for (int64_t i = i0; i < n2; i += step) {
         float d = 0.f;
         if (Y < R && X < S) {
             const float y = out[b * n2 + i], t = target[tb * R * S + Y * S + X];
@@ -1438,6 +1447,12 @@ __global__ __launch_bounds__(kWG) void k_tl_grad(const float* __restrict__ out, 
             d = gm * (y - t) + gk * (ps - pe) + gs * ps;
         }
         dout[b * n2 + i] = d;
+        X += dx;
+        Y += dy;
+        if (X >= W2) {
+            X -= W2;
+            ++Y;
+        }
     }
 }
 
