@@ -2,19 +2,23 @@
 // network from (logAge, [M/H], distance modulus, error scale) to a
 // log-template, trained on templates from the deposition kernel.
 //
-// Every matrix product is C = alpha A B^T (+ beta C) with A (M x K) and B
-// (N x K) row-major and K contiguous ("NT"), on the matrix cores:
-// v_mfma_f32_32x32x16_bf16 for BF16 operands, v_mfma_f32_32x32x2f32 for FP32
-// operands (used to check gradients), FP32 accumulation either way.  Callers
-// keep K a multiple of 16 with zero padding.  The stride-2, 4x4 transposed
-// convolutions of the decoder become such products plus gathers:
+// Every matrix product runs on the matrix cores: v_mfma_f32_32x32x16_bf16 for
+// BF16 operands, v_mfma_f32_32x32x2f32 for FP32 operands (used to check
+// gradients), FP32 accumulation either way.  Plain products are C = alpha A B^T
+// (+ beta C) with A (M x K) and B (N x K) row-major and K contiguous ("NT"),
+// K a multiple of 16 with zero padding.  The stride-2, 4x4 transposed
+// convolutions of the decoder are, in matrix form,
 //   forward   cols[p, (ky,kx,co)] = in[p, :] . W[(ky,kx,co), :]    (GEMM)
 //             out[b,Y,X,co] = sum of the 4 taps (ky,kx) that land on (Y,X) (col2im)
 //   backward  dcols[p, (ky,kx,co)] = dout[b, 2y-1+ky, 2x-1+kx, co]    (im2col)
 //             din = dcols W,  dW = dcols^T in                          (GEMMs)
-// Activations are channels-last (pixel-major, channel fastest) and weight
-// rows tap-major ((ky*4 + kx)*Co + co), so both gathers move runs of Co
-// consecutive floats.
+// and that composition is kept as entry points; the training path runs
+// each line as one product whose operands are gathered as they are loaded
+// (k_conv_fwd by output parity class, k_conv_dinput, k_conv_dweight), so no
+// column matrix is written.  The single-channel output layer works on the
+// pixel grid with 16 x 16 x 4 FP32 MFMA tap sums (k_conv1_*).  Activations are
+// channels-last (pixel-major, channel fastest) and weight rows tap-major
+// ((ky*4 + kx)*Co + co), so the gathers move runs of Co consecutive floats.
 #include "sfh_internal.h"
 
 #include <cmath>
