@@ -260,6 +260,33 @@ __global__ __launch_bounds__(kWG) void k_bias_act(const float* __restrict__ X, i
     }
 }
 
+// Y = act(x) with x = (the sum of a split product's nparts slices) + bias[c]
+// over rows x cols (slice p of element e at part[p rows cols + e]); the
+// slices are added as k_colsum_final adds them (four interleaved groups,
+// ((s0 + s1) + s2) + s3), so the result equals that sum followed by
+// k_bias_act bit for bit.  pre (may be null) keeps x; Y (ldy >= cols) gets
+// zeros in cols..ldy-1.
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_splitk_bias_act(const float* __restrict__ part, int64_t nparts, int64_t rows,
+                                                         int64_t cols, const float* __restrict__ bias, int act,
+                                                         float* __restrict__ pre, T* __restrict__ Y, int64_t ldy)
+{
+    const int64_t n = rows * ldy, rc = rows * cols;
+    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) {
+        const int64_t rr = i / ldy, c = i - rr * ldy;
+        float y = 0.f;
+        if (c < cols) {
+            const int64_t e = rr * cols + c;
+            float g[4] = {0.f, 0.f, 0.f, 0.f};
+            for (int64_t p = 0; p < nparts; ++p) g[p & 3] += part[p * rc + e];
+            const float x = 1.f * (((g[0] + g[1]) + g[2]) + g[3]) + (bias ? bias[c] : 0.f);
+            if (pre) pre[e] = x;
+            y = act == 1 ? x * sigmoid(x) : x;
+        }
+        Y[i] = from_f32<T>(y);
+    }
+}
+
 // dX = dY * act'(pre), elementwise over n (in place allowed)
 __global__ __launch_bounds__(kWG) void k_act_bwd(const float* dY, const float* __restrict__ pre, int64_t n, int act,
                                                  float* dX)
@@ -1686,8 +1713,12 @@ int colsum(sfh_ctx* ctx, const float* X, int64_t ld, int64_t rows, int64_t cols,
 // order.  A workgroup's K-steps run one after another, so a few tiles with a
 // long K would leave the launch bound by the latency of its loads.
 int gemm(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* B, int64_t ldb, float* C, int64_t ldc,
-         int64_t M, int64_t N, int64_t K, float alpha, float beta)
+         int64_t M, int64_t N, int64_t K, float alpha, float beta, float** slices = nullptr,
+         int64_t* nslices = nullptr)
 {
+    // with slices: a split product's raw slices (splits x M N) are returned
+    // unsummed for the caller to add (alpha 1, beta 0 then), else C is written
+    if (nslices) *nslices = 1;
     if (M == 0 || N == 0) return SFH_OK;
     const bool narrow = N <= 32;
     const int64_t tm = narrow ? 256 : 128, tn = narrow ? 32 : 128;
@@ -1732,6 +1763,11 @@ int gemm(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* B, int
     }
     if (int rc = check_launch(ctx, "k_gemm_nt")) return rc;
     // the slices are the rows of a splits x (M N) matrix
+    if (splits > 1 && slices) {
+        *slices = out;
+        *nslices = splits;
+        return SFH_OK;
+    }
     if (splits > 1) return colsum(ctx, out, M * N, splits, M * N, alpha, beta, C, N, ldc, "nn_splitk_part");
     return SFH_OK;
 }
@@ -1799,6 +1835,37 @@ int sfh_nn_bias_act(sfh_ctx* ctx, int dtype, const float* X, int64_t ldx, const 
     else
         hipLaunchKernelGGL(k_bias_act<float>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, X, ldx, bias, rows, cols,
                            act, pre, (float*)Y, ldy);
+    return check_launch(ctx, "k_bias_act");
+}
+
+int sfh_nn_linear(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* W, int64_t ldw, int64_t M,
+                  int64_t N, int64_t K, const float* bias, int act, float* z, float* pre, void* Y, int64_t ldy)
+{
+    SFH_NN_BEGIN(ctx);
+    SFH_ARG_CHECK(ctx, A && W && z && Y && (dtype == 0 || dtype == 1) && (act == 0 || act == 1) && M >= 0 && N >= 0 &&
+                           K >= 0 && K % 16 == 0 && lda >= K && ldw >= K && lda % 8 == 0 && ldw % 8 == 0 && ldy >= N,
+                  "sfh_nn_linear: dtype 0 or 1, act 0 or 1, K a multiple of 16, lda and ldw >= K (multiples of 8), "
+                  "ldy >= N");
+    if (M == 0) return SFH_OK;
+    float* slices = nullptr;
+    int64_t ns = 1;
+    if (int rc = gemm(ctx, dtype, A, lda, W, ldw, z, N, M, N, K, 1.f, 0.f, &slices, &ns)) return rc;
+    const int64_t n = M * ldy;
+    if (ns > 1) {
+        if (dtype == 0)
+            hipLaunchKernelGGL(k_splitk_bias_act<uint16_t>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream,
+                               (const float*)slices, ns, M, N, bias, act, pre, (uint16_t*)Y, ldy);
+        else
+            hipLaunchKernelGGL(k_splitk_bias_act<float>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream,
+                               (const float*)slices, ns, M, N, bias, act, pre, (float*)Y, ldy);
+        return check_launch(ctx, "k_splitk_bias_act");
+    }
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_bias_act<uint16_t>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, (const float*)z, N, bias,
+                           M, N, act, pre, (uint16_t*)Y, ldy);
+    else
+        hipLaunchKernelGGL(k_bias_act<float>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, (const float*)z, N, bias, M,
+                           N, act, pre, (float*)Y, ldy);
     return check_launch(ctx, "k_bias_act");
 }
 

@@ -166,6 +166,11 @@ class TemplateSurrogate:
         self._c(self.lib.sfh_nn_gemm_nt(self.ctx.handle, self.dt, A.data_ptr(), lda, Bm.data_ptr(), ldb,
                                         Cm.data_ptr(), ldc, M, N, K, 1.0, beta))
 
+    def _linear(self, A, lda, Wm, ldw, M, N, K, bias, pre, Y, ldy):
+        """pre = A Wm^T + bias, Y = SiLU(pre) in the working precision (self.z as workspace)."""
+        self._c(self.lib.sfh_nn_linear(self.ctx.handle, self.dt, A.data_ptr(), lda, Wm.data_ptr(), ldw, M, N, K,
+                                       bias.data_ptr(), SILU, self.z.data_ptr(), pre.data_ptr(), Y.data_ptr(), ldy))
+
     def _bias_act(self, X, ldx, bias, rows, cols, act, pre, Y, ldy, dt=None):
         self._c(self.lib.sfh_nn_bias_act(self.ctx.handle, self.dt if dt is None else dt, X.data_ptr(), ldx,
                                          bias.data_ptr() if bias is not None else None, rows, cols, act,
@@ -211,12 +216,9 @@ class TemplateSurrogate:
         self._refresh_weights()
         self._bias_act(self.x0f, 16, None, B, 16, IDENT, None, self.x0, 16)
         # MLP
-        self._gemm(self.x0, 16, self.w["W0"], 16, self.z, HID0, B, HID0, 16)
-        self._bias_act(self.z, HID0, v["b0"], B, HID0, SILU, self.pre0, self.a0, HID0)
-        self._gemm(self.a0, HID0, self.w["W1"], HID0, self.z, HID1, B, HID1, HID0)
-        self._bias_act(self.z, HID1, v["b1"], B, HID1, SILU, self.pre1, self.a1, HID1)
-        self._gemm(self.a1, HID1, self.w["W2"], HID1, self.z, LAT, B, LAT, HID1)
-        self._bias_act(self.z, LAT, v["b2"], B, LAT, SILU, self.pre2, self.a2, LAT)
+        self._linear(self.x0, 16, self.w["W0"], 16, B, HID0, 16, v["b0"], self.pre0, self.a0, HID0)
+        self._linear(self.a0, HID0, self.w["W1"], HID0, B, HID1, HID0, v["b1"], self.pre1, self.a1, HID1)
+        self._linear(self.a1, HID1, self.w["W2"], HID1, B, LAT, HID1, v["b2"], self.pre2, self.a2, LAT)
         # decoder
         H, W = H0, W0
         ins = (self.a2, self.a3, self.a4)

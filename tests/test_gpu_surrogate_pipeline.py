@@ -471,3 +471,38 @@ def test_act_bwd_chansum(sfh, gpu_ctx, P, act):
     assert torch.equal(dX, dX2)
     ref = dX2.double().cpu().numpy()
     assert np.all(np.abs(dsum.cpu().numpy() - ref.sum(0)) <= 1e-5 * np.abs(ref).sum(0) + 1e-6)
+
+
+@pytest.mark.parametrize("dtype", [0, 1])
+@pytest.mark.parametrize("M,N,K,act,with_pre", [(32, 7904, 512, 1, True), (32, 512, 256, 1, True),
+                                                (32, 256, 16, 1, True), (1000, 64, 256, 0, False),
+                                                (16, 40, 2048, 1, False)])
+def test_linear_equals_product_then_bias_act(sfh, gpu_ctx, dtype, M, N, K, act, with_pre):
+    """A dense layer in one call (split products summed by the activation pass)
+    equals the product followed by the bias and activation pass, bit for bit."""
+    import torch
+    rng = np.random.default_rng(M * 7 + N + K)
+    tdt = torch.bfloat16 if dtype == 0 else torch.float32
+    A = torch.from_numpy(rng.normal(size=(M, K)).astype(np.float32)).cuda().to(tdt)
+    W = torch.from_numpy(rng.normal(scale=0.1, size=(N, K)).astype(np.float32)).cuda().to(tdt)
+    bias = torch.from_numpy(rng.normal(size=N).astype(np.float32)).cuda()
+    ldy = N + 8
+    z = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    pre = torch.full((M, N), np.nan, dtype=torch.float32, device="cuda")
+    Y = torch.full((M, ldy), np.nan, dtype=tdt, device="cuda")
+    z2 = torch.empty_like(z)
+    pre2 = torch.empty_like(pre)
+    Y2 = torch.empty_like(Y)
+    torch.cuda.synchronize()
+    lib, h = gpu_ctx.lib, gpu_ctx.handle
+    gpu_ctx.check(lib.sfh_nn_linear(h, dtype, A.data_ptr(), K, W.data_ptr(), K, M, N, K, bias.data_ptr(), act,
+                                    z.data_ptr(), pre.data_ptr() if with_pre else None, Y.data_ptr(), ldy))
+    gpu_ctx.check(lib.sfh_nn_gemm_nt(h, dtype, A.data_ptr(), K, W.data_ptr(), K, z2.data_ptr(), N, M, N, K, 1.0, 0.0))
+    gpu_ctx.check(lib.sfh_nn_bias_act(h, dtype, z2.data_ptr(), N, bias.data_ptr(), M, N, act, pre2.data_ptr(),
+                                      Y2.data_ptr(), ldy))
+    gpu_ctx.synchronize()
+    assert torch.equal(Y, Y2)
+    if with_pre:
+        assert torch.equal(pre, pre2)
+    ref = A.double() @ W.double().T + bias.double()
+    assert float((pre2.double() - ref).abs().max()) <= 1e-5 * float(ref.abs().max()) + 1e-5
