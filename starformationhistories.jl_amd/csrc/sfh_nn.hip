@@ -353,12 +353,22 @@ __global__ __launch_bounds__(kWG) void k_weight_copies(WCopies wc)
     T* dst = (T*)wc.dst[m];
     T* dstT = (T*)wc.dstT[m];
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
-    for (int j = ty; j < 32; j += 8) {
+    // the four loads issued together from clamped (in-range) positions; the
+    // positions past the matrix are neither stored nor transposed
+    float v[4];
+    const int64_t cc = c0 + tx < cols ? c0 + tx : cols - 1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t r = r0 + ty + 8 * k;
+        v[k] = src[(r < rows ? r : rows - 1) * cols + cc];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int j = ty + 8 * k;
         const int64_t r = r0 + j, c = c0 + tx;
         const bool ok = r < rows && c < cols;
-        const float v = ok ? src[r * cols + c] : 0.f;
-        if (ok) dst[r * cols + c] = from_f32<T>(v);
-        tile[j][tx] = v;
+        if (ok) dst[r * cols + c] = from_f32<T>(v[k]);
+        tile[j][tx] = ok ? v[k] : 0.f;
     }
     if (!dstT) return;
     __syncthreads();
