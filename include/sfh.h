@@ -406,11 +406,13 @@ int sfh_nn_bias_act(sfh_ctx* ctx, int dtype, const float* X, int64_t ldx, const 
                     int64_t cols, int act, float* pre, void* Y, int64_t ldy);
 /* A dense layer: pre (M x N FP32, may be NULL) = A W^T + bias (A: M x K, W:
  * N x K, dtype, K a multiple of 16) and Y (M x ldy, dtype) = act(pre) with zero
- * padding past N; z (M x N FP32) is workspace.  sfh_nn_gemm_nt followed by
- * sfh_nn_bias_act, bit for bit, with a split product's slices summed by the
- * activation pass itself. */
+ * padding past N; YT (N x ldt, dtype, may be NULL) = Y^T with zeros in rows
+ * M..ldt-1 (what sfh_nn_transpose of act(pre) gives); z (M x N FP32) is
+ * workspace.  sfh_nn_gemm_nt followed by sfh_nn_bias_act, bit for bit, with a
+ * split product's slices summed by the activation pass itself. */
 int sfh_nn_linear(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* W, int64_t ldw, int64_t M,
-                  int64_t N, int64_t K, const float* bias, int act, float* z, float* pre, void* Y, int64_t ldy);
+                  int64_t N, int64_t K, const float* bias, int act, float* z, float* pre, void* Y, int64_t ldy,
+                  void* YT, int64_t ldt);
 /* dX = dY * act'(pre) elementwise (dX may alias dY). */
 int sfh_nn_act_bwd(sfh_ctx* ctx, const float* dY, const float* pre, int64_t n, int act, float* dX);
 /* The same over P pixels of 32 channels (16-byte aligned), with the channel

@@ -145,7 +145,8 @@ SIGNATURES = [
     ("sfh_bin_cmd_catalog", _int, [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, sfh_range, sfh_range, _vp]),
     ("sfh_nn_gemm_nt", _int, [_vp, _int, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _i64, C.c_float, C.c_float]),
     ("sfh_nn_bias_act", _int, [_vp, _int, _vp, _i64, _vp, _i64, _i64, _int, _vp, _vp, _i64]),
-    ("sfh_nn_linear", _int, [_vp, _int, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _vp, _int, _vp, _vp, _vp, _i64]),
+    ("sfh_nn_linear", _int, [_vp, _int, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _vp, _int, _vp, _vp, _vp, _i64,
+                             _vp, _i64]),
     ("sfh_nn_act_bwd", _int, [_vp, _vp, _vp, _i64, _int, _vp]),
     ("sfh_nn_act_bwd_chansum", _int, [_vp, _vp, _vp, _i64, _int, _vp, _vp]),
     ("sfh_nn_transpose", _int, [_vp, _int, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _int]),

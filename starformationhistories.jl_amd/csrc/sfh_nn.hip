@@ -242,10 +242,21 @@ __global__ __launch_bounds__(kWG) void k_gemm_nt(const T* __restrict__ A, int64_
 
 // Y = act(X + bias[col]) over rows x cols (row-major, ldx); pre (rows x cols,
 // may be null) keeps X + bias; Y (ldy >= cols) gets zeros in cols..ldy-1.
+// YT (may be null) also gets Y's transpose: YT[c ldt + r] for c < cols, with
+// zeros for rows <= r < ldt.
+template <typename T>
+__device__ inline void store_transposed(T* __restrict__ YT, int64_t ldt, int64_t rows, int64_t rr, int64_t c, T y)
+{
+    YT[c * ldt + rr] = y;
+    if (rr == rows - 1)
+        for (int64_t r2 = rows; r2 < ldt; ++r2) YT[c * ldt + r2] = from_f32<T>(0.f);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_bias_act(const float* __restrict__ X, int64_t ldx,
                                                   const float* __restrict__ bias, int64_t rows, int64_t cols,
-                                                  int act, float* __restrict__ pre, T* __restrict__ Y, int64_t ldy)
+                                                  int act, float* __restrict__ pre, T* __restrict__ Y, int64_t ldy,
+                                                  T* __restrict__ YT, int64_t ldt)
 {
     const int64_t n = rows * ldy;
     for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) {
@@ -255,6 +266,7 @@ __global__ __launch_bounds__(kWG) void k_bias_act(const float* __restrict__ X, i
             const float x = X[rr * ldx + c] + (bias ? bias[c] : 0.f);
             if (pre) pre[rr * cols + c] = x;
             y = act == 1 ? x * sigmoid(x) : x;
+            if (YT) store_transposed(YT, ldt, rows, rr, c, from_f32<T>(y));
         }
         Y[i] = from_f32<T>(y);
     }
@@ -269,7 +281,8 @@ __global__ __launch_bounds__(kWG) void k_bias_act(const float* __restrict__ X, i
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_splitk_bias_act(const float* __restrict__ part, int64_t nparts, int64_t rows,
                                                          int64_t cols, const float* __restrict__ bias, int act,
-                                                         float* __restrict__ pre, T* __restrict__ Y, int64_t ldy)
+                                                         float* __restrict__ pre, T* __restrict__ Y, int64_t ldy,
+                                                         T* __restrict__ YT, int64_t ldt)
 {
     const int64_t n = rows * ldy, rc = rows * cols;
     for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (int64_t)gridDim.x * kWG) {
@@ -282,6 +295,7 @@ __global__ __launch_bounds__(kWG) void k_splitk_bias_act(const float* __restrict
             const float x = 1.f * (((g[0] + g[1]) + g[2]) + g[3]) + (bias ? bias[c] : 0.f);
             if (pre) pre[e] = x;
             y = act == 1 ? x * sigmoid(x) : x;
+            if (YT) store_transposed(YT, ldt, rows, rr, c, from_f32<T>(y));
         }
         Y[i] = from_f32<T>(y);
     }
@@ -1841,21 +1855,23 @@ int sfh_nn_bias_act(sfh_ctx* ctx, int dtype, const float* X, int64_t ldx, const 
     const int64_t n = rows * ldy;
     if (dtype == 0)
         hipLaunchKernelGGL(k_bias_act<uint16_t>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, X, ldx, bias, rows,
-                           cols, act, pre, (uint16_t*)Y, ldy);
+                           cols, act, pre, (uint16_t*)Y, ldy, (uint16_t*)nullptr, (int64_t)0);
     else
         hipLaunchKernelGGL(k_bias_act<float>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, X, ldx, bias, rows, cols,
-                           act, pre, (float*)Y, ldy);
+                           act, pre, (float*)Y, ldy, (float*)nullptr, (int64_t)0);
     return check_launch(ctx, "k_bias_act");
 }
 
 int sfh_nn_linear(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const void* W, int64_t ldw, int64_t M,
-                  int64_t N, int64_t K, const float* bias, int act, float* z, float* pre, void* Y, int64_t ldy)
+                  int64_t N, int64_t K, const float* bias, int act, float* z, float* pre, void* Y, int64_t ldy,
+                  void* YT, int64_t ldt)
 {
     SFH_NN_BEGIN(ctx);
     SFH_ARG_CHECK(ctx, A && W && z && Y && (dtype == 0 || dtype == 1) && (act == 0 || act == 1) && M >= 0 && N >= 0 &&
-                           K >= 0 && K % 16 == 0 && lda >= K && ldw >= K && lda % 8 == 0 && ldw % 8 == 0 && ldy >= N,
+                           K >= 0 && K % 16 == 0 && lda >= K && ldw >= K && lda % 8 == 0 && ldw % 8 == 0 &&
+                           ldy >= N && (!YT || ldt >= M),
                   "sfh_nn_linear: dtype 0 or 1, act 0 or 1, K a multiple of 16, lda and ldw >= K (multiples of 8), "
-                  "ldy >= N");
+                  "ldy >= N, ldt >= M");
     if (M == 0) return SFH_OK;
     float* slices = nullptr;
     int64_t ns = 1;
@@ -1864,18 +1880,18 @@ int sfh_nn_linear(sfh_ctx* ctx, int dtype, const void* A, int64_t lda, const voi
     if (ns > 1) {
         if (dtype == 0)
             hipLaunchKernelGGL(k_splitk_bias_act<uint16_t>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream,
-                               (const float*)slices, ns, M, N, bias, act, pre, (uint16_t*)Y, ldy);
+                               (const float*)slices, ns, M, N, bias, act, pre, (uint16_t*)Y, ldy, (uint16_t*)YT, ldt);
         else
             hipLaunchKernelGGL(k_splitk_bias_act<float>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream,
-                               (const float*)slices, ns, M, N, bias, act, pre, (float*)Y, ldy);
+                               (const float*)slices, ns, M, N, bias, act, pre, (float*)Y, ldy, (float*)YT, ldt);
         return check_launch(ctx, "k_splitk_bias_act");
     }
     if (dtype == 0)
         hipLaunchKernelGGL(k_bias_act<uint16_t>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, (const float*)z, N, bias,
-                           M, N, act, pre, (uint16_t*)Y, ldy);
+                           M, N, act, pre, (uint16_t*)Y, ldy, (uint16_t*)YT, ldt);
     else
         hipLaunchKernelGGL(k_bias_act<float>, dim3(grid_for(n)), dim3(kWG), 0, ctx->stream, (const float*)z, N, bias, M,
-                           N, act, pre, (float*)Y, ldy);
+                           N, act, pre, (float*)Y, ldy, (float*)YT, ldt);
     return check_launch(ctx, "k_bias_act");
 }
 
@@ -2044,7 +2060,7 @@ static int conv1_activated(sfh_ctx* ctx, const float* in, int64_t P, int64_t Ci,
     void* p = nullptr;
     if (int rc = scratch(ctx, "nn_conv1_act", (size_t)P * Ci * sizeof(float), &p)) return rc;
     hipLaunchKernelGGL(k_bias_act<float>, dim3(grid_for(P * Ci)), dim3(kWG), 0, ctx->stream, in, Ci,
-                       (const float*)nullptr, P, Ci, act, (float*)nullptr, (float*)p, Ci);
+                       (const float*)nullptr, P, Ci, act, (float*)nullptr, (float*)p, Ci, (float*)nullptr, (int64_t)0);
     *out = (const float*)p;
     return check_launch(ctx, "k_bias_act");
 }

@@ -152,7 +152,9 @@ class TemplateSurrogate:
         self.dB = self._buf(max(P4 * CH, B * LAT))
         self.dcols = self._buf(B * LAT, t)                       # low-precision copies of MLP gradients
         self.dcolsT = self._buf(LAT * Bp, t)                     # transposed MLP gradients
-        self.actT = self._buf(HID1 * Bp, t)                       # transposed MLP activations
+        self.actT = self._buf(16 * Bp, t)                         # transposed network input
+        self.a0T = self._buf(HID0 * Bp, t)                        # transposed MLP activations (from the forward)
+        self.a1T = self._buf(HID1 * Bp, t)
         # low-precision weight copies and their transposes
         self.w = {k: self._buf(self.views[k].numel(), t) for k in ("W0", "W1", "W2", "K1", "K2", "K3")}
         self.wT = {k: self._buf(self.views[k].numel(), t) for k in ("W1", "W2", "K1", "K2", "K3")}
@@ -166,10 +168,13 @@ class TemplateSurrogate:
         self._c(self.lib.sfh_nn_gemm_nt(self.ctx.handle, self.dt, A.data_ptr(), lda, Bm.data_ptr(), ldb,
                                         Cm.data_ptr(), ldc, M, N, K, 1.0, beta))
 
-    def _linear(self, A, lda, Wm, ldw, M, N, K, bias, pre, Y, ldy):
-        """pre = A Wm^T + bias, Y = SiLU(pre) in the working precision (self.z as workspace)."""
+    def _linear(self, A, lda, Wm, ldw, M, N, K, bias, pre, Y, ldy, YT=None):
+        """pre = A Wm^T + bias, Y = SiLU(pre) in the working precision and, with
+        YT, its transpose (N x bp, zero padded) for the weight gradient
+        (self.z as workspace)."""
         self._c(self.lib.sfh_nn_linear(self.ctx.handle, self.dt, A.data_ptr(), lda, Wm.data_ptr(), ldw, M, N, K,
-                                       bias.data_ptr(), SILU, self.z.data_ptr(), pre.data_ptr(), Y.data_ptr(), ldy))
+                                       bias.data_ptr(), SILU, self.z.data_ptr(), pre.data_ptr(), Y.data_ptr(), ldy,
+                                       YT.data_ptr() if YT is not None else None, self.bp))
 
     def _bias_act(self, X, ldx, bias, rows, cols, act, pre, Y, ldy, dt=None):
         self._c(self.lib.sfh_nn_bias_act(self.ctx.handle, self.dt if dt is None else dt, X.data_ptr(), ldx,
@@ -216,8 +221,8 @@ class TemplateSurrogate:
         self._refresh_weights()
         self._bias_act(self.x0f, 16, None, B, 16, IDENT, None, self.x0, 16)
         # MLP
-        self._linear(self.x0, 16, self.w["W0"], 16, B, HID0, 16, v["b0"], self.pre0, self.a0, HID0)
-        self._linear(self.a0, HID0, self.w["W1"], HID0, B, HID1, HID0, v["b1"], self.pre1, self.a1, HID1)
+        self._linear(self.x0, 16, self.w["W0"], 16, B, HID0, 16, v["b0"], self.pre0, self.a0, HID0, self.a0T)
+        self._linear(self.a0, HID0, self.w["W1"], HID0, B, HID1, HID0, v["b1"], self.pre1, self.a1, HID1, self.a1T)
         self._linear(self.a1, HID1, self.w["W2"], HID1, B, LAT, HID1, v["b2"], self.pre2, self.a2, LAT)
         # decoder
         H, W = H0, W0
@@ -289,12 +294,11 @@ class TemplateSurrogate:
             self._c(self.lib.sfh_nn_dense_bwd_prep(self.ctx.handle, self.dt, d.data_ptr(), pre_out.data_ptr(), B, nout,
                                                    SILU, g[bn].data_ptr(), self.dcols.data_ptr(),
                                                    self.dcolsT.data_ptr(), Bp))
-            # dW = d^T a_in
-            if pre_in is not None:
-                self._transpose(pre_in, nin, B, nin, self.actT, Bp, act=SILU)
-            else:
+            # dW = d^T a_in, a_in transposed by the forward (the network input here)
+            if pre_in is None:
                 self._transpose(self.x0f, 16, B, 16, self.actT, Bp)
-            self._gemm(self.dcolsT, Bp, self.actT, Bp, g[Wn], nin, nout, nin, Bp)
+            aT = {"W2": self.a1T, "W1": self.a0T, "W0": self.actT}[Wn]
+            self._gemm(self.dcolsT, Bp, aT, Bp, g[Wn], nin, nout, nin, Bp)
             if pre_in is None:
                 break
             # d a_in = d W  (low-precision copy of d times the transposed weights)

@@ -479,7 +479,8 @@ def test_act_bwd_chansum(sfh, gpu_ctx, P, act):
                                                 (16, 40, 2048, 1, False)])
 def test_linear_equals_product_then_bias_act(sfh, gpu_ctx, dtype, M, N, K, act, with_pre):
     """A dense layer in one call (split products summed by the activation pass)
-    equals the product followed by the bias and activation pass, bit for bit."""
+    equals the product followed by the bias and activation pass, bit for bit,
+    and its transposed output the transpose kernel's (zero padding included)."""
     import torch
     rng = np.random.default_rng(M * 7 + N + K)
     tdt = torch.bfloat16 if dtype == 0 else torch.float32
@@ -495,13 +496,21 @@ def test_linear_equals_product_then_bias_act(sfh, gpu_ctx, dtype, M, N, K, act, 
     Y2 = torch.empty_like(Y)
     torch.cuda.synchronize()
     lib, h = gpu_ctx.lib, gpu_ctx.handle
+    ldt = (M + 15) // 16 * 16 + 16
+    YT = torch.full((N, ldt), np.nan, dtype=tdt, device="cuda")
+    YT2 = torch.empty_like(YT)
+    torch.cuda.synchronize()
     gpu_ctx.check(lib.sfh_nn_linear(h, dtype, A.data_ptr(), K, W.data_ptr(), K, M, N, K, bias.data_ptr(), act,
-                                    z.data_ptr(), pre.data_ptr() if with_pre else None, Y.data_ptr(), ldy))
+                                    z.data_ptr(), pre.data_ptr() if with_pre else None, Y.data_ptr(), ldy,
+                                    YT.data_ptr(), ldt))
     gpu_ctx.check(lib.sfh_nn_gemm_nt(h, dtype, A.data_ptr(), K, W.data_ptr(), K, z2.data_ptr(), N, M, N, K, 1.0, 0.0))
     gpu_ctx.check(lib.sfh_nn_bias_act(h, dtype, z2.data_ptr(), N, bias.data_ptr(), M, N, act, pre2.data_ptr(),
                                       Y2.data_ptr(), ldy))
+    # the transposed activations (zero padded to ldt) as the transpose kernel gives them
+    gpu_ctx.check(lib.sfh_nn_transpose(h, dtype, pre2.data_ptr(), N, M, N, YT2.data_ptr(), ldt, N, act))
     gpu_ctx.synchronize()
     assert torch.equal(Y, Y2)
+    assert torch.equal(YT, YT2)
     if with_pre:
         assert torch.equal(pre, pre2)
     ref = A.double() @ W.double().T + bias.double()
