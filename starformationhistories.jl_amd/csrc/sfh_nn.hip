@@ -1439,12 +1439,17 @@ __global__ __launch_bounds__(kWG) void k_tl_sums(const float* __restrict__ out, 
     const int64_t tb = rows ? rows[b] : b;
     const int64_t c0 = n * blockIdx.x / kTLChunks, c1 = n * (blockIdx.x + 1) / kTLChunks;
     double ss = 0.0, se = 0.0, a = 0.0, q = 0.0;
-    // (Y, X) of element i, stepped with i instead of divided per element
+    // (Y, X) of element i, stepped with i (by kWG / S rows and kWG % S columns)
+    // instead of divided per element; unrolled so that several iterations'
+    // loads are in flight together
     int64_t Y = (c0 + threadIdx.x) / S, X = (c0 + threadIdx.x) % S;
+    const int64_t dy = kWG / S, dx = kWG % S;
+#pragma unroll 4
     for (int64_t i = c0 + threadIdx.x; i < c1; i += kWG) {
         const float y = out[(b * H2 + Y) * W2 + X], t = target[tb * n + i];
-        X += kWG;
-        while (X >= S) {
+        X += dx;
+        Y += dy;
+        if (X >= S) {
             X -= S;
             ++Y;
         }
@@ -1494,6 +1499,7 @@ __global__ __launch_bounds__(kWG) void k_tl_grad(const float* __restrict__ out, 
     const int64_t i0 = (int64_t)blockIdx.x * kWG + threadIdx.x, step = (int64_t)gridDim.x * kWG;
     const int64_t dy = step / W2, dx = step % W2;
     int64_t Y = i0 / W2, X = i0 % W2;
+#pragma unroll 4
     for (int64_t i = i0; i < n2; i += step) {
         float d = 0.f;
         if (Y < R && X < S) {
