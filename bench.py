@@ -50,6 +50,9 @@ def parse():
                     help="birth mass (Msun) of the mock population sampled, observed and binned on the GPU")
     ap.add_argument("--no-mock", action="store_true")
     ap.add_argument("--no-binaries", action="store_true")
+    ap.add_argument("--section", default="all", choices=["all", "templates", "plain"],
+                    help="templates: only the deposition of the 700 templates; plain: deposition, then only the "
+                         "single-walker fg! loop (short runs for the profiler's counter passes)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates HBM counters)")
     return ap.parse_args()
@@ -475,6 +478,104 @@ def _fill_bench(S, torch, ctx, O, args, ncpu, net, la, mh, nstars, exact_rate):
     return out["fill"]
 
 
+# VALU issue peak of the MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-instructions/s
+VALU_PEAK = 256 * 64 * 2.4e9
+
+
+def templates_bench(sfh, torch, ctx, O, args, ncpu):
+    """The 700-template workload (70 ages x 10 metallicities, 2000-point
+    isochrones, 200 x 300 grid, IMF 0.08-120 Msun, colour-magnitude
+    correlation 0.6) built by partial_cmd_smooth_batch: the host preparation
+    and the deposition on the device.  Returns the report and the device
+    templates."""
+    from sfh_amd import synth
+    edges = synth.edges_200x300()
+    nbins = (edges[0].length - 1) * (edges[1].length - 1)
+    specs = synth.specs_70x10()
+    T = len(specs)
+    imf = synth.Kroupa2001(0.08, 120.0)
+    t0 = time.time()
+    isos = [synth.isochrone(s.log_age, s.mh, 2000, s.seed, mmax_imf=120.0) for s in specs]
+    gen_s = time.time() - t0               # synthetic inputs, not part of the path
+    kw = dict(dmod=24.0, normalize_value=1.0, mean_mass=imf.mean(), correlation=0.6, edges=edges)
+    errs, comps = [synth.PHOTERR] * 2, [synth.COMPLETENESS] * 2
+    out, (npts, pairs) = sfh.partial_cmd_smooth_batch(isos, errs, 1, (0, 1), imf, comps, return_info=True,
+                                                      ctx=ctx, **kw)
+    nrep = 3
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(nrep):
+        out = sfh.partial_cmd_smooth_batch(isos, errs, 1, (0, 1), imf, comps, ctx=ctx, **kw)
+    torch.cuda.synchronize()
+    e2e_s = (time.time() - t0) / nrep
+    ctx.set_timing(True)
+    ctx.reset_stats()
+    for _ in range(nrep):
+        out = sfh.partial_cmd_smooth_batch(isos, errs, 1, (0, 1), imf, comps, ctx=ctx, **kw)
+    ctx.synchronize()
+    kern = {k: ctx.kernel_stats(k) for k in ("pcs_count", "pcs_points", "pcs_desc", "deposit_cov")}
+    ctx.set_timing(False)
+    kms = {k: v[0] / max(v[1], 1) for k, v in kern.items()}
+    dep_ms = kms["deposit_cov"]
+    prep_ms = kms["pcs_count"] + kms["pcs_points"] + kms["pcs_desc"]
+    pairs_per_s = pairs / (dep_ms * 1e-3)
+    # VALU work per pair of the gather's dominant form (degree-1 series, full
+    # width): per GL node and bin 5 VALU instructions (column coordinate, its
+    # square, the series polynomial, exp2, the accumulation); 3 nodes per pair
+    ops_per_pair = 15.0
+    achieved = pairs_per_s * ops_per_pair
+    roof = {"bound": "valu", "kernel": "k_dep_gather<false>", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
+            "unit": "T VALU lane-instructions/s", "frac": achieved / VALU_PEAK, "traffic": None,
+            "algorithmic": f"{ops_per_pair:g} VALU instructions per (point, bin) pair (3 GL nodes x (FMA coordinate, "
+                           f"square, degree-1 series FMA, exp2, accumulate FMA)) x {pairs:.4g} pairs per launch; "
+                           f"peak = 256 CUs x 64 lanes x 2.4 GHz"}
+    # the same templates through the host (NumPy) preparation, for comparison
+    t0 = time.time()
+    pc = [synth.template_points(s, 2000, edges, 120.0, correlation=0.6) for s in specs]
+    host_prep_s = time.time() - t0
+    rep = {"templates": T, "bins": nbins, "kernel_points": npts, "pairs": pairs,
+           "end_to_end_s": e2e_s, "templates_per_s_end_to_end": T / e2e_s,
+           "kernel_ms": kms, "device_prep_ms": prep_ms, "deposit_ms": dep_ms,
+           "templates_per_s_kernels": T / ((prep_ms + dep_ms) * 1e-3), "pairs_per_s": pairs_per_s,
+           "hbm_GBps_algorithmic": (nbins * T * 4 + npts * 80) / (dep_ms * 1e-3) / 1e9,
+           "host_numpy_prep_s": host_prep_s, "synthetic_isochrones_s": gen_s, "roofline": roof,
+           "note": "end-to-end = host isochrone arrays in, templates resident on the device out (upload, device "
+                   "preparation, deposition); the isochrones themselves are synthetic inputs generated beforehand"}
+    log(f"templates: {T} in {e2e_s * 1e3:.1f} ms end to end ({T / e2e_s:.4g}/s); kernels: prep {prep_ms:.2f} ms, "
+        f"deposit {dep_ms:.2f} ms ({pairs_per_s:.3g} pairs/s, VALU frac {roof['frac']:.2f}); host NumPy prep "
+        f"would take {host_prep_s:.2f} s")
+    # parity of the bench's own templates on three of them
+    ex, ey = (O.Range(e.first, e.step, e.length) for e in edges)
+    errs_t = []
+    for t in (0, T // 2, T - 1):
+        sp, cm = pc[t]
+        ref, _ = O.bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, 1, sp.weights, ex, ey, cmult=cm,
+                                  nthreads=ncpu)
+        got = out[t].cpu().numpy().reshape(ref.shape, order="F")
+        errs_t.append(float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))))
+    rep["max_norm_err_vs_oracle"] = max(errs_t)
+    if not args.no_cpu:
+        # CPU baseline: the NumPy preparation + the C FP64 bin_cmd_smooth of the
+        # oracle on all host cores, bounded to ~cpu_seconds
+        k, cpu_s, cpu_pairs = 0, 0.0, 0
+        while cpu_s < args.cpu_seconds and k < T:
+            s = specs[(k * 37) % T]
+            t1 = time.time()
+            sp, cm = synth.template_points(s, 2000, edges, 120.0, correlation=0.6)
+            _, pr = O.bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, 1, sp.weights, ex, ey, cmult=cm,
+                                     nthreads=ncpu)
+            cpu_s += time.time() - t1
+            cpu_pairs += pr
+            k += 1
+        rep["cpu_baseline"] = {"value": k / cpu_s, "unit": "templates/s", "cores": ncpu, "kind": "port",
+                               "pairs_per_s": cpu_pairs / cpu_s,
+                               "sample": f"{k} of the {T} templates: NumPy preparation + the C FP64 restatement of "
+                                         f"bin_cmd_smooth (OpenMP), {cpu_s:.1f} s"}
+        rep["gpu_over_cpu_end_to_end"] = (T / e2e_s) / (k / cpu_s)
+        log(f"templates cpu oracle: {k / cpu_s:.3g} templates/s on {ncpu} threads")
+    return rep, out
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -502,30 +603,12 @@ def main():
     # ---------------------------------------------------- templates (deposition)
     edges = synth.edges_200x300()
     nbins = (edges[0].length - 1) * (edges[1].length - 1)
-    specs = synth.specs_70x10()
-    t0 = time.time()
-    pc = [synth.template_points(s, 2000, edges, 120.0, correlation=0.6) for s in specs]
-    prep_s = time.time() - t0
-    batch = sfh.PointBatch([p for p, _ in pc], edges, cmult_list=[c for _, c in pc])
-    log(f"host prep of {len(specs)} templates: {prep_s:.2f} s, {batch.npoints} kernel points")
-    out, pairs = sfh.deposit_batch(batch, ctx=ctx, return_pairs=True)
-    ctx.set_timing(True)
-    ctx.reset_stats()
-    nrep = 3
-    with Timer(torch) as tm:
-        for _ in range(nrep):
-            sfh.deposit_batch(batch, out=out, ctx=ctx)
-    dep_ms = tm.ms / nrep
-    kms, kn = ctx.kernel_stats("deposit_cov")
-    dep_kernel_ms = kms / max(kn, 1)
-    ctx.set_timing(False)
-    dep_bytes = nbins * T_ * 4 + batch.npoints * 6 * 8
-    res["templates"] = {
-        "templates_per_s": T_ / (dep_ms * 1e-3), "ms_per_700": dep_ms, "kernel_ms": dep_kernel_ms,
-        "pairs": pairs, "pairs_per_s": pairs / (dep_kernel_ms * 1e-3),
-        "hbm_GBps_algorithmic": dep_bytes / (dep_kernel_ms * 1e-3) / 1e9, "host_prep_s": prep_s,
-        "note": "VALU/transcendental bound: ~3 exp + 3 polynomial (or erf) evaluations per (point, bin) pair"}
-    log(f"deposition: {dep_ms:.2f} ms / 700 templates, {pairs:.3e} pairs")
+    O = ge.load_oracle()
+    ncpu = os.cpu_count() or 1
+    res["templates"], out = templates_bench(sfh, torch, ctx, O, args, ncpu)
+    if args.section == "templates":
+        print(json.dumps({"section": "templates", **res["templates"]}), flush=True)
+        return
 
     tpl = sfh.templates_from_device(out, nbins, ctx=ctx)
     models = np.asfortranarray(out.cpu().numpy().T)        # nbins x 700 FP32 (host copy for the oracle)
@@ -559,8 +642,6 @@ def main():
         return tm.ms / steps, stats
 
     # oracle on a sample of walkers (errors of every form)
-    O = ge.load_oracle()
-    ncpu = os.cpu_count() or 1
     nsample = 8
     rnl, rG = O.fg_batched(models, data, X[:, :nsample], nthreads=ncpu)
 
@@ -570,6 +651,17 @@ def main():
         return {"ll_rel_err_max": float(np.max(np.abs(nl - rnl) / np.abs(rnl))),
                 "grad_norm_rel_err_max": float(np.max(np.linalg.norm(G - rG, axis=0) / np.linalg.norm(rG, axis=0)))}
 
+    if args.section == "plain":
+        d1 = torch.from_numpy(X[:, 0].copy()).cuda()
+        n1 = torch.empty(1, dtype=torch.float64, device="cuda")
+        g1 = torch.empty(T_, dtype=torch.float64, device="cuda")
+        for _ in range(5):
+            sfh.fg_device(tpl, hd, d1, n1, g1)
+        with Timer(torch) as tm:
+            for _ in range(50):
+                sfh.fg_device(tpl, hd, d1, n1, g1)
+        print(json.dumps({"section": "plain", "ms_per_eval": tm.ms / 50}), flush=True)
+        return
     ms, stats = batched(args.precision, args.steps, args.warmup)
     err = errors()
     value = B / (ms * 1e-3)
@@ -590,6 +682,9 @@ def main():
     roofline = {"bound": "mfma", "kernel": "k_gemm_f64_ll" if args.precision == "f64" else "k_gemm<LL>",
                 "achieved": achieved, "peak": PEAK[peak_key], "unit": "TFLOP/s",
                 "frac": achieved / PEAK[peak_key], "traffic": traffic,
+                "traffic_source": "not measured in this run: calibrated FETCH_SIZE + WRITE_SIZE bytes per dispatch "
+                                  "of this kernel on this workload from rocprofv3 --pmc runs "
+                                  "(profiles/hbm_traffic.json)" if traffic else None,
                 "algorithmic": f"2*T*N*B = {flops:.3e} FLOP per launch (composite), {PEAK[peak_key]} TFLOP/s "
                                f"dense {peak_key} peak"}
 
@@ -621,14 +716,18 @@ def main():
             sfh.fg_device(tpl, hd, d1, n1, g1)
         pk = {k: ctx.kernel_stats(k) for k in ("single_fg", "single_reduce")}
         ctx.set_timing(False)
-        plain_bytes = 2 * T_ * nbins * 4 + nbins * 12
+        # one pass: every template element read once (FP32), the FP64 data once
+        plain_bytes = T_ * nbins * 4 + nbins * 8
         fgk = pk["single_fg"][0] / max(pk["single_fg"][1], 1)
+        bw = plain_bytes / (fgk * 1e-3) / 1e9
         extras["plain"] = {
             "evals_per_s": 1e3 / pms, "ms_per_eval": pms,
             "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in pk.items()},
-            "hbm_GBps_algorithmic": plain_bytes / (pms * 1e-3) / 1e9,
-            "fg_kernel_GBps_read": T_ * nbins * 4 / (fgk * 1e-3) / 1e9,
-            "fg_kernel_GBps_algorithmic": plain_bytes / (fgk * 1e-3) / 1e9,
+            "roofline": {"bound": "hbm", "kernel": "k_single_fg", "achieved": bw, "peak": PEAK["hbm"],
+                         "unit": "GB/s", "frac": bw / PEAK["hbm"],
+                         "algorithmic": f"{plain_bytes / 1e6:.1f} MB per evaluation (templates read once, FP32, "
+                                        f"+ FP64 data); the 168 MB template set fits the 256 MB Infinity Cache, "
+                                        f"so repeated evaluations can be served above the HBM rate"},
             "ll_rel_err": float(abs(n1.item() - rnl[0]) / abs(rnl[0]))}
         extras["batched_over_plain"] = value / extras["plain"]["evals_per_s"]
         log(f"plain: {pms:.4f} ms/eval ({1e3 / pms:.4g} evals/s); batched/plain = {extras['batched_over_plain']:.1f}x")
@@ -661,11 +760,15 @@ def main():
         "metric": "loglike+grad evals/s (700 tpl×60k bins) batched vs plain; templates/s; HBM GB/s",
         "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"f64": "f64", "bf16x2": "bf16", "fp32": "f32"}[args.precision],
+        "dtype": {"f64": "f64/bf16x2", "bf16x2": "bf16x2", "fp32": "f32"}[args.precision],
         "data": "synthetic (seeded isochrones deposited on the GPU; Poisson data; log-normal walkers)",
         "config": {"workload": f"batched fg! of {B} walkers, 700 templates x 60,000 bins "
                                f"({args.precision}: composite on the matrix cores, FP64 log-likelihood)",
-                   "templates": T_, "bins": nbins, "walkers": B, "precision": args.precision},
+                   "templates": T_, "bins": nbins, "walkers": B, "precision": args.precision,
+                   "precision_detail": {"f64": "composite on the FP64 matrix cores, FP64 log-likelihood, gradient "
+                                               "in split-BF16 hi+lo products with FP32 accumulation",
+                                        "bf16x2": "split-BF16 hi+lo products, FP32 accumulation, FP64 LL",
+                                        "fp32": "FP32 matrix-core products, FP64 LL"}[args.precision]},
         "roofline": roofline, "cpu_baseline": cpu, "errors": err, "kernel_ms": kernels,
         "templates": res["templates"], **extras,
     }

@@ -129,29 +129,34 @@ int sfh_loglikelihood_coeffs(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_d
                              const double* coeffs, double eps, double* out);
 
 /* ∇loglikelihood(models::AbstractMatrix, composite, data)
- * -- src/fitting/fitting_base.jl:177-182 (composite left unchanged). */
+ * -- src/fitting/fitting_base.jl:177-182 (composite left unchanged).  `eps`
+ * is eps(promote_type(eltype(models), eltype(composite), eltype(data))) (:148). */
 int sfh_grad_loglikelihood(sfh_ctx* ctx, const sfh_templates* tpl, const double* composite,
                            const sfh_data* data, double eps, double* G);
 
 /* ∇loglikelihood!(G, composite::AbstractVector, models::AbstractMatrix, data)
  * -- src/fitting/fitting_base.jl:265-285.  Overwrites composite with
- * 1 - n_i/max(m_i, eps) exactly as the reference does (:274-280). */
+ * 1 - n_i/max(m_i, eps) exactly as the reference does (:274-280); `eps` is
+ * eps(eltype(composite)) (:277). */
 int sfh_grad_loglikelihood_inplace(sfh_ctx* ctx, const sfh_templates* tpl, double* composite,
                                    const sfh_data* data, double eps, double* G);
 
 /* fg!(F, G, coeffs, models, data, composite) -- src/fitting/solvers.jl:20-38.
  * want_F selects the objective; G may be NULL (gradient not wanted).  On
  * return *negLL = -logL (when want_F), G = -∇logL, and composite (if not
- * NULL) holds what the reference leaves there: 1 - n/max(m,eps) when G was
- * requested, else m. */
+ * NULL) holds what the reference leaves there: 1 - n/max(m,eps_grad) when G
+ * was requested, else m.  fg! calls loglikelihood(composite, data), whose
+ * clamp is eps(promote_type(eltype(composite), eltype(data)))
+ * (fitting_base.jl:86) -> `eps`, and ∇loglikelihood!, whose clamp is
+ * eps(eltype(composite)) (:277) -> `eps_grad`. */
 int sfh_fg(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
-           const double* coeffs, int want_F, double eps, double* negLL, double* G,
-           double* composite);
+           const double* coeffs, int want_F, double eps, double eps_grad, double* negLL,
+           double* G, double* composite);
 /* Device-resident form: d_negLL (1 double), d_G (ntemplates) and
  * d_composite (nbins) may each be NULL. */
 int sfh_fg_device(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
-                  const double* d_coeffs, int want_F, double eps, double* d_negLL,
-                  double* d_G, double* d_composite);
+                  const double* d_coeffs, int want_F, double eps, double eps_grad,
+                  double* d_negLL, double* d_G, double* d_composite);
 
 /* --------------------------------------- P2: many coefficient vectors (new) */
 
@@ -159,13 +164,14 @@ int sfh_fg_device(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
  * call pattern of LogDensityProblems.logdensity_and_gradient(::HMCModel)
  * (src/fitting/hmc_sample.jl:24-37) and MCMCModel (src/fitting/mcmc_sample.jl:12-23)
  * over many walkers.  negLL[b] = -logL(X[:,b]); G[:,b] = -∇logL (G may be
- * NULL).  The final log-likelihood reduction is FP64 in every precision. */
+ * NULL).  The final log-likelihood reduction is FP64 in every precision.
+ * eps / eps_grad: the two clamps of sfh_fg. */
 int sfh_fg_batched(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
-                   const double* X, int64_t B, int precision, double eps,
+                   const double* X, int64_t B, int precision, double eps, double eps_grad,
                    double* negLL, double* G);
 int sfh_fg_batched_device(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
                           const double* d_X, int64_t B, int precision, double eps,
-                          double* d_negLL, double* d_G);
+                          double eps_grad, double* d_negLL, double* d_G);
 
 /* ------------------------------- P2: many independent fits of uneven size (new) */
 
@@ -195,10 +201,12 @@ int sfh_multifit_autotune(sfh_multifit* mf, int reps, int64_t* item_elems, int* 
 
 /* fg!(true, G, coeffs_f, models_f, data_f, composite) of src/fitting/solvers.jl:20-38
  * for every fit f in one launch: negLL[f] = -logL_f, G (concatenated, may be
- * NULL) = -∇logL_f.  X: concatenated coefficient vectors. */
-int sfh_multifit_fg(sfh_multifit* mf, const double* X, double eps, double* negLL, double* G);
-int sfh_multifit_fg_device(sfh_multifit* mf, const double* d_X, double eps, double* d_negLL,
-                           double* d_G);
+ * NULL) = -∇logL_f.  X: concatenated coefficient vectors.  eps / eps_grad:
+ * the two clamps of sfh_fg. */
+int sfh_multifit_fg(sfh_multifit* mf, const double* X, double eps, double eps_grad, double* negLL,
+                    double* G);
+int sfh_multifit_fg_device(sfh_multifit* mf, const double* d_X, double eps, double eps_grad,
+                           double* d_negLL, double* d_G);
 
 #define SFH_FIT_LOG 0    /* fit_templates: log(coeffs), MAP (Jacobian -sum log x) then MLE */
 #define SFH_FIT_SQRT 1   /* fit_templates_fast: coeffs = theta^2, MLE only */
@@ -217,7 +225,10 @@ typedef struct {
     int steps_per_graph;     /* optimiser steps per captured HIP graph (1..256) */
     double g_abstol;         /* Optim.Options g_abstol (default 1e-8), on the transformed variables */
     int64_t max_iterations;  /* per stage (Optim.Options iterations, default 1000) */
-    double eps;              /* the likelihood's clamp, eps(Float64) by default */
+    double eps;              /* the log-likelihood's clamp, eps(promote_type(models, data));
+                                eps(Float64) by default */
+    double eps_grad;         /* the gradient's clamp, eps(eltype(composite)) = eps(eltype(models))
+                                in fit_templates (solvers.jl:173); eps(Float64) by default */
 } sfh_fit_options;
 
 /* Fills *opt with the defaults above (LOG, BFGS, memory 10, 16 steps per graph). */
@@ -385,6 +396,73 @@ int sfh_bin_cmd(sfh_ctx* ctx, int64_t n, const double* colors, const double* mag
                 sfh_range xedges, sfh_range yedges, double* out);
 int sfh_bin_cmd_catalog(sfh_ctx* ctx, const sfh_catalog* c, int32_t color_a, int32_t color_b, int32_t y_filter,
                         sfh_range xedges, sfh_range yedges, double* out);
+
+/* ------------------------------ P1: partial_cmd_smooth, prepared on the device */
+
+/* A scalar callable of the template path -- a magnitude -> error, completeness
+ * or bias function, or the IMF's mass -> pdf -- in a form the device
+ * evaluates (FP64):
+ *   SFH_FN_CONST        p[0]
+ *   SFH_FN_EXP_PHOTERR  exp_photerr(m, a=p[0], b=p[1], c=p[2], d=p[3]) = a^(b (m - c)) + d
+ *                       (src/utilities.jl:218)
+ *   SFH_FN_MARTIN2016   Martin2016_complete(m, A=p[0], m50=p[1], rho=p[2]) =
+ *                       A / (1 + exp((m - m50) / rho))  (src/utilities.jl:205)
+ *   SFH_FN_TABLE        `table`, linearly interpolated and clamped at its ends
+ *                       (any other callable, tabulated by the caller)
+ *   SFH_FN_POWERLAW     p[0] coefs[k] m^-slopes[k] on [breaks[k], breaks[k+1])
+ *                       of `powerlaw`, the last segment closed, 0 outside (an IMF pdf
+ *                       with normalisation p[0], e.g. Kroupa2001, Salpeter1955) */
+#define SFH_FN_CONST 0
+#define SFH_FN_EXP_PHOTERR 1
+#define SFH_FN_MARTIN2016 2
+#define SFH_FN_TABLE 3
+#define SFH_FN_POWERLAW 4
+typedef struct sfh_fn {
+    int32_t kind;
+    double p[4];
+    sfh_table table;
+    sfh_imf powerlaw;
+} sfh_fn;
+
+/* partial_cmd_smooth(m_ini, mags, mag_err_funcs, y_index, color_indices, imf,
+ * completeness_funcs, bias_funcs; dmod, normalize_value, mean_mass, edges)
+ * -- src/StarFormationHistories.jl:829-893 (binary_model = NoBinaries()) -- for
+ * ntemplates isochrones in one call, from host memory: isochrone t owns points
+ * offsets[t] .. offsets[t+1]-1 of m_ini and of each filter's magnitude array
+ * mags[f] (nfilters host pointers), and has its own dmod, normalize_value and
+ * mean_mass.  y_index, color_a, color_b are 0-based filter indices (the
+ * reference's y_index and color_indices).  mag_err holds nfilters callables;
+ * completeness / bias hold nfilters callables or are NULL (one / zero, the
+ * reference's defaults).  The host preparation -- mini_spacing (:110-148),
+ * interpolate_mini (:77-81), the callables, calculate_weights (:765-776),
+ * midpoints (:417-431) -- runs on the device in FP64, then the kernels are
+ * deposited as sfh_bin_cmd_smooth does (FP32 templates).  m_ini must be
+ * strictly increasing per isochrone (sort_ingested is the caller's), and each
+ * isochrone has 0 or >= 2 points.  correlation: 0 gives the reference's
+ * cov_mult; r in (-1, 1) instead gives each covariant kernel the per-point
+ * multiplier c = -r / sqrt(1 - r^2) * color_err / mag_err (a colour-magnitude
+ * correlation r; no reference counterpart).  Outputs: out (host, nbins x
+ * ntemplates column-major, nbins = nx*ny, colour bin fastest) and/or d_out
+ * (device, column t at d_out + t*ld_out); *npoints_out / *pairs_out (may be
+ * NULL) receive the kernel points and (point, bin) pairs deposited. */
+int sfh_partial_cmd_smooth_batch(sfh_ctx* ctx, int64_t ntemplates, const int64_t* offsets, const double* m_ini,
+                                 int32_t nfilters, const double* const* mags, int32_t y_index, int32_t color_a,
+                                 int32_t color_b, const sfh_fn* mag_err, const sfh_fn* completeness,
+                                 const sfh_fn* bias, const sfh_fn* imf, const double* dmod,
+                                 const double* normalize_value, const double* mean_mass, double correlation,
+                                 sfh_range xedges, sfh_range yedges, float* out, float* d_out, int64_t ld_out,
+                                 int64_t* npoints_out, int64_t* pairs_out);
+
+/* ------------------------------------------------ device buffers (no reference counterpart) */
+
+/* Device memory on the context's device for callers without a device array
+ * type of their own (a Julia ccall binding): the *_device entry points and
+ * the sfh_nn_* primitives take these pointers.  Copies are ordered on the
+ * context's stream and return when complete. */
+int sfh_buffer_alloc(sfh_ctx* ctx, int64_t bytes, void** d_ptr);
+int sfh_buffer_free(sfh_ctx* ctx, void* d_ptr);
+int sfh_buffer_upload(sfh_ctx* ctx, void* d_dst, const void* h_src, int64_t bytes);
+int sfh_buffer_download(sfh_ctx* ctx, void* h_dst, const void* d_src, int64_t bytes);
 
 /* ------------------------------- learned template surrogate primitives (new) */
 

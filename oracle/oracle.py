@@ -68,9 +68,11 @@ def lib():
         L.orc_grad_single_f64.restype = C.c_double
         L.orc_grad_inplace_f32.argtypes = [_dp, _dp, _fp, _dp, C.c_int64, C.c_int64, C.c_double, C.c_int]
         L.orc_grad_inplace_f64.argtypes = [_dp, _dp, _dp, _dp, C.c_int64, C.c_int64, C.c_double, C.c_int]
-        L.orc_fg_f32.argtypes = [C.c_int, _dp, _dp, _fp, _dp, _dp, C.c_int64, C.c_int64, C.c_double, C.c_int]
+        L.orc_fg_f32.argtypes = [C.c_int, _dp, _dp, _fp, _dp, _dp, C.c_int64, C.c_int64, C.c_double, C.c_double,
+                                 C.c_int]
         L.orc_fg_f32.restype = C.c_double
-        L.orc_fg_batched_f32.argtypes = [_fp, _dp, _dp, C.c_int64, C.c_int64, C.c_int64, C.c_double, _dp, _dp, C.c_int]
+        L.orc_fg_batched_f32.argtypes = [_fp, _dp, _dp, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_double,
+                                         _dp, _dp, C.c_int]
         L.orc_histogram_pix.argtypes = [C.c_double, Range]
         L.orc_histogram_pix.restype = C.c_double
         L.orc_histogram_data.argtypes = [C.c_double, Range]
@@ -162,9 +164,14 @@ def grad_inplace(G, comp, models, data, eps: float = np.finfo(np.float64).eps, n
     return G
 
 
-def fg(models, data, coeffs, want_F=True, want_G=True, eps=np.finfo(np.float64).eps, nthreads=1):
+def fg(models, data, coeffs, want_F=True, want_G=True, eps=np.finfo(np.float64).eps, nthreads=1,
+       eps_grad=None):
     """fg!(F, G, coeffs, models, data, composite) (src/fitting/solvers.jl:20-38),
-    for float32 models.  Returns (negLL or None, G or None, composite-after)."""
+    for float32 models.  Returns (negLL or None, G or None, composite-after).
+    ``eps`` clamps the log-likelihood (eps of the promoted type,
+    fitting_base.jl:86), ``eps_grad`` the gradient's R (eps(eltype(composite)),
+    :277); it defaults to ``eps``."""
+    eps_grad = eps if eps_grad is None else eps_grad
     m = np.asfortranarray(models, dtype=np.float32)
     N, T = m.shape
     comp = np.empty(N, dtype=np.float64)
@@ -172,12 +179,13 @@ def fg(models, data, coeffs, want_F=True, want_G=True, eps=np.finfo(np.float64).
     c = _f64(coeffs)
     dt = _f64(data)
     r = lib().orc_fg_f32(1 if want_F else 0, _d(G) if G is not None else None, _d(c), _f(m),
-                         _d(dt), _d(comp), N, T, eps, nthreads)
+                         _d(dt), _d(comp), N, T, eps, eps_grad, nthreads)
     return (float(r) if want_F else None), G, comp
 
 
-def fg_batched(models, data, X, eps=np.finfo(np.float64).eps, want_G=True, nthreads=1):
+def fg_batched(models, data, X, eps=np.finfo(np.float64).eps, want_G=True, nthreads=1, eps_grad=None):
     """B independent fg! calls, one per column of X (T x B)."""
+    eps_grad = eps if eps_grad is None else eps_grad
     m = np.asfortranarray(models, dtype=np.float32)
     N, T = m.shape
     Xf = np.asfortranarray(X, dtype=np.float64)
@@ -185,7 +193,7 @@ def fg_batched(models, data, X, eps=np.finfo(np.float64).eps, want_G=True, nthre
     B = Xf.shape[1]
     negll = np.empty(B, dtype=np.float64)
     G = np.empty((T, B), dtype=np.float64, order="F") if want_G else None
-    lib().orc_fg_batched_f32(_f(m), _d(_f64(data)), _d(Xf), B, N, T, eps, _d(negll),
+    lib().orc_fg_batched_f32(_f(m), _d(_f64(data)), _d(Xf), B, N, T, eps, eps_grad, _d(negll),
                              _d(G) if G is not None else None, nthreads)
     return negll, G
 

@@ -119,17 +119,20 @@ ORC_API void orc_grad_inplace_f64(double* G, double* composite, const double* mo
 /* fg!(F, G, coeffs, models, data, composite) -- src/fitting/solvers.jl:20-38.
  * want_F / G!=NULL select the branch.  Returns -logL when F is wanted (logL
  * computed before ∇loglikelihood! clobbers composite, :27-29), else 0.
- * G receives the gradient of -logL (G .*= -1, :30). */
+ * G receives the gradient of -logL (G .*= -1, :30).  The two clamps differ in
+ * type: loglikelihood uses eps(promote_type(composite, data))
+ * (fitting_base.jl:86,90) -> eps; ∇loglikelihood! uses eps(eltype(composite))
+ * (:277) -> eps_grad. */
 ORC_API double orc_fg_f32(int want_F, double* G, const double* coeffs,
                           const float* models, const double* data,
                           double* composite, int64_t N, int64_t T, double eps,
-                          int nthreads)
+                          double eps_grad, int nthreads)
 {
     orc_composite_f32(models, coeffs, N, T, composite, nthreads);
     double negll = 0.0;
     if (want_F) negll = -orc_loglikelihood(composite, data, N, eps);
     if (G) {
-        orc_grad_inplace_f32(G, composite, models, data, N, T, eps, nthreads);
+        orc_grad_inplace_f32(G, composite, models, data, N, T, eps_grad, nthreads);
         for (int64_t j = 0; j < T; ++j) G[j] = -G[j];
     }
     return negll;
@@ -140,12 +143,13 @@ ORC_API double orc_fg_f32(int want_F, double* G, const double* coeffs,
  * src/fitting/mcmc_sample.jl:12-23).  negLL[b] = -logL, G[:,b] = -∇logL. */
 ORC_API void orc_fg_batched_f32(const float* models, const double* data,
                                 const double* X, int64_t B, int64_t N, int64_t T,
-                                double eps, double* negLL, double* G, int nthreads)
+                                double eps, double eps_grad, double* negLL, double* G,
+                                int nthreads)
 {
     double* comp = (double*)malloc((size_t)N * sizeof(double));
     for (int64_t b = 0; b < B; ++b) {
         double nl = orc_fg_f32(1, G ? G + b * T : NULL, X + b * T, models, data,
-                               comp, N, T, eps, nthreads);
+                               comp, N, T, eps, eps_grad, nthreads);
         if (negLL) negLL[b] = nl;
     }
     free(comp);

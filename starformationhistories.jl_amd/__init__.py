@@ -12,7 +12,7 @@ from . import _lib
 _lib.load()
 
 from ._lib import ArgumentError, DeviceError, DomainError, InexactError  # noqa: E402
-from .core import (Context, HIPData, HIPTemplates, composite_, default_context, fg_,  # noqa: E402
+from .core import (Context, HIPData, HIPTemplates, composite_, default_context, driver_clamps, fg_,  # noqa: E402
                    fg_batched, fg_batched_device, fg_device, grad_loglikelihood,
                    grad_loglikelihood_, loglikelihood, stack_models)
 from .fitting import (HMCModel, LogTransformFTResult, MCMCModel, construct_x0,  # noqa: E402
@@ -32,11 +32,12 @@ from .templates import (Histogram, NoBinaries, PointBatch, RandomBinaryPairs, Ra
                         calculate_weights, deposit_batch, edges_range, exp_photerr,
                         histogram_data, histogram_pix, interpolate_mini, Martin2016_complete,
                         midpoints, mini_spacing, partial_cmd_smooth, partial_cmd_smooth_points,
-                        templates_from_device)
+                        templates_from_device, partial_cmd_smooth_batch, ExpPhotErr, Martin2016Completeness,
+                        Constant, Tabulated)
 
 __all__ = [
     "ArgumentError", "DeviceError", "InexactError", "Context", "HIPData", "HIPTemplates",
-    "composite_", "default_context", "fg_", "fg_batched", "fg_batched_device", "fg_device",
+    "composite_", "default_context", "driver_clamps", "fg_", "fg_batched", "fg_batched_device", "fg_device",
     "grad_loglikelihood", "grad_loglikelihood_", "loglikelihood", "stack_models", "Histogram",
     "PointBatch", "Range", "bin_cmd_smooth", "calculate_edges", "calculate_weights",
     "deposit_batch", "edges_range", "exp_photerr", "histogram_data", "histogram_pix",
@@ -49,5 +50,6 @@ __all__ = [
     "BinaryMassRatio", "PowerLawIMF", "Salpeter1955", "bin_cmd", "generate_stars_mag",
     "generate_stars_mag_composite", "generate_stars_mass", "generate_stars_mass_composite", "ingest_mags",
     "mass_limits", "model_cmd", "recycling_fraction", "recycling_mass_fraction", "sort_ingested",
-    "surviving_fraction", "surviving_mass_fraction",
+    "surviving_fraction", "surviving_mass_fraction", "partial_cmd_smooth_batch", "ExpPhotErr",
+    "Martin2016Completeness", "Constant", "Tabulated",
 ]

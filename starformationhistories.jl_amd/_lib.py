@@ -50,7 +50,8 @@ class sfh_table(C.Structure):
 
 class sfh_fit_options(C.Structure):
     _fields_ = [("transform", C.c_int), ("method", C.c_int), ("memory", C.c_int), ("steps_per_graph", C.c_int),
-                ("g_abstol", C.c_double), ("max_iterations", C.c_int64), ("eps", C.c_double)]
+                ("g_abstol", C.c_double), ("max_iterations", C.c_int64), ("eps", C.c_double),
+                ("eps_grad", C.c_double)]
 
 
 class sfh_fit_results(C.Structure):
@@ -68,6 +69,17 @@ class sfh_nn_wcopy(C.Structure):
 
 class sfh_imf(C.Structure):
     _fields_ = [("nseg", C.c_int32), ("breaks", C.c_void_p), ("slopes", C.c_void_p), ("coefs", C.c_void_p)]
+
+
+class sfh_fn(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("p", C.c_double * 4), ("table", sfh_table), ("powerlaw", sfh_imf)]
+
+
+SFH_FN_CONST = 0
+SFH_FN_EXP_PHOTERR = 1
+SFH_FN_MARTIN2016 = 2
+SFH_FN_TABLE = 3
+SFH_FN_POWERLAW = 4
 
 
 class sfh_binary_model(C.Structure):
@@ -111,17 +123,17 @@ SIGNATURES = [
     ("sfh_loglikelihood_coeffs", _int, [_vp, _vp, _vp, _vp, _dbl, _pd]),
     ("sfh_grad_loglikelihood", _int, [_vp, _vp, _vp, _vp, _dbl, _vp]),
     ("sfh_grad_loglikelihood_inplace", _int, [_vp, _vp, _vp, _vp, _dbl, _vp]),
-    ("sfh_fg", _int, [_vp, _vp, _vp, _vp, _int, _dbl, _pd, _vp, _vp]),
-    ("sfh_fg_device", _int, [_vp, _vp, _vp, _vp, _int, _dbl, _vp, _vp, _vp]),
-    ("sfh_fg_batched", _int, [_vp, _vp, _vp, _vp, _i64, _int, _dbl, _vp, _vp]),
-    ("sfh_fg_batched_device", _int, [_vp, _vp, _vp, _vp, _i64, _int, _dbl, _vp, _vp]),
+    ("sfh_fg", _int, [_vp, _vp, _vp, _vp, _int, _dbl, _dbl, _pd, _vp, _vp]),
+    ("sfh_fg_device", _int, [_vp, _vp, _vp, _vp, _int, _dbl, _dbl, _vp, _vp, _vp]),
+    ("sfh_fg_batched", _int, [_vp, _vp, _vp, _vp, _i64, _int, _dbl, _dbl, _vp, _vp]),
+    ("sfh_fg_batched_device", _int, [_vp, _vp, _vp, _vp, _i64, _int, _dbl, _dbl, _vp, _vp]),
     ("sfh_multifit_create", _int, [_vp, _i64, _vp, _vp, C.POINTER(_vp)]),
     ("sfh_multifit_destroy", _int, [_vp]),
     ("sfh_multifit_get_shape", _int, [_vp, _p64, C.POINTER(_int), C.POINTER(_int)]),
     ("sfh_multifit_set_shape", _int, [_vp, _i64, _int, _int]),
     ("sfh_multifit_autotune", _int, [_vp, _int, _p64, C.POINTER(_int), C.POINTER(_int)]),
-    ("sfh_multifit_fg", _int, [_vp, _vp, _dbl, _vp, _vp]),
-    ("sfh_multifit_fg_device", _int, [_vp, _vp, _dbl, _vp, _vp]),
+    ("sfh_multifit_fg", _int, [_vp, _vp, _dbl, _dbl, _vp, _vp]),
+    ("sfh_multifit_fg_device", _int, [_vp, _vp, _dbl, _dbl, _vp, _vp]),
     ("sfh_fit_options_default", _int, [C.POINTER(sfh_fit_options)]),
     ("sfh_multifit_fit", _int, [_vp, _vp, C.POINTER(sfh_fit_options), C.POINTER(sfh_fit_results)]),
     ("sfh_binary_pairs_hist", _int,
@@ -179,6 +191,13 @@ SIGNATURES = [
     ("sfh_nn_template_kl", _int, [_vp, _vp, _i64, _vp, _i64, _i64, _i64, C.c_double, _vp]),
     ("sfh_nn_adam", _int, [_vp, _vp, _vp, _vp, _vp, _i64, C.c_float, C.c_float, C.c_float, C.c_float, _i64,
                            C.c_float]),
+    ("sfh_partial_cmd_smooth_batch", _int,
+     [_vp, _i64, _vp, _vp, C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int32, _vp, _vp, _vp, C.POINTER(sfh_fn), _vp,
+      _vp, _vp, _dbl, sfh_range, sfh_range, _vp, _vp, _i64, _p64, _p64]),
+    ("sfh_buffer_alloc", _int, [_vp, _i64, C.POINTER(_vp)]),
+    ("sfh_buffer_free", _int, [_vp, _vp]),
+    ("sfh_buffer_upload", _int, [_vp, _vp, _vp, _i64]),
+    ("sfh_buffer_download", _int, [_vp, _vp, _vp, _i64]),
     ("sfh_bin_cmd_smooth_batch_device", _int,
      [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, sfh_range, sfh_range, _vp, _i64, _p64]),
 ]

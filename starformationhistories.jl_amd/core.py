@@ -117,6 +117,7 @@ class HIPTemplates:
         else:
             if isinstance(models, (list, tuple)):
                 models = stack_models(models)
+            src_dtype = np.asarray(models).dtype
             m = np.asfortranarray(models, dtype=np.float32)
             if m.ndim != 2:
                 raise ArgumentError("models must be a 2-D (nbins, ntemplates) array")
@@ -125,6 +126,13 @@ class HIPTemplates:
                                                     C.byref(h)))
         self.handle = h
         self.shape = (int(nbins), int(ntpl))
+        # The element type of the caller's models: the reference's drivers
+        # allocate their composite with it (fit_templates, solvers.jl:173;
+        # hmc_sample, hmc_sample.jl:107), which fixes the gradient's clamp
+        # eps(eltype(composite)).  Device-pointer templates are FP32.
+        self.eltype = np.dtype(np.float32)
+        if device_ptr is None:
+            self.eltype = np.dtype(np.float32) if src_dtype == np.float32 else np.dtype(np.float64)
 
     @property
     def nbins(self):
@@ -194,13 +202,36 @@ def _vec64(a, n=None, name="array"):
     return v
 
 
+def _dtype_of(a):
+    """Element type of a host array, a dtype, or a handle (HIPTemplates: the
+    caller's models' type; HIPData: FP64 counts)."""
+    if isinstance(a, HIPTemplates):
+        return a.eltype
+    if isinstance(a, HIPData):
+        return np.dtype(np.float64)
+    if isinstance(a, np.dtype):
+        return a
+    return np.asarray(a).dtype
+
+
 def _eps_of(*arrays) -> float:
-    """eps(promote_type(...)) of the reference's clamp (fitting_base.jl:86)."""
-    kinds = [np.asarray(a).dtype for a in arrays if a is not None and not isinstance(a, (HIPData, HIPTemplates))]
+    """eps(promote_type(...)) of the reference's clamps (fitting_base.jl:86,148):
+    eps(Float32) when every element type is Float32 or an integer (with at
+    least one Float32), else eps(Float64)."""
+    kinds = [_dtype_of(a) for a in arrays if a is not None]
     if kinds and all(k in (np.float32, np.float16) or np.issubdtype(k, np.integer) for k in kinds) \
             and any(k == np.float32 for k in kinds):
         return float(np.finfo(np.float32).eps)
     return _f64eps
+
+
+def driver_clamps(models: "HIPTemplates", data) -> tuple:
+    """(eps, eps_grad) of fg! as the reference's drivers call it: their
+    composite is a Vector of the models' element type S (fit_templates,
+    solvers.jl:173; hmc_sample, hmc_sample.jl:107), so the log-likelihood
+    clamps with eps(promote_type(S, eltype(data))) (fitting_base.jl:86) and the
+    gradient with eps(S) (:277)."""
+    return _eps_of(models, data), _eps_of(models.eltype)
 
 
 def composite_(composite: np.ndarray, coeffs, models: HIPTemplates):
@@ -242,8 +273,9 @@ def loglikelihood(*args, eps: float | None = None):
         d = _as_data(data, models)
         if d.nbins != models.nbins:
             raise ArgumentError("axes(data,1) == axes(models,1)")
+        # composite = Vector{S}, S = promote_type(coeffs, models, data) (:121)
+        e = _eps_of(coeffs, models, data) if eps is None else eps
         c = _vec64(coeffs, models.ntemplates, "coeffs")
-        e = _f64eps if eps is None else eps
         out = C.c_double()
         models.ctx.check(models.ctx.lib.sfh_loglikelihood_coeffs(models.ctx.handle, models.handle, d.handle,
                                                                  c.ctypes.data, e, C.byref(out)))
@@ -257,16 +289,20 @@ def grad_loglikelihood(*args, eps: float | None = None) -> np.ndarray:
     if len(args) != 3:
         raise TypeError("grad_loglikelihood takes 3 arguments")
     a, b, c = args
-    e = _f64eps if eps is None else eps
     if isinstance(a, HIPTemplates):
         models, composite, data = a, b, c
+        comp_dtype = _dtype_of(composite)
         comp = _vec64(composite, models.nbins, "composite")
     elif isinstance(b, HIPTemplates):
         coeffs, models, data = a, b, c
+        # composite = Vector{S}, S = promote_type(coeffs, models, data) (:207)
+        comp_dtype = np.result_type(_dtype_of(coeffs), models.eltype, _dtype_of(data))
         comp = np.empty(models.nbins, dtype=np.float64)
         composite_(comp, coeffs, models)
     else:
         raise TypeError("one argument must be a HIPTemplates")
+    # eps(promote_type(eltype(model), eltype(composite), eltype(data))) (:148)
+    e = _eps_of(models, comp_dtype, data) if eps is None else eps
     d = _as_data(data, models)
     if d.nbins != models.nbins:
         raise ArgumentError("axes(composite,1) == axes(data,1) == axes(models,1)")
@@ -279,7 +315,8 @@ def grad_loglikelihood(*args, eps: float | None = None) -> np.ndarray:
 def grad_loglikelihood_(G: np.ndarray, composite: np.ndarray, models: HIPTemplates, data,
                         eps: float | None = None):
     """∇loglikelihood!(G, composite, models, data) -- src/fitting/fitting_base.jl:265-285.
-    Fills G and overwrites ``composite`` with 1 - n/max(m, eps) (:274-280)."""
+    Fills G and overwrites ``composite`` with 1 - n/max(m, eps) (:274-280),
+    eps = eps(eltype(composite)) (:277) unless given."""
     if not isinstance(models, HIPTemplates):
         raise TypeError("models must be a HIPTemplates")
     if G.size != models.ntemplates:
@@ -287,7 +324,7 @@ def grad_loglikelihood_(G: np.ndarray, composite: np.ndarray, models: HIPTemplat
     d = _as_data(data, models)
     if composite.size != models.nbins or d.nbins != models.nbins:
         raise ArgumentError("axes(models,1) == axes(data,1) == axes(composite,1)")
-    e = _f64eps if eps is None else eps
+    e = _eps_of(_dtype_of(composite)) if eps is None else eps
     comp = _vec64(composite)
     g = np.empty(models.ntemplates, dtype=np.float64)
     models.ctx.check(models.ctx.lib.sfh_grad_loglikelihood_inplace(models.ctx.handle, models.handle,
@@ -297,10 +334,18 @@ def grad_loglikelihood_(G: np.ndarray, composite: np.ndarray, models: HIPTemplat
     return G
 
 
-def fg_(F, G, coeffs, models: HIPTemplates, data, composite=None, eps: float | None = None):
+def fg_(F, G, coeffs, models: HIPTemplates, data, composite=None, eps: float | None = None,
+        eps_grad: float | None = None):
     """fg!(F, G, coeffs, models, data, composite) -- src/fitting/solvers.jl:20-38.
     Returns -logL when F is not None; fills G (if not None) with -∇logL;
-    ``composite`` (optional) is left as the reference leaves it."""
+    ``composite`` (optional) is left as the reference leaves it.
+
+    The reference's two clamps follow the composite's element type: the
+    log-likelihood uses eps(promote_type(eltype(composite), eltype(data)))
+    (fitting_base.jl:86) and ∇loglikelihood! eps(eltype(composite)) (:277).
+    With a host ``composite`` both default to those; without one the composite
+    is taken as FP64 (both eps(Float64)); the drivers pass
+    ``driver_clamps(models, data)``."""
     if not isinstance(models, HIPTemplates):
         raise TypeError("models must be a HIPTemplates")
     N, T = models.shape
@@ -312,7 +357,12 @@ def fg_(F, G, coeffs, models: HIPTemplates, data, composite=None, eps: float | N
         raise ArgumentError("axes(models,1) == axes(data,1) == axes(composite,1)")
     if G is not None and G.size != T:
         raise ArgumentError("axes(G,1) == axes(models,2)")
-    e = _f64eps if eps is None else eps
+    if composite is not None:
+        e = _eps_of(composite, data) if eps is None else eps
+        eg = _eps_of(_dtype_of(composite)) if eps_grad is None else eps_grad
+    else:
+        e = _f64eps if eps is None else eps
+        eg = e if eps_grad is None else eps_grad
     want_F = F is not None
     if not want_F and G is None:
         if composite is not None:
@@ -322,7 +372,7 @@ def fg_(F, G, coeffs, models: HIPTemplates, data, composite=None, eps: float | N
     g = np.empty(T, dtype=np.float64) if G is not None else None
     comp = np.empty(N, dtype=np.float64) if composite is not None else None
     models.ctx.check(models.ctx.lib.sfh_fg(models.ctx.handle, models.handle, d.handle, c.ctypes.data,
-                                           1 if want_F else 0, e, C.byref(negll),
+                                           1 if want_F else 0, e, eg, C.byref(negll),
                                            g.ctypes.data if g is not None else None,
                                            comp.ctypes.data if comp is not None else None))
     if G is not None:
@@ -335,18 +385,23 @@ def fg_(F, G, coeffs, models: HIPTemplates, data, composite=None, eps: float | N
 
 
 def fg_device(models: HIPTemplates, data: HIPData, d_coeffs, d_negLL=None, d_G=None, d_composite=None,
-              want_F: bool = True, eps: float = _f64eps):
-    """Asynchronous fg! on device buffers (torch tensors or raw pointers)."""
+              want_F: bool = True, eps: float = _f64eps, eps_grad: float | None = None):
+    """Asynchronous fg! on device buffers (torch tensors or raw pointers);
+    ``eps_grad`` defaults to ``eps``."""
+    eg = eps if eps_grad is None else eps_grad
     models.ctx.check(models.ctx.lib.sfh_fg_device(models.ctx.handle, models.handle, data.handle,
-                                                  _ptr(d_coeffs), 1 if want_F else 0, eps, _ptr(d_negLL),
+                                                  _ptr(d_coeffs), 1 if want_F else 0, eps, eg, _ptr(d_negLL),
                                                   _ptr(d_G), _ptr(d_composite)))
 
 
+
 def fg_batched(X, models: HIPTemplates, data, precision: str = "f64", want_G: bool = True,
-               eps: float = _f64eps):
+               eps: float = _f64eps, eps_grad: float | None = None):
     """B independent fg! evaluations (columns of X, ntemplates x B) -- the
     HMC / ensemble call pattern (src/fitting/hmc_sample.jl:24-37,
-    src/fitting/mcmc_sample.jl:12-23).  Returns (negLL[B], G[T,B] or None)."""
+    src/fitting/mcmc_sample.jl:12-23).  Returns (negLL[B], G[T,B] or None).
+    ``eps_grad`` (the gradient's clamp) defaults to ``eps``."""
+    eg = eps if eps_grad is None else eps_grad
     T = models.ntemplates
     Xf = np.asfortranarray(X, dtype=np.float64)
     if Xf.ndim != 2 or Xf.shape[0] != T:
@@ -356,14 +411,15 @@ def fg_batched(X, models: HIPTemplates, data, precision: str = "f64", want_G: bo
     negll = np.empty(B, dtype=np.float64)
     G = np.empty((T, B), dtype=np.float64, order="F") if want_G else None
     models.ctx.check(models.ctx.lib.sfh_fg_batched(models.ctx.handle, models.handle, d.handle, Xf.ctypes.data,
-                                                   B, _lib.PRECISIONS[precision], eps, negll.ctypes.data,
+                                                   B, _lib.PRECISIONS[precision], eps, eg, negll.ctypes.data,
                                                    G.ctypes.data if G is not None else None))
     return negll, G
 
 
 def fg_batched_device(models: HIPTemplates, data: HIPData, d_X, B: int, d_negLL, d_G=None,
-                      precision: str = "f64", eps: float = _f64eps):
+                      precision: str = "f64", eps: float = _f64eps, eps_grad: float | None = None):
     """Asynchronous batched fg! on device buffers (X: T x B column-major FP64)."""
+    eg = eps if eps_grad is None else eps_grad
     models.ctx.check(models.ctx.lib.sfh_fg_batched_device(models.ctx.handle, models.handle, data.handle,
-                                                          _ptr(d_X), int(B), _lib.PRECISIONS[precision], eps,
+                                                          _ptr(d_X), int(B), _lib.PRECISIONS[precision], eps, eg,
                                                           _ptr(d_negLL), _ptr(d_G)))

@@ -182,6 +182,7 @@ __device__ inline void ll_term(double n, double c, double& ll, float& r)
     r = (float)(1.0 - q);
 }
 
+
 // ------------------------------------------------------------ layout builders
 
 // Per-template power-of-two scale: e_t = exponent of max_i |C[i,t]| (0 if the
@@ -409,12 +410,24 @@ struct EpiLL {
     const double* data;      // n_i
     int64_t nbins;
     int64_t B;
-    double eps;
+    double eps;              // LL clamp: eps(promote_type(composite, data))
+    double eps_g;            // R clamp: eps(eltype(composite))
+    double eps_hi;           // max(eps, eps_g): below it R is recomputed with eps_g
     double* ll_part;         // [row tile][Bpad]
     uint4* R;                // gradient B operand tiles [kb2][cb][2][64]
     int64_t ncb;             // column blocks (Bpad/32)
     int64_t Bpad;
 };
+
+// The LL and R of one (bin, walker) with the reference's two clamps: LL with
+// eps of the promoted type (:86), R with eps(eltype(composite)) (:277).  They
+// differ only where m < max(eps, eps_g); there R is recomputed (the same
+// quotient when the two are equal).
+__device__ inline void ll_r_term(const EpiLL& el, double n, double m, double& ll, float& r)
+{
+    ll_term(n, m > el.eps ? m : el.eps, ll, r);
+    if (m < el.eps_hi) r = (float)(1.0 - quot(n, m > el.eps_g ? m : el.eps_g));
+}
 
 struct EpiGrad {
     float* gpart;            // [split][Bpad][Tpad2]
@@ -588,7 +601,7 @@ __global__ __launch_bounds__(WR * WC * 64, 1) void k_gemm(const uint4* __restric
                     float r = 0.f;
                     if (bin < el.nbins && wok) {
                         const double m = acc64[a][b][i];
-                        ll_term(el.data[bin], m > el.eps ? m : el.eps, part[b], r);
+                        ll_r_term(el, el.data[bin], m, part[b], r);
                     }
                     rv[i] = r;
                 }
@@ -806,7 +819,7 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm_f64_ll(const uint4* __restrict_
                 float r = 0.f;
                 if (bin < el.nbins && w < el.B) {
                     const double m = acc[a][b][i];
-                    ll_term(el.data[bin], m > el.eps ? m : el.eps, part[b], r);
+                    ll_r_term(el, el.data[bin], m, part[b], r);
                 }
                 sR[rl * kBN + cl] = r;
             }
@@ -980,7 +993,8 @@ static unsigned swizzle_grid(const Swizzle& s)
 }
 
 static int fg_batched_impl(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data, const double* d_X,
-                           int64_t B, int precision, double eps, double* d_negLL, double* d_G)
+                           int64_t B, int precision, double eps, double eps_g, double* d_negLL,
+                           double* d_G)
 {
     SFH_ARG_CHECK(ctx, tpl && data && d_X && d_negLL, "null pointer");
     SFH_ARG_CHECK(ctx, tpl->nbins == data->nbins, "axes(models,1) == axes(data,1)");
@@ -1021,7 +1035,7 @@ static int fg_batched_impl(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_dat
     // composite GEMM + log-likelihood / R epilogue
     {
         const Swizzle sw = make_swizzle((int)nrt1, (int)nct, 4, 8);
-        EpiLL el{data->d_data, N, B, eps, (double*)pll, (uint4*)pr, ncb, Bpad};
+        EpiLL el{data->d_data, N, B, eps, eps_g, eps > eps_g ? eps : eps_g, (double*)pll, (uint4*)pr, ncb, Bpad};
         EpiGrad eg{};
         const dim3 grid(swizzle_grid(sw), 1);
         TimedLaunch tl(ctx, "batched_composite_ll");
@@ -1089,16 +1103,16 @@ using namespace sfh;
 extern "C" {
 
 int sfh_fg_batched_device(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data, const double* d_X,
-                          int64_t B, int precision, double eps, double* d_negLL, double* d_G)
+                          int64_t B, int precision, double eps, double eps_grad, double* d_negLL, double* d_G)
 {
     if (!ctx) return SFH_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (int rc = bind(ctx)) return rc;
-    return fg_batched_impl(ctx, tpl, data, d_X, B, precision, eps, d_negLL, d_G);
+    return fg_batched_impl(ctx, tpl, data, d_X, B, precision, eps, eps_grad, d_negLL, d_G);
 }
 
 int sfh_fg_batched(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data, const double* X, int64_t B,
-                   int precision, double eps, double* negLL, double* G)
+                   int precision, double eps, double eps_grad, double* negLL, double* G)
 {
     if (!ctx) return SFH_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
@@ -1112,7 +1126,7 @@ int sfh_fg_batched(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
     if (G)
         if (int rc = scratch(ctx, "batx_g", (size_t)T * B * sizeof(double), &pg)) return rc;
     SFH_HIP_TRY(ctx, hipMemcpyAsync(px, X, (size_t)T * B * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    if (int rc = fg_batched_impl(ctx, tpl, data, (const double*)px, B, precision, eps, (double*)pn, (double*)pg))
+    if (int rc = fg_batched_impl(ctx, tpl, data, (const double*)px, B, precision, eps, eps_grad, (double*)pn, (double*)pg))
         return rc;
     SFH_HIP_TRY(ctx, hipMemcpyAsync(negLL, pn, (size_t)B * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (G)

@@ -205,12 +205,14 @@ __device__ inline double block_sum(double v, double* sh)
 
 // Per bin: m = sum over splits (fixed order).  Then, for bins < nbins:
 //   LL term (fitting_base.jl:90-92) accumulated into one FP64 partial per block;
-//   R = 1 - n/max(m,eps) (fitting_base.jl:277-279) written to R (zero for the
-//   padding rows so the gradient pass reads finite values);
+//   R = 1 - n/max(m,eps_g) (fitting_base.jl:277-279) written to R (zero for
+//   the padding rows so the gradient pass reads finite values); eps clamps the
+//   LL with eps of the promoted type (:86), eps_g the gradient with
+//   eps(eltype(composite)) (:277);
 //   composite_out gets m (or R when clobber_with_R, the ∇loglikelihood! contract).
 __global__ __launch_bounds__(kBlock) void k_composite_finish(
     const double* __restrict__ part, int64_t ld, int nsplit, int64_t nbins,
-    const double* __restrict__ data, double eps, int want_ll, double* __restrict__ ll_part,
+    const double* __restrict__ data, double eps, double eps_g, int want_ll, double* __restrict__ ll_part,
     double* __restrict__ R, double* __restrict__ composite_out, int clobber_with_R)
 {
     __shared__ double sh[kBlock];
@@ -221,9 +223,10 @@ __global__ __launch_bounds__(kBlock) void k_composite_finish(
         for (int s = 0; s < nsplit; ++s) m += part[(int64_t)s * ld + i];
         if (i < nbins) {
             const double c = m > eps ? m : eps;
+            const double cg = m > eps_g ? m : eps_g;
             const double n = data ? data[i] : 0.0;
             if (want_ll) term = (n > 0.0) ? (n - c - n * log(n / c)) : -c;
-            const double rr = 1.0 - n / c;
+            const double rr = 1.0 - n / cg;
             if (R) R[i] = rr;
             if (composite_out) composite_out[i] = clobber_with_R ? rr : m;
         } else if (R) {
@@ -344,7 +347,8 @@ struct FgArgs {
     const double* d_comp_in = nullptr;  // composite given by the caller (loglikelihood(composite, data))
     const double* d_data = nullptr;
     int64_t nbins = 0;
-    double eps = 0.0;
+    double eps = 0.0;                   // LL clamp: eps(promote_type(composite, data))
+    double eps_g = 0.0;                 // gradient clamp: eps(eltype(composite))
     int want_ll = 0;
     double* d_negLL = nullptr;
     double* d_ll = nullptr;
@@ -361,7 +365,7 @@ static int run_fg(sfh_ctx* ctx, const sfh_templates* tpl, const FgArgs& a)
     const int64_t T = tpl ? tpl->ntemplates : 0;
     // fg! from coefficients: one pass over the templates where the shape allows
     if (a.d_coeffs && a.d_data && (a.want_ll || a.d_G) && single_pass_covers(tpl))
-        return single_pass_fg(ctx, tpl, a.d_data, a.d_coeffs, a.eps, a.want_ll, a.d_negLL, a.d_ll, a.d_G, a.gsign,
+        return single_pass_fg(ctx, tpl, a.d_data, a.d_coeffs, a.eps, a.eps_g, a.want_ll, a.d_negLL, a.d_ll, a.d_G, a.gsign,
                               a.d_comp_out, a.clobber_with_R);
     int nsplit = 1;
     const double* part = a.d_comp_in;
@@ -386,7 +390,7 @@ static int run_fg(sfh_ctx* ctx, const sfh_templates* tpl, const FgArgs& a)
     {
         TimedLaunch t(ctx, "composite_finish");
         hipLaunchKernelGGL(k_composite_finish, dim3((unsigned)nfin), dim3(kBlock), 0, ctx->stream,
-                           part, ld, nsplit, nbins, a.d_data, a.eps, a.want_ll, (double*)llp,
+                           part, ld, nsplit, nbins, a.d_data, a.eps, a.eps_g, a.want_ll, (double*)llp,
                            (double*)rbuf, a.d_comp_out, a.clobber_with_R);
     }
     rc = check_launch(ctx, "k_composite_finish");
@@ -798,6 +802,7 @@ int sfh_composite_device(sfh_ctx* ctx, const sfh_templates* tpl, const double* d
     a.d_data = nullptr;            // no LL, no R: data is not read
     a.d_comp_out = d_composite;
     a.eps = 0.0;
+    a.eps_g = 0.0;
     return run_fg(ctx, tpl, a);
 }
 
@@ -836,6 +841,7 @@ int sfh_loglikelihood(sfh_ctx* ctx, const double* composite, const double* data,
     a.d_data = dd;
     a.nbins = n;
     a.eps = eps;
+    a.eps_g = eps;
     a.want_ll = 1;
     a.d_ll = (double*)r;
     if (int rc = run_fg(ctx, nullptr, a)) return rc;
@@ -845,8 +851,8 @@ int sfh_loglikelihood(sfh_ctx* ctx, const double* composite, const double* data,
 }
 
 int sfh_fg_device(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
-                  const double* d_coeffs, int want_F, double eps, double* d_negLL, double* d_G,
-                  double* d_composite)
+                  const double* d_coeffs, int want_F, double eps, double eps_grad, double* d_negLL,
+                  double* d_G, double* d_composite)
 {
     SFH_ENTER(ctx);
     SFH_ARG_CHECK(ctx, tpl && data && d_coeffs, "null pointer");
@@ -858,6 +864,7 @@ int sfh_fg_device(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
     a.d_data = data->d_data;
     a.nbins = tpl->nbins;
     a.eps = eps;
+    a.eps_g = eps_grad;
     a.want_ll = want_F ? 1 : 0;
     a.d_negLL = d_negLL;
     a.d_G = d_G;
@@ -868,7 +875,7 @@ int sfh_fg_device(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
 }
 
 int sfh_fg(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data, const double* coeffs,
-           int want_F, double eps, double* negLL, double* G, double* composite)
+           int want_F, double eps, double eps_grad, double* negLL, double* G, double* composite)
 {
     SFH_ENTER(ctx);
     SFH_ARG_CHECK(ctx, tpl && data && coeffs, "null pointer");
@@ -883,7 +890,7 @@ int sfh_fg(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data, const d
     if (composite) {
         if (int rc = scratch(ctx, "h_composite", (size_t)tpl->nbins * sizeof(double), &pm)) return rc;
     }
-    if (int rc = sfh_fg_device(ctx, tpl, data, dc, want_F, eps, (double*)pn, (double*)pg, (double*)pm))
+    if (int rc = sfh_fg_device(ctx, tpl, data, dc, want_F, eps, eps_grad, (double*)pn, (double*)pg, (double*)pm))
         return rc;
     if (want_F)
         if (int rc = download(ctx, negLL, (const double*)pn, 1)) return rc;
@@ -901,7 +908,7 @@ int sfh_loglikelihood_coeffs(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_d
     SFH_ENTER(ctx);
     SFH_ARG_CHECK(ctx, out, "null pointer");
     double negll = 0.0;
-    if (int rc = sfh_fg(ctx, tpl, data, coeffs, 1, eps, &negll, nullptr, nullptr)) return rc;
+    if (int rc = sfh_fg(ctx, tpl, data, coeffs, 1, eps, eps, &negll, nullptr, nullptr)) return rc;
     *out = -negll;
     return SFH_OK;
 }
@@ -926,6 +933,7 @@ static int grad_common(sfh_ctx* ctx, const sfh_templates* tpl, double* composite
     a.d_data = data->d_data;
     a.nbins = n;
     a.eps = eps;
+    a.eps_g = eps;
     a.want_ll = 0;
     a.d_G = (double*)pg;
     a.gsign = -1.0;                // ∇loglikelihood: G = -models' * R (fitting_base.jl:283)
@@ -951,6 +959,46 @@ int sfh_grad_loglikelihood_inplace(sfh_ctx* ctx, const sfh_templates* tpl, doubl
 {
     SFH_ENTER(ctx);
     return grad_common(ctx, tpl, composite, data, eps, G, true);
+}
+
+
+int sfh_buffer_alloc(sfh_ctx* ctx, int64_t bytes, void** d_ptr)
+{
+    SFH_ENTER(ctx);
+    SFH_ARG_CHECK(ctx, d_ptr && bytes >= 0, "sfh_buffer_alloc: bad arguments");
+    *d_ptr = nullptr;
+    void* p = nullptr;
+    if (hipMalloc(&p, (size_t)(bytes > 0 ? bytes : 1)) != hipSuccess)
+        return set_error(ctx, SFH_ERR_ALLOC, "hipMalloc(%lld) failed", (long long)bytes);
+    *d_ptr = p;
+    return SFH_OK;
+}
+
+int sfh_buffer_free(sfh_ctx* ctx, void* d_ptr)
+{
+    SFH_ENTER(ctx);
+    if (!d_ptr) return SFH_OK;
+    SFH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    SFH_HIP_TRY(ctx, hipFree(d_ptr));
+    return SFH_OK;
+}
+
+int sfh_buffer_upload(sfh_ctx* ctx, void* d_dst, const void* h_src, int64_t bytes)
+{
+    SFH_ENTER(ctx);
+    SFH_ARG_CHECK(ctx, bytes >= 0 && (bytes == 0 || (d_dst && h_src)), "sfh_buffer_upload: bad arguments");
+    if (bytes) SFH_HIP_TRY(ctx, hipMemcpyAsync(d_dst, h_src, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+    SFH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SFH_OK;
+}
+
+int sfh_buffer_download(sfh_ctx* ctx, void* h_dst, const void* d_src, int64_t bytes)
+{
+    SFH_ENTER(ctx);
+    SFH_ARG_CHECK(ctx, bytes >= 0 && (bytes == 0 || (d_src && h_dst)), "sfh_buffer_download: bad arguments");
+    if (bytes) SFH_HIP_TRY(ctx, hipMemcpyAsync(h_dst, d_src, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+    SFH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SFH_OK;
 }
 
 }  // extern "C"

@@ -4,18 +4,21 @@
 //
 // Two kernels per call:
 //   k_dep_prep    one thread per point: the reference's FP64 footprint / index
-//                 math (bit-exact, no FMA contraction) -> a 48-byte PointDesc.
-//   k_dep_cov / k_dep_asym
-//                 bin-owner gather: a workgroup owns 256 consecutive "slots" of
-//                 one template, a slot being one row and 8 consecutive columns,
-//                 held by one thread in registers.  Point descriptors are staged
-//                 through LDS 64 at a time; every thread adds its own bins'
-//                 contributions, so there are no atomics and every bin sums its
-//                 points in point order (deterministic).  Waves whose rows miss a
-//                 footprint skip it uniformly.  Contributions accumulate in FP32
-//                 per 64-point chunk and are flushed into FP64 accumulators.
-// The pair work is VALU/transcendental bound; HBM traffic is the descriptors in
-// and one write of each template out.
+//                 math (bit-exact, no FMA contraction) and every per-point
+//                 constant of the evaluation -> an 80-byte PointDesc.
+//   k_dep_gather  bin-owner gather: a thread owns one row and kCPT = 16
+//                 consecutive columns of one template, in registers; a
+//                 workgroup 256 such strips.  Every wave walks its template's
+//                 descriptors in order through scalar loads (the descriptor is
+//                 uniform across the wave, so it sits in scalar registers: no
+//                 LDS staging, no barriers), skips points whose rows miss its
+//                 strips, and adds each point's contribution to its bins, so
+//                 there are no atomics and every bin sums its points in point
+//                 order (deterministic).  Contributions accumulate in FP32 per
+//                 64 points and are flushed into FP64 accumulators.
+// The pair work is VALU bound (about 3 + DEG VALU operations and one exp2 per
+// GL node and bin for the series forms); HBM traffic is the descriptors in and
+// one write of each template out.
 #include "sfh_internal.h"
 #include "sfh_psf.h"
 
@@ -24,13 +27,8 @@ namespace sfh {
 using psf::PointDesc;
 
 constexpr int kDepBlock = 256;
-constexpr int kCPT = 8;          // columns per thread
-constexpr int kChunk = 64;       // descriptors staged per LDS round
-
-struct DepRange {
-    double first, step;
-    int64_t length;
-};
+constexpr int kCPT = psf::kCPT;  // columns per thread
+constexpr int kChunk = 64;       // points between FP32 -> FP64 flushes
 
 __global__ __launch_bounds__(256) void k_dep_prep(
     const double* __restrict__ colors, const double* __restrict__ mags,
@@ -60,20 +58,16 @@ __global__ __launch_bounds__(256) void k_dep_prep(
     if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(pairs, cnt);
 }
 
-__device__ inline void stage_chunk(const PointDesc* __restrict__ desc, int64_t pc, int n, PointDesc* sdesc)
-{
-    const float4* src = reinterpret_cast<const float4*>(desc + pc);
-    float4* dst = reinterpret_cast<float4*>(sdesc);
-    for (int k = threadIdx.x; k < n * 3; k += kDepBlock) dst[k] = src[k];
-}
-
+// The FP64 accumulators live in LDS (16 per thread, 32 KiB per workgroup),
+// touched once per 64 points, so that the FP32 strip and the per-point
+// constants fit four waves per SIMD.
 template <bool kAsym>
-__global__ __launch_bounds__(kDepBlock) void k_dep_gather(
+__global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dep_gather(
     const PointDesc* __restrict__ desc, const int64_t* __restrict__ offsets, int nx, int ny,
     int ncg, int nblk_per_tpl, float* __restrict__ out32, double* __restrict__ out64,
     int64_t ld_out)
 {
-    __shared__ PointDesc sdesc[kChunk];
+    __shared__ double sacc[kCPT][kDepBlock];
     const int64_t t = blockIdx.x / nblk_per_tpl;
     const int blk = blockIdx.x - (int)(t * nblk_per_tpl);
     const int64_t p0 = offsets[t], p1 = offsets[t + 1];
@@ -81,71 +75,49 @@ __global__ __launch_bounds__(kDepBlock) void k_dep_gather(
     const int row = slot / ncg;
     const int col0 = (slot - row * ncg) * kCPT;
     const bool valid = row < ny;
-    const int wslot = blk * kDepBlock + (threadIdx.x & ~63);
+    // rows of this wave's 64 strips (wave-uniform)
+    const int wslot = __builtin_amdgcn_readfirstlane(blk * kDepBlock + (int)(threadIdx.x & ~63u));
     const int row_lo = wslot / ncg, row_hi = (wslot + 63) / ncg;
 
-    double acc[kCPT];
+    // volatile: the accumulators stay in LDS instead of being promoted to registers
+    volatile double* acc = &sacc[0][threadIdx.x];
     float part[kCPT];
 #pragma unroll
     for (int c = 0; c < kCPT; ++c) {
-        acc[c] = 0.0;
+        acc[c * kDepBlock] = 0.0;
         part[c] = 0.0f;
     }
-    for (int64_t pc = p0; pc < p1; pc += kChunk) {
-        const int n = (int)((p1 - pc) < kChunk ? (p1 - pc) : kChunk);
-        __syncthreads();
-        stage_chunk(desc, pc, n, sdesc);
-        __syncthreads();
-        for (int q = 0; q < n; ++q) {
-            const PointDesc d = sdesc[q];
-            if (d.ya > row_hi || d.yb < row_lo || d.xa > d.xb) continue;   // wave-uniform
+    // descriptors are uniform: scalar loads, the next one in flight while a
+    // point is evaluated (32-bit indices: scalar compares)
+    const PointDesc* __restrict__ td = desc + p0;
+    const int np = (int)(p1 - p0);
+    PointDesc dn{};
+    if (np > 0) dn = td[0];
+    for (int pc = 0; pc < np; pc += kChunk) {
+        const int pe = (np - pc) < kChunk ? np : pc + kChunk;
+        for (int p = pc; p < pe; ++p) {
+            const PointDesc d = dn;
+            dn = td[p + 1 < np ? p + 1 : p];
+            if (d.ya > row_hi || d.yb < row_lo || d.xa > d.xb) continue;
             if (!valid || row < d.ya || row > d.yb || col0 > d.xb || col0 + kCPT - 1 < d.xa) continue;
-            const bool xser = d.ex < psf::kSeriesEps;
-            const psf::Series sx = psf::series_coeffs(d.ex);
-            if (kAsym) {
-                // A/4 * D(u_i, ex) * D(v_j, ey)
-                const float v = fmaf((float)(row - d.ya), d.dv, d.ay);
-                float dy;
-                if (d.ey < psf::kSeriesEps) {
-                    dy = psf::d_series(v, psf::series_coeffs(d.ey));
-                } else {
-                    dy = psf::d_direct(v, d.ey);
-                }
-                const float fy = d.pref * dy;
-#pragma unroll
-                for (int c = 0; c < kCPT; ++c) {
-                    const int col = col0 + c;
-                    const float u = fmaf((float)(col - d.xa), d.du, d.ax);
-                    const float dx = xser ? psf::d_series(u, sx) : psf::d_direct(u, d.ex);
-                    part[c] += (col >= d.xa && col <= d.xb) ? fy * dx : 0.0f;
-                }
+            // full width: every column of the grid is in the footprint, and the
+            // strips' columns past the grid are never written, so no mask
+            const bool full = d.xa == 0 && d.xb == nx - 1;
+            if constexpr (kAsym) {
+                float W[1], h[1] = {0.0f};
+                W[0] = d.pref * psf::y_factor(fmaf((float)(row - d.ya), d.dv, d.ay), d.ey);
+                if (full) psf::add_point<1, false>(d, W, h, col0, part);
+                else psf::add_point<1, true>(d, W, h, col0, part);
             } else {
-                // sum_k w_k exp(-t_k^2/2) D(u_i + g t_k, ex), GL nodes t_k = t_j + {-1,0,1}*ey
-                const float tj = fmaf((float)(row - d.ya), d.dv, d.ay);
-                const float t0 = tj - d.ey, t2 = tj + d.ey;
-                const float w0 = psf::kW0 * psf::fast_exp(-0.5f * t0 * t0);
-                const float w1 = psf::kW1 * psf::fast_exp(-0.5f * tj * tj);
-                const float w2 = psf::kW0 * psf::fast_exp(-0.5f * t2 * t2);
-                const float h0 = d.g * t0, h1 = d.g * tj, h2 = d.g * t2;
-#pragma unroll
-                for (int c = 0; c < kCPT; ++c) {
-                    const int col = col0 + c;
-                    const float u = fmaf((float)(col - d.xa), d.du, d.ax);
-                    float v;
-                    if (xser) {
-                        v = w0 * psf::d_series(u + h0, sx) + w1 * psf::d_series(u + h1, sx) +
-                            w2 * psf::d_series(u + h2, sx);
-                    } else {
-                        v = w0 * psf::d_direct(u + h0, d.ex) + w1 * psf::d_direct(u + h1, d.ex) +
-                            w2 * psf::d_direct(u + h2, d.ex);
-                    }
-                    part[c] += (col >= d.xa && col <= d.xb) ? d.pref * v : 0.0f;
-                }
+                float W[3], h[3];
+                psf::cov_row(d, row, W, h);
+                if (full) psf::add_point<3, false>(d, W, h, col0, part);
+                else psf::add_point<3, true>(d, W, h, col0, part);
             }
         }
 #pragma unroll
         for (int c = 0; c < kCPT; ++c) {
-            acc[c] += (double)part[c];
+            acc[c * kDepBlock] += (double)part[c];
             part[c] = 0.0f;
         }
     }
@@ -155,8 +127,9 @@ __global__ __launch_bounds__(kDepBlock) void k_dep_gather(
     for (int c = 0; c < kCPT; ++c) {
         const int col = col0 + c;
         if (col < nx) {
-            if (out32) out32[base + col] = (float)acc[c];
-            if (out64) out64[base + col] = acc[c];
+            const double v = acc[c * kDepBlock];
+            if (out32) out32[base + col] = (float)v;
+            if (out64) out64[base + col] = v;
         }
     }
 }
@@ -183,7 +156,7 @@ static int deposit(sfh_ctx* ctx, const double* colors, const double* mags, const
     SFH_ARG_CHECK(ctx, ld_out >= (int64_t)nx * ny, "ld_out < nx*ny");
     const bool asym = (cov_mult == 0.0) && cmult == nullptr;
     void *pdesc = nullptr, *pcnt = nullptr;
-    if (int rc = scratch(ctx, "dep_desc", (size_t)(npoints_total > 0 ? npoints_total : 1) * sizeof(PointDesc), &pdesc))
+    if (int rc = scratch(ctx, "dep_desc", (size_t)(npoints_total + 1) * sizeof(PointDesc), &pdesc))
         return rc;
     if (int rc = scratch(ctx, "dep_counters", 64, &pcnt)) return rc;
     SFH_HIP_TRY(ctx, hipMemsetAsync(pcnt, 0, 64, ctx->stream));
@@ -222,6 +195,30 @@ static int deposit(sfh_ctx* ctx, const double* colors, const double* mags, const
     if (bad) return set_error(ctx, SFH_ERR_DOMAIN, "non-finite kernel parameter (InexactError in round(Int, ·))");
     if (pairs_out) *pairs_out = (int64_t)pairs;
     return SFH_OK;
+}
+
+// The gather of descriptors already built (by the device preparation of
+// partial_cmd_smooth): template t owns desc[koff[t] .. koff[t+1]).
+int deposit_from_desc(sfh_ctx* ctx, const void* desc, const int64_t* d_koff, int64_t ntemplates, bool asym,
+                      sfh_range xe, sfh_range ye, float* out32, int64_t ld_out)
+{
+    if (int rc = validate_ranges(ctx, xe, ye)) return rc;
+    const int nx = (int)(xe.length - 1), ny = (int)(ye.length - 1);
+    SFH_ARG_CHECK(ctx, ld_out >= (int64_t)nx * ny, "ld_out < nx*ny");
+    const int ncg = (int)cdiv(nx, kCPT);
+    const int nblk = (int)cdiv((int64_t)ncg * ny, kDepBlock);
+    const int64_t grid = (int64_t)nblk * ntemplates;
+    SFH_ARG_CHECK(ctx, grid < (int64_t)1 << 31, "too many templates for one launch");
+    if (asym) {
+        TimedLaunch tl(ctx, "deposit_asym");
+        hipLaunchKernelGGL(k_dep_gather<true>, dim3((unsigned)grid), dim3(kDepBlock), 0, ctx->stream,
+                           (const PointDesc*)desc, d_koff, nx, ny, ncg, nblk, out32, (double*)nullptr, ld_out);
+    } else {
+        TimedLaunch tl(ctx, "deposit_cov");
+        hipLaunchKernelGGL(k_dep_gather<false>, dim3((unsigned)grid), dim3(kDepBlock), 0, ctx->stream,
+                           (const PointDesc*)desc, d_koff, nx, ny, ncg, nblk, out32, (double*)nullptr, ld_out);
+    }
+    return check_launch(ctx, "k_dep_gather");
 }
 
 }  // namespace sfh

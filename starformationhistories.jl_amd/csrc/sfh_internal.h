@@ -128,8 +128,14 @@ void release_batched(const sfh_templates* t);
 // says whether the shape fits it (1..1024 templates, 32-bit offsets).
 bool single_pass_covers(const sfh_templates* tpl);
 int single_pass_fg(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_data, const double* d_coeffs, double eps,
-                   int want_ll, double* d_negLL, double* d_ll, double* d_G, double gsign, double* d_comp_out,
+                   double eps_g, int want_ll, double* d_negLL, double* d_ll, double* d_G, double gsign, double* d_comp_out,
                    int clobber_with_R);
+
+// Deposition gather over descriptors already built on the device (template t
+// owns desc[koff[t] .. koff[t+1]), koff on the device); FP32 out, column t at
+// out32 + t*ld_out.
+int deposit_from_desc(sfh_ctx* ctx, const void* desc, const int64_t* d_koff, int64_t ntemplates, bool asym,
+                      sfh_range xe, sfh_range ye, float* out32, int64_t ld_out);
 
 // Launchers implemented per translation unit.
 int launch_composite(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_coeffs,

@@ -11,7 +11,8 @@
 // keeps them in registers through the whole block (the next block's loads
 // in flight meanwhile):
 //   composite  m_i  = sum_t C[i,t] x_t      FP64 FMAs, reduced over the 8 NW column slices
-//   LL, R      on 32 threads: n - c - n log(n/c), 1 - n/c with c = max(m, eps)
+//   LL, R      on 32 threads: n - c - n log(n/c), 1 - n/c' with c = max(m, eps),
+//              c' = max(m, eps_g)
 //   gradient   g_t += sum_i C[i,t] R_i      FP64 FMAs into per-thread registers
 // so the templates are read from HBM once per evaluation.  Each item writes
 // one LL partial and T gradient partials; the per-fit reduction sums them in
@@ -78,7 +79,7 @@ struct Item {
 // LL and R on 32 threads, gradient accumulation into g.
 // OUT: also write the composite m (or R when rout_R) of every bin to rout.
 template <int K, int NW, bool OUT = false>
-__device__ __forceinline__ void fg_block(const FitDesc& fd, int64_t b0, double eps, int want_G, const double* sx,
+__device__ __forceinline__ void fg_block(const FitDesc& fd, int64_t b0, double eps, double eps_g, int want_G, const double* sx,
                                          double (*sm)[kSB], double* sr, f4 (&c)[K], double nv, double (&g)[K],
                                          double& ll, double* rout = nullptr, int rout_R = 0)
 {
@@ -117,8 +118,15 @@ __device__ __forceinline__ void fg_block(const FitDesc& fd, int64_t b0, double e
         if (bin < fd.N) {
             const double cl = m > eps ? m : eps;
             const double n = nv;
-            ll += (n > 0.0) ? (n - cl - n * log(n / cl)) : -cl;
-            r = 1.0 - n / cl;
+            const double q = n / cl;
+            ll += (n > 0.0) ? (n - cl - n * log(q)) : -cl;
+            // the gradient's clamp is eps(eltype(composite)) (fitting_base.jl:277),
+            // the LL's eps of the promoted type (:86); they differ only where m
+            // is below one of them
+            const double cg = m > eps_g ? m : eps_g;
+            double qg = q;
+            if (cg != cl) qg = n / cg;
+            r = 1.0 - qg;
             if constexpr (OUT) {
                 if (rout) rout[bin] = rout_R ? r : m;
             }
@@ -152,7 +160,7 @@ __device__ __forceinline__ void fg_block(const FitDesc& fd, int64_t b0, double e
 // blocked copy (template stride 32, block stride 32 T), and the per-bin
 // composite output of fg_block is on.
 template <int K, int NW, bool COLMAJOR = false>
-__device__ __forceinline__ void fg_item(const FitDesc& fd, const Item& item, int64_t it, double eps, int want_G,
+__device__ __forceinline__ void fg_item(const FitDesc& fd, const Item& item, int64_t it, double eps, double eps_g, int want_G,
                                         const double* sx, double (*sm)[kSB], double* sr, double* __restrict__ ll_part,
                                         double* __restrict__ g_part, double* rout = nullptr, int rout_R = 0)
 {
@@ -196,11 +204,11 @@ __device__ __forceinline__ void fg_item(const FitDesc& fd, const Item& item, int
     SFH_LOAD(ca, na, 0);
     for (int b = 0; b <= last; b += 2) {
         SFH_LOAD(cb, nb, b + 1);
-        fg_block<K, NW, COLMAJOR>(fd, item.bin0 + (int64_t)b * kSB, eps, want_G, sx, sm, sr, ca, na, g, ll, rout,
+        fg_block<K, NW, COLMAJOR>(fd, item.bin0 + (int64_t)b * kSB, eps, eps_g, want_G, sx, sm, sr, ca, na, g, ll, rout,
                                   rout_R);
         SFH_LOAD(ca, na, b + 2);
         if (b + 1 <= last)
-            fg_block<K, NW, COLMAJOR>(fd, item.bin0 + (int64_t)(b + 1) * kSB, eps, want_G, sx, sm, sr, cb, nb, g,
+            fg_block<K, NW, COLMAJOR>(fd, item.bin0 + (int64_t)(b + 1) * kSB, eps, eps_g, want_G, sx, sm, sr, cb, nb, g,
                                       ll, rout, rout_R);
     }
 #undef SFH_LOAD
@@ -227,16 +235,16 @@ __device__ __forceinline__ void fg_item(const FitDesc& fd, const Item& item, int
 // 22, 32, capped at KMAX) with K >= kmax.
 template <int NW, int KMAX, int I = 0>
 __device__ __forceinline__ void fg_dispatch(int kmax, const FitDesc& fd, const Item& item, int64_t it, double eps,
-                                            int want_G, const double* sx, double (*sm)[kSB], double* sr,
+                                            double eps_g, int want_G, const double* sx, double (*sm)[kSB], double* sr,
                                             double* __restrict__ ll_part, double* __restrict__ g_part)
 {
     constexpr int kBuckets[] = {1, 2, 3, 4, 6, 8, 11, 16, 22, 32};
     constexpr int K = kBuckets[I];
     if constexpr (K >= KMAX) {
-        fg_item<KMAX, NW>(fd, item, it, eps, want_G, sx, sm, sr, ll_part, g_part);
+        fg_item<KMAX, NW>(fd, item, it, eps, eps_g, want_G, sx, sm, sr, ll_part, g_part);
     } else {
-        if (kmax <= K) fg_item<K, NW>(fd, item, it, eps, want_G, sx, sm, sr, ll_part, g_part);
-        else fg_dispatch<NW, KMAX, I + 1>(kmax, fd, item, it, eps, want_G, sx, sm, sr, ll_part, g_part);
+        if (kmax <= K) fg_item<K, NW>(fd, item, it, eps, eps_g, want_G, sx, sm, sr, ll_part, g_part);
+        else fg_dispatch<NW, KMAX, I + 1>(kmax, fd, item, it, eps, eps_g, want_G, sx, sm, sr, ll_part, g_part);
     }
 }
 
@@ -249,10 +257,9 @@ constexpr int fg_waves_per_eu(int kmax) { return kmax <= 4 ? 4 : kmax <= 8 ? 3 :
 
 template <int NW, int KMAX>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(fg_waves_per_eu(KMAX))))
-void k_multi_fg(const FitDesc* __restrict__ fits, const Item* __restrict__ items,
-                                                      int64_t nitems, const double* __restrict__ X,
-                                                      const int32_t* __restrict__ active, double eps, int want_G,
-                                                      double* __restrict__ ll_part, double* __restrict__ g_part)
+void k_multi_fg(const FitDesc* __restrict__ fits, const Item* __restrict__ items, int64_t nitems,
+                const double* __restrict__ X, const int32_t* __restrict__ active, double eps, double eps_g,
+                int want_G, double* __restrict__ ll_part, double* __restrict__ g_part)
 {
     constexpr int NT = 64 * NW, SL = 8 * NW;
     __shared__ double sx[KMAX * SL];
@@ -267,7 +274,7 @@ void k_multi_fg(const FitDesc* __restrict__ fits, const Item* __restrict__ items
         __syncthreads();    // the previous item is done with sx
         for (int t = threadIdx.x; t < KMAX * SL; t += NT) sx[t] = t < T ? ((gdouble*)X)[fd.xoff + t] : 0.0;
         __syncthreads();
-        fg_dispatch<NW, KMAX>(kmax, fd, item, it, eps, want_G, sx, sm, sr, ll_part, g_part);
+        fg_dispatch<NW, KMAX>(kmax, fd, item, it, eps, eps_g, want_G, sx, sm, sr, ll_part, g_part);
     }
 }
 
@@ -816,10 +823,10 @@ int kmax_for(int64_t maxT, int nw)
 
 template <int NW, int KMAX>
 void launch_fg_t(hipStream_t s, int grid, const FitDesc* fits, const Item* items, int64_t nitems, const double* X,
-                 const int32_t* active, double eps, int want_G, double* llp, double* gp)
+                 const int32_t* active, double eps, double eps_g, int want_G, double* llp, double* gp)
 {
     hipLaunchKernelGGL((k_multi_fg<NW, KMAX>), dim3(grid), dim3(64 * NW), 0, s, fits, items, nitems, X, active, eps,
-                       want_G, llp, gp);
+                       eps_g, want_G, llp, gp);
 }
 
 // ------------------------------------------- one fit, templates read in place
@@ -833,7 +840,7 @@ void launch_fg_t(hipStream_t s, int grid, const FitDesc* fits, const Item* items
 // needs, the coefficients staged once per workgroup.
 template <int NW, int K>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(fg_waves_per_eu(K))))
-void k_single_fg(FitDesc fd, int64_t nitems, const double* __restrict__ X, double eps, int want_G,
+void k_single_fg(FitDesc fd, int64_t nitems, const double* __restrict__ X, double eps, double eps_g, int want_G,
                  double* __restrict__ ll_part, double* __restrict__ g_part, double* rout, int rout_R)
 {
     constexpr int NT = 64 * NW, SL = 8 * NW;
@@ -846,7 +853,7 @@ void k_single_fg(FitDesc fd, int64_t nitems, const double* __restrict__ X, doubl
     for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int64_t b0 = it * nblk / nitems, b1 = (it + 1) * nblk / nitems;
         const Item item{b0 * kSB, 0, (int32_t)(b1 - b0)};
-        fg_item<K, NW, true>(fd, item, it, eps, want_G, sx, sm, sr, ll_part, g_part, rout, rout_R);
+        fg_item<K, NW, true>(fd, item, it, eps, eps_g, want_G, sx, sm, sr, ll_part, g_part, rout, rout_R);
     }
 }
 
@@ -894,14 +901,14 @@ __global__ __launch_bounds__(kWG) void k_single_reduce(const double* __restrict_
 
 template <int K>
 void launch_single_t(hipStream_t s, int cus, const FitDesc& fd, int64_t nblk, const double* X, double eps,
-                     int want_G, double* llp, double* gp, double* rout, int rout_R, int64_t* nitems_out)
+                     double eps_g, int want_G, double* llp, double* gp, double* rout, int rout_R, int64_t* nitems_out)
 {
     // workgroups resident per CU: two of 8 waves where K leaves <= 128 VGPRs
     const int64_t grid = (int64_t)cus * (K <= 4 ? 2 : 1);
     const int64_t nitems = nblk < grid ? nblk : grid;
     *nitems_out = nitems;
     hipLaunchKernelGGL((k_single_fg<kSingleNW, K>), dim3((unsigned)nitems), dim3(64 * kSingleNW), 0, s, fd, nitems,
-                       X, eps, want_G, llp, gp, rout, rout_R);
+                       X, eps, eps_g, want_G, llp, gp, rout, rout_R);
 }
 
 }  // namespace mf
@@ -913,18 +920,39 @@ bool single_pass_covers(const sfh_templates* tpl)
            (tpl->ntemplates - 1) * ld + mf::kSB < ((int64_t)1 << 31);
 }
 
-int single_pass_fg(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_data, const double* d_coeffs, double eps,
-                   int want_ll, double* d_negLL, double* d_ll, double* d_G, double gsign, double* d_comp_out,
-                   int clobber_with_R)
+// Compute units of each device (cached at the first call on it).
+static int g_cus_of[64] = {};
+
+static int device_cus(sfh_ctx* ctx, int* cus)
+{
+    if (ctx->device < 0 || ctx->device >= 64) return set_error(ctx, SFH_ERR_ARGUMENT, "device index out of range");
+    int c = __atomic_load_n(&g_cus_of[ctx->device], __ATOMIC_RELAXED);
+    if (!c) {
+        SFH_HIP_TRY(ctx, hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        __atomic_store_n(&g_cus_of[ctx->device], c, __ATOMIC_RELAXED);
+    }
+    *cus = c;
+    return SFH_OK;
+}
+
+// Partials of one single-pass fg!: one LL value per work item, and ntemplates
+// gradient values per item when the gradient is wanted; at most 2 items per CU.
+static int64_t single_pass_partials(const sfh_templates* tpl, int cus, bool want_G, int64_t* cap)
+{
+    const int64_t nblk = cdiv(tpl->nbins, mf::kSB);
+    *cap = std::min<int64_t>(nblk, 2 * (int64_t)cus);
+    return *cap * (1 + (want_G ? tpl->ntemplates : 0));
+}
+
+// The launches of a single-pass fg! into caller-provided partial buffers
+// (llp: cap doubles, gp: cap x ntemplates doubles or NULL without the
+// gradient).  `timed` records the launches in the context's kernel timing.
+static int single_pass_launch(sfh_ctx* ctx, const sfh_templates* tpl, int cus, const double* d_data,
+                              const double* d_coeffs, double eps, double eps_g, int want_ll, double* d_negLL,
+                              double* d_ll, double* d_G, double gsign, double* d_comp_out, int clobber_with_R,
+                              double* llp, double* gp, bool timed)
 {
     using namespace mf;
-    static int cus_of[64] = {};
-    if (ctx->device < 0 || ctx->device >= 64) return set_error(ctx, SFH_ERR_ARGUMENT, "device index out of range");
-    int cus = __atomic_load_n(&cus_of[ctx->device], __ATOMIC_RELAXED);
-    if (!cus) {
-        SFH_HIP_TRY(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-        __atomic_store_n(&cus_of[ctx->device], cus, __ATOMIC_RELAXED);
-    }
     const int64_t T = tpl->ntemplates, N = tpl->nbins;
     FitDesc fd{};
     fd.C = fd.Cb = tpl->d_models;
@@ -933,36 +961,51 @@ int single_pass_fg(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_data,
     fd.N = N;
     fd.T = T;
     // one item (a run of blocks, the next block's loads in flight) per
-    // resident workgroup; partials sized for two per CU
+    // resident workgroup
     const int64_t nblk = cdiv(N, kSB);
-    const int64_t cap = std::min<int64_t>(nblk, 2 * (int64_t)cus);
-    void *llp = nullptr, *gp = nullptr;
-    if (int rc = scratch(ctx, "fg1_llpart", (size_t)cap * sizeof(double), &llp)) return rc;
-    if (d_G)
-        if (int rc = scratch(ctx, "fg1_gpart", (size_t)cap * T * sizeof(double), &gp)) return rc;
     const int want_G = d_G ? 1 : 0;
     const int need = (int)cdiv(T, 8 * kSingleNW);
     int64_t nitems = 0;
     {
-        TimedLaunch tl(ctx, "single_fg");
+        hipEvent_t t0{};
+        if (timed) timing_begin(ctx, "single_fg", &t0);
         hipStream_t s = ctx->stream;
-        double* l = (double*)llp;
-        double* g = (double*)gp;
-        if (need <= 1) launch_single_t<1>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
-        else if (need <= 2) launch_single_t<2>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
-        else if (need <= 3) launch_single_t<3>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
-        else if (need <= 4) launch_single_t<4>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
-        else if (need <= 6) launch_single_t<6>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
-        else if (need <= 8) launch_single_t<8>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
-        else launch_single_t<kSingleMaxK>(s, cus, fd, nblk, d_coeffs, eps, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        double* l = llp;
+        double* g = gp;
+        if (need <= 1) launch_single_t<1>(s, cus, fd, nblk, d_coeffs, eps, eps_g, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        else if (need <= 2) launch_single_t<2>(s, cus, fd, nblk, d_coeffs, eps, eps_g, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        else if (need <= 3) launch_single_t<3>(s, cus, fd, nblk, d_coeffs, eps, eps_g, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        else if (need <= 4) launch_single_t<4>(s, cus, fd, nblk, d_coeffs, eps, eps_g, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        else if (need <= 6) launch_single_t<6>(s, cus, fd, nblk, d_coeffs, eps, eps_g, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        else if (need <= 8) launch_single_t<8>(s, cus, fd, nblk, d_coeffs, eps, eps_g, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        else launch_single_t<kSingleMaxK>(s, cus, fd, nblk, d_coeffs, eps, eps_g, want_G, l, g, d_comp_out, clobber_with_R, &nitems);
+        if (timed) timing_end(ctx, "single_fg", t0);
     }
     if (int rc = check_launch(ctx, "k_single_fg")) return rc;
     if (want_ll || d_G) {
-        TimedLaunch tl(ctx, "single_reduce");
+        hipEvent_t t0{};
+        if (timed) timing_begin(ctx, "single_reduce", &t0);
         hipLaunchKernelGGL(k_single_reduce, dim3((unsigned)(d_G ? cdiv(T, 16) : 1)), dim3(kWG), 0, ctx->stream,
                            (const double*)llp, (const double*)gp, nitems, T, want_ll, d_negLL, d_ll, gsign, d_G);
+        if (timed) timing_end(ctx, "single_reduce", t0);
     }
     return check_launch(ctx, "k_single_reduce");
+}
+
+int single_pass_fg(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_data, const double* d_coeffs, double eps,
+                   double eps_g, int want_ll, double* d_negLL, double* d_ll, double* d_G, double gsign,
+                   double* d_comp_out, int clobber_with_R)
+{
+    int cus = 0;
+    if (int rc = device_cus(ctx, &cus)) return rc;
+    int64_t cap = 0;
+    single_pass_partials(tpl, cus, d_G != nullptr, &cap);
+    void *llp = nullptr, *gp = nullptr;
+    if (int rc = scratch(ctx, "fg1_llpart", (size_t)cap * sizeof(double), &llp)) return rc;
+    if (d_G)
+        if (int rc = scratch(ctx, "fg1_gpart", (size_t)cap * tpl->ntemplates * sizeof(double), &gp)) return rc;
+    return single_pass_launch(ctx, tpl, cus, d_data, d_coeffs, eps, eps_g, want_ll, d_negLL, d_ll, d_G, gsign,
+                              d_comp_out, clobber_with_R, (double*)llp, (double*)gp, true);
 }
 
 }  // namespace sfh
@@ -1052,12 +1095,12 @@ int build_items(sfh_multifit* mf)
     return SFH_OK;
 }
 
-void launch_fg(sfh_multifit* mf, const double* d_X, const int32_t* active, double eps, int want_G)
+void launch_fg(sfh_multifit* mf, const double* d_X, const int32_t* active, double eps, double eps_g, int want_G)
 {
     const int grid = mf->cus * mf->wg_per_cu;
     hipStream_t s = mf->ctx->stream;
     const int km = kmax_for(mf->maxT, mf->nw);
-#define SFH_L(NW, KM) launch_fg_t<NW, KM>(s, grid, mf->d_fits, mf->d_items, mf->nitems, d_X, active, eps, want_G, mf->d_llp, mf->d_gp)
+#define SFH_L(NW, KM) launch_fg_t<NW, KM>(s, grid, mf->d_fits, mf->d_items, mf->nitems, d_X, active, eps, eps_g, want_G, mf->d_llp, mf->d_gp)
     if (mf->nw == 4) {
         if (km <= 2) SFH_L(4, 2); else if (km <= 4) SFH_L(4, 4); else if (km <= 8) SFH_L(4, 8);
         else if (km <= 11) SFH_L(4, 11); else if (km <= 16) SFH_L(4, 16); else SFH_L(4, 32);
@@ -1070,12 +1113,12 @@ void launch_fg(sfh_multifit* mf, const double* d_X, const int32_t* active, doubl
 #undef SFH_L
 }
 
-int fg_device_impl(sfh_multifit* mf, const double* d_X, double eps, double* d_negLL, double* d_G)
+int fg_device_impl(sfh_multifit* mf, const double* d_X, double eps, double eps_g, double* d_negLL, double* d_G)
 {
     sfh_ctx* ctx = mf->ctx;
     {
         TimedLaunch tl(ctx, "multifit_fg");
-        launch_fg(mf, d_X, nullptr, eps, d_G ? 1 : 0);
+        launch_fg(mf, d_X, nullptr, eps, eps_g, d_G ? 1 : 0);
     }
     if (int rc = check_launch(ctx, "k_multi_fg")) return rc;
     {
@@ -1269,9 +1312,9 @@ int sfh_multifit_autotune(sfh_multifit* mf, int reps, int64_t* item_elems, int* 
             mf->nw = nw;
             mf->wg_per_cu = wg;
             if ((rc = build_items(mf))) break;
-            launch_fg(mf, d_X, nullptr, eps, 1);
+            launch_fg(mf, d_X, nullptr, eps, eps, 1);
             (void)hipEventRecord(e0, ctx->stream);
-            for (int r = 0; r < reps; ++r) launch_fg(mf, d_X, nullptr, eps, 1);
+            for (int r = 0; r < reps; ++r) launch_fg(mf, d_X, nullptr, eps, eps, 1);
             (void)hipEventRecord(e1, ctx->stream);
             if ((rc = check_launch(ctx, "k_multi_fg"))) break;
             if (hipEventSynchronize(e1) != hipSuccess) {
@@ -1302,17 +1345,18 @@ int sfh_multifit_autotune(sfh_multifit* mf, int reps, int64_t* item_elems, int* 
     return SFH_OK;
 }
 
-int sfh_multifit_fg_device(sfh_multifit* mf, const double* d_X, double eps, double* d_negLL, double* d_G)
+int sfh_multifit_fg_device(sfh_multifit* mf, const double* d_X, double eps, double eps_grad, double* d_negLL,
+                           double* d_G)
 {
     if (!mf) return SFH_ERR_ARGUMENT;
     sfh_ctx* ctx = mf->ctx;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     SFH_ARG_CHECK(ctx, d_X && d_negLL, "sfh_multifit_fg_device: X and negLL are required");
     if (int rc = bind(ctx)) return rc;
-    return fg_device_impl(mf, d_X, eps, d_negLL, d_G);
+    return fg_device_impl(mf, d_X, eps, eps_grad, d_negLL, d_G);
 }
 
-int sfh_multifit_fg(sfh_multifit* mf, const double* X, double eps, double* negLL, double* G)
+int sfh_multifit_fg(sfh_multifit* mf, const double* X, double eps, double eps_grad, double* negLL, double* G)
 {
     if (!mf) return SFH_ERR_ARGUMENT;
     sfh_ctx* ctx = mf->ctx;
@@ -1325,7 +1369,7 @@ int sfh_multifit_fg(sfh_multifit* mf, const double* X, double eps, double* negLL
     if (G)
         if (int rc = scratch(ctx, "mf_g", mf->sumT * sizeof(double), &pg)) return rc;
     SFH_HIP_TRY(ctx, hipMemcpyAsync(px, X, mf->sumT * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    if (int rc = fg_device_impl(mf, (const double*)px, eps, (double*)pn, (double*)pg)) return rc;
+    if (int rc = fg_device_impl(mf, (const double*)px, eps, eps_grad, (double*)pn, (double*)pg)) return rc;
     SFH_HIP_TRY(ctx, hipMemcpyAsync(negLL, pn, mf->nfits * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (G) SFH_HIP_TRY(ctx, hipMemcpyAsync(G, pg, mf->sumT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     SFH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -1343,6 +1387,7 @@ int sfh_fit_options_default(sfh_fit_options* opt)
     opt->g_abstol = 1e-8;
     opt->max_iterations = 1000;
     opt->eps = 2.220446049250313e-16;
+    opt->eps_grad = 2.220446049250313e-16;
     return SFH_OK;
 }
 
@@ -1359,7 +1404,8 @@ int sfh_multifit_fit(sfh_multifit* mf, const double* x0, const sfh_fit_options* 
     SFH_ARG_CHECK(ctx, o.method != SFH_OPT_LBFGS || (o.memory >= 1 && o.memory <= kMaxMem),
                   "sfh_multifit_fit: 1 <= memory <= %d", kMaxMem);
     SFH_ARG_CHECK(ctx, o.steps_per_graph >= 1 && o.steps_per_graph <= 256, "sfh_multifit_fit: 1 <= steps_per_graph <= 256");
-    SFH_ARG_CHECK(ctx, o.max_iterations >= 1 && o.g_abstol >= 0.0 && o.eps > 0.0, "sfh_multifit_fit: bad options");
+    SFH_ARG_CHECK(ctx, o.max_iterations >= 1 && o.g_abstol >= 0.0 && o.eps > 0.0 && o.eps_grad > 0.0,
+                  "sfh_multifit_fit: bad options");
     SFH_ARG_CHECK(ctx, o.method == SFH_OPT_BFGS || (!res || (!res->invH_map && !res->invH_mle)),
                   "sfh_multifit_fit: the inverse Hessian is returned by the BFGS method only");
     if (int rc = bind(ctx)) return rc;
@@ -1423,7 +1469,7 @@ int sfh_multifit_fit(sfh_multifit* mf, const double* x0, const sfh_fit_options* 
     for (int k = 0; k < o.steps_per_graph; ++k) {
         if (bfgs) hipLaunchKernelGGL(k_opt_propose<SFH_OPT_BFGS>, dim3(nf), dim3(kWG), 0, ctx->stream, p);
         else hipLaunchKernelGGL(k_opt_propose<SFH_OPT_LBFGS>, dim3(nf), dim3(kWG), 0, ctx->stream, p);
-        launch_fg(mf, mf->d_X, mf->d_active, o.eps, 1);
+        launch_fg(mf, mf->d_X, mf->d_active, o.eps, o.eps_grad, 1);
         if (bfgs) hipLaunchKernelGGL(k_opt_accept<SFH_OPT_BFGS>, dim3(nf), dim3(kWG), 0, ctx->stream, p);
         else hipLaunchKernelGGL(k_opt_accept<SFH_OPT_LBFGS>, dim3(nf), dim3(kWG), 0, ctx->stream, p);
     }
@@ -1492,5 +1538,6 @@ int sfh_multifit_fit(sfh_multifit* mf, const double* x0, const sfh_fit_options* 
     }
     return SFH_OK;
 }
+
 
 }  // extern "C"

@@ -17,9 +17,17 @@
 //                g = c sy / (sqrt(2) sx)  (the reference's Δx = dx + c Δy, :328);
 //   asymmetric : A/4 * D(u_i, ex) * D(v_j, ey)  (:203-204; the two sign flips cancel).
 // For e < kSeriesEps, D is evaluated without the erf-difference cancellation:
-//   D = (4 e / sqrt(pi)) exp(-u^2) sum_n H_2n(u) e^2n / (2n+1)!
-// (mean of exp(-(u+s)^2) over |s| < e; Hermite polynomials H_2n), truncated
-// after n = 4 (relative truncation < 1e-9 for |u| < 6).
+//   D = (4 e / sqrt(pi)) exp(-u^2) S(u^2),  S(w) = sum_n H_2n(sqrt w) e^2n / (2n+1)!
+// (the mean of exp(-(u+s)^2) over |s| < e; Hermite polynomials H_2n), whose
+// coefficients in powers of w are kept up to w^DEG.  Relative to the local
+// value at |u| <= 4 the dropped terms are below ~1e-7 for DEG = 1 when
+// e < 0.0077, DEG = 2 when e < 0.026 and DEG = 3 when e < 0.1 (the terms of
+// order e^2n |H_2n(4)| / (2n+1)! are ~10 e^2, 28 e^4, 30 e^6, 15 e^8).
+//
+// The series forms run in scaled coordinates a' = a sqrt(log2 e), so
+// exp(-a^2) = exp2(-a'^2) is one hardware exp2, with the coefficients of S
+// rescaled to w' = a'^2.  The direct form shares erf values between adjacent
+// columns: the right edge of column c is the left edge of column c + 1.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,23 +39,35 @@ namespace sfh {
 namespace psf {
 
 constexpr float kSeriesEps = 0.1f;
+constexpr float kDeg1Eps = 0.0077f;
+constexpr float kDeg2Eps = 0.026f;
 constexpr double kGLNode = 0.7745966692414834;   // :303
 constexpr float kW0 = 0.5555555555555556f;      // :304
 constexpr float kW1 = 0.8888888888888888f;
+constexpr double kLog2e = 1.4426950408889634;
+constexpr double kSqrtLog2e = 1.2011224087864498;
+constexpr int kCPT = 16;                        // consecutive columns evaluated per thread
+
+// x evaluation of a point: the erf difference or the series of degree 1..3
+enum : int32_t { kDirect = 0, kSeries1 = 1, kSeries2 = 2, kSeries3 = 3 };
 
 // One prepared kernel: the clipped footprint and its FP32 parameters.
 // Footprint indices are 0-based and inclusive; xa > xb marks "nothing to
 // deposit" (the reference's length>1 guard, :359 / :401).
+// Series kinds store ax, du and g in the scaled coordinate a' (see header).
 struct alignas(16) PointDesc {
     int32_t xa, xb, ya, yb;
-    float ax;     // u of the first footprint column centre
-    float du;     // u step per column
-    float ex;     // half column width in u
-    float g;      // covariant: shear c*sy/(sqrt2 sx); asymmetric: unused
+    float ax;     // x coordinate of the first footprint column centre
+    float du;     // x step per column
+    float ex;     // half column width in u (direct form)
+    float g;      // covariant: shear c*sy/(sqrt2 sx); asymmetric: 0
     float ay;     // covariant: t of first footprint row centre; asymmetric: v of first row
     float dv;     // t (or v) step per row
     float ey;     // covariant: GL node offset xi*hy/sy; asymmetric: half row width in v
-    float pref;   // amplitude prefactor (see header)
+    float pref;   // amplitude prefactor (see header), with the series form factors
+    float s0, s1, s2, s3;   // x series coefficients in w' (series kinds)
+    int32_t kind;           // kDirect or kSeriesN for the x factor
+    int32_t pad0, pad1, pad2;
 };
 
 struct Range {
@@ -84,6 +104,47 @@ __host__ __device__ inline PointDesc empty_desc()
     return d;
 }
 
+// Coefficients of S(w) = s0 + s1 w + s2 w^2 + s3 w^3 + s4 w^4 (w = u^2) for
+// the series D = (4e/sqrt(pi)) exp(-w) S(w).
+struct Series {
+    float s0, s1, s2, s3, s4;
+};
+
+__host__ __device__ inline Series series_coeffs(float e)
+{
+    const float q = e * e;
+    const float c1 = q / 6.0f, c2 = q * q / 120.0f, c3 = q * q * q / 5040.0f,
+                c4 = q * q * q * q / 362880.0f;
+    Series s;
+    s.s0 = 1.0f - 2.0f * c1 + 12.0f * c2 - 120.0f * c3 + 1680.0f * c4;
+    s.s1 = 4.0f * c1 - 48.0f * c2 + 720.0f * c3 - 13440.0f * c4;
+    s.s2 = 16.0f * c2 - 480.0f * c3 + 13440.0f * c4;
+    s.s3 = 64.0f * c3 - 3584.0f * c4;
+    s.s4 = 256.0f * c4;
+    return s;
+}
+
+// The x-factor form of half width e (in u): its kind, and for the series
+// kinds the coefficients rescaled to w' = w log2(e) in d.
+__host__ __device__ inline void set_x_form(PointDesc* d, float e)
+{
+    if (!(e < kSeriesEps)) {
+        d->kind = kDirect;
+        return;
+    }
+    d->kind = e < kDeg1Eps ? kSeries1 : e < kDeg2Eps ? kSeries2 : kSeries3;
+    const Series s = series_coeffs(e);
+    const double l1 = 1.0 / kLog2e, l2 = l1 * l1, l3 = l2 * l1;
+    d->s0 = s.s0;
+    d->s1 = (float)(s.s1 * l1);
+    d->s2 = (float)(s.s2 * l2);
+    d->s3 = (float)(s.s3 * l3);
+    // coordinates of the series forms in a' = a sqrt(log2 e)
+    d->ax = (float)((double)d->ax * kSqrtLog2e);
+    d->du = (float)((double)d->du * kSqrtLog2e);
+    d->g = (float)((double)d->g * kSqrtLog2e);
+}
+
 // Covariant kernel centred at (x0, y0) (real units) -- addstar! :366-408.
 // Returns the number of (point, bin) pairs in the footprint (0 if skipped).
 // The FP64 expressions keep the reference's operation order; compile with
@@ -113,7 +174,7 @@ __host__ __device__ inline int64_t prep_covariant(double x0, double y0, double s
     const double xst = (xe - xs) / (double)(lx - 1), yst = (ye - ys) / (double)(ly - 1);
     const double hx = xst / 2.0, hy = yst / 2.0;
     const double s2x = 1.4142135623730951 * sx;
-    PointDesc d;
+    PointDesc d{};
     d.xa = (int32_t)(xa - 1);
     d.xb = (int32_t)(xb - 1);
     d.ya = (int32_t)(ya - 1);
@@ -128,6 +189,7 @@ __host__ __device__ inline int64_t prep_covariant(double x0, double y0, double s
     double pref = A * hy / (2.0 * 2.5066282746310002 * sy);          // prefac*h_y, :318,:323
     if (d.ex < kSeriesEps) pref *= 4.0 * hx / s2x / 1.7724538509055160;   // series form factor
     d.pref = (float)pref;
+    set_x_form(&d, d.ex);
     *out = d;
     return lx * ly;
 }
@@ -154,7 +216,7 @@ __host__ __device__ inline int64_t prep_asymmetric(double col, double mag, doubl
         return 0;
     }
     const double s2x = 1.4142135623730951 * sx, s2y = 1.4142135623730951 * sy;
-    PointDesc d;
+    PointDesc d{};
     d.xa = (int32_t)(xa - 1);
     d.xb = (int32_t)(xb - 1);
     d.ya = (int32_t)(ya - 1);
@@ -170,28 +232,9 @@ __host__ __device__ inline int64_t prep_asymmetric(double col, double mag, doubl
     if (d.ex < kSeriesEps) pref *= 4.0 * (0.5 / s2x) / 1.7724538509055160;
     if (d.ey < kSeriesEps) pref *= 4.0 * (0.5 / s2y) / 1.7724538509055160;
     d.pref = (float)pref;
+    set_x_form(&d, d.ex);
     *out = d;
     return lx * ly;
-}
-
-// Coefficients of S(w) = s0 + s1 w + s2 w^2 + s3 w^3 + s4 w^4 (w = u^2) for
-// the series D = (4e/sqrt(pi)) exp(-w) S(w).
-struct Series {
-    float s0, s1, s2, s3, s4;
-};
-
-__host__ __device__ inline Series series_coeffs(float e)
-{
-    const float q = e * e;
-    const float c1 = q / 6.0f, c2 = q * q / 120.0f, c3 = q * q * q / 5040.0f,
-                c4 = q * q * q * q / 362880.0f;
-    Series s;
-    s.s0 = 1.0f - 2.0f * c1 + 12.0f * c2 - 120.0f * c3 + 1680.0f * c4;
-    s.s1 = 4.0f * c1 - 48.0f * c2 + 720.0f * c3 - 13440.0f * c4;
-    s.s2 = 16.0f * c2 - 480.0f * c3 + 13440.0f * c4;
-    s.s3 = 64.0f * c3 - 3584.0f * c4;
-    s.s4 = 256.0f * c4;
-    return s;
 }
 
 __host__ __device__ inline float fast_exp(float x)
@@ -203,7 +246,17 @@ __host__ __device__ inline float fast_exp(float x)
 #endif
 }
 
-// exp(-u^2) * S(u^2)  (the series form of D without its 4e/sqrt(pi) factor)
+__host__ __device__ inline float fast_exp2(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_exp2f(x);
+#else
+    return std::exp2(x);
+#endif
+}
+
+// exp(-u^2) * S(u^2)  (the full-degree series form of D without its 4e/sqrt(pi)
+// factor; the asymmetric kernel's y factor)
 __host__ __device__ inline float d_series(float u, const Series& s)
 {
     float w = u * u;
@@ -212,9 +265,146 @@ __host__ __device__ inline float d_series(float u, const Series& s)
     return fast_exp(-w) * poly;
 }
 
+__host__ __device__ inline float fast_rcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
+// erfc(x) for x >= 0, branch-free: the Chebyshev-fitted form of Numerical
+// Recipes (Press et al., 2nd ed., section 6.2, erfcc), fractional error below
+// 1.2e-7 for every x >= 0 in exact arithmetic.
+__host__ __device__ inline float erfc_pos(float x)
+{
+    const float t = fast_rcp(fmaf(0.5f, x, 1.0f));
+    float p = 0.17087277f;
+    p = fmaf(p, t, -0.82215223f);
+    p = fmaf(p, t, 1.48851587f);
+    p = fmaf(p, t, -1.13520398f);
+    p = fmaf(p, t, 0.27886807f);
+    p = fmaf(p, t, -0.18628806f);
+    p = fmaf(p, t, 0.09678418f);
+    p = fmaf(p, t, 0.37409196f);
+    p = fmaf(p, t, 1.00002368f);
+    p = fmaf(p, t, -1.26551223f);
+    return t * fast_exp(fmaf(-x, x, p));
+}
+
+// erf(u + e) - erf(u - e) through erfc of non-negative arguments (D is even
+// in u), so that it keeps its relative accuracy in the tails
 __host__ __device__ inline float d_direct(float u, float e)
 {
-    return erff(u + e) - erff(u - e);
+    const float a = fabsf(u);
+    const float hi = erfc_pos(a + e);
+    return a >= e ? erfc_pos(a - e) - hi : 2.0f - hi - erfc_pos(e - a);
+}
+
+// The asymmetric kernel's y factor of a row at coordinate v (half height ey).
+__host__ __device__ inline float y_factor(float v, float ey)
+{
+    return ey < kSeriesEps ? d_series(v, series_coeffs(ey)) : d_direct(v, ey);
+}
+
+// Row weights of the covariant kernel: W_k = pref w_k exp(-t_k^2/2) at the GL
+// nodes t_k = tj + (k-1) ey, and the x shifts h_k = g t_k.
+__host__ __device__ inline void cov_row(const PointDesc& d, int row, float* W, float* h)
+{
+    const float tj = fmaf((float)(row - d.ya), d.dv, d.ay);
+    const float t0 = tj - d.ey, t2 = tj + d.ey;
+    const float hl = (float)(-0.5 * kLog2e);
+    W[0] = d.pref * kW0 * fast_exp2(hl * t0 * t0);
+    W[1] = d.pref * kW1 * fast_exp2(hl * tj * tj);
+    W[2] = d.pref * kW0 * fast_exp2(hl * t2 * t2);
+    h[0] = d.g * t0;
+    h[1] = d.g * tj;
+    h[2] = d.g * t2;
+}
+
+// Adds the contributions of one point to the kCPT columns col0 .. col0+kCPT-1
+// of one row: part[c] += sum_k W[k] X(a_k(c)), a_k(c) the x coordinate of
+// column col0 + c shifted by h[k] (NODES = 3: covariant GL nodes; NODES = 1:
+// asymmetric, W[0] = pref * y factor).  DEG: the x form (kDirect or a series
+// degree).  MASK: only columns inside [xa, xb] receive anything (without it
+// every column does: the footprint spans the whole grid and columns past the
+// grid are never written).
+template <int NODES, int DEG, bool MASK>
+__host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const float* W, const float* h, int col0,
+                                                   float* part)
+{
+    const float base = fmaf((float)(col0 - d.xa), d.du, d.ax);
+    const int lo = d.xa - col0, hi = d.xb - col0;
+    if constexpr (DEG == kDirect) {
+        // erf at the column edges b(c) = a(c) - ex, c = 0..kCPT, as erfc(|b|) so
+        // that D(c) = erf(b(c+1)) - erf(b(c)) keeps its relative accuracy in
+        // the tails: both edges >= 0: erfc(b0) - erfc(b1); both < 0:
+        // erfc(-b1) - erfc(-b0); across 0: 2 - erfc(b1) - erfc(-b0)
+        float prev[NODES];
+        bool pneg[NODES];
+#pragma unroll
+        for (int k = 0; k < NODES; ++k) {
+            const float b0 = base + h[k] - d.ex;
+            pneg[k] = b0 < 0.0f;
+            prev[k] = erfc_pos(fabsf(b0));
+        }
+#pragma unroll
+        for (int c = 0; c < kCPT; ++c) {
+            float v = part[c];
+#pragma unroll
+            for (int k = 0; k < NODES; ++k) {
+                const float b1 = fmaf((float)(c + 1), d.du, base + h[k] - d.ex);
+                const bool neg = b1 < 0.0f;
+                const float e1 = erfc_pos(fabsf(b1));
+                const float dd = neg ? e1 - prev[k] : (pneg[k] ? 2.0f - e1 - prev[k] : prev[k] - e1);
+                v = fmaf(W[k], dd, v);
+                prev[k] = e1;
+                pneg[k] = neg;
+            }
+            if (MASK) part[c] = (c >= lo && c <= hi) ? v : part[c];
+            else part[c] = v;
+        }
+    } else {
+        float a0[NODES], c0[NODES], c1[NODES], c2[NODES], c3[NODES];
+#pragma unroll
+        for (int k = 0; k < NODES; ++k) {
+            a0[k] = base + h[k];
+            c0[k] = W[k] * d.s0;
+            c1[k] = W[k] * d.s1;
+            c2[k] = DEG >= 2 ? W[k] * d.s2 : 0.0f;
+            c3[k] = DEG >= 3 ? W[k] * d.s3 : 0.0f;
+        }
+#pragma unroll
+        for (int c = 0; c < kCPT; ++c) {
+            float v = part[c];
+#pragma unroll
+            for (int k = 0; k < NODES; ++k) {
+                const float a = fmaf((float)c, d.du, a0[k]);
+                const float w = a * a;
+                float S;
+                if constexpr (DEG == 1) S = fmaf(c1[k], w, c0[k]);
+                else if constexpr (DEG == 2) S = fmaf(fmaf(c2[k], w, c1[k]), w, c0[k]);
+                else S = fmaf(fmaf(fmaf(c3[k], w, c2[k]), w, c1[k]), w, c0[k]);
+                v = fmaf(S, fast_exp2(-w), v);
+            }
+            if (MASK) part[c] = (c >= lo && c <= hi) ? v : part[c];
+            else part[c] = v;
+        }
+    }
+}
+
+// One point's contributions to one row's strip, dispatched on its form.
+template <int NODES, bool MASK>
+__host__ __device__ __forceinline__ void add_point(const PointDesc& d, const float* W, const float* h, int col0,
+                                                   float* part)
+{
+    switch (d.kind) {
+    case kSeries1: add_strip<NODES, 1, MASK>(d, W, h, col0, part); break;
+    case kSeries2: add_strip<NODES, 2, MASK>(d, W, h, col0, part); break;
+    case kSeries3: add_strip<NODES, 3, MASK>(d, W, h, col0, part); break;
+    default: add_strip<NODES, kDirect, MASK>(d, W, h, col0, part); break;
+    }
 }
 
 }  // namespace psf

@@ -18,7 +18,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._lib import ArgumentError
-from .core import HIPData, HIPTemplates, composite_, fg_, fg_batched, loglikelihood
+from .core import HIPData, HIPTemplates, composite_, driver_clamps, fg_, fg_batched, loglikelihood, stack_models
 
 
 def _models_data(models, data):
@@ -36,9 +36,16 @@ def _data_sum(data) -> float:
     return float(np.sum(np.asarray(data, dtype=np.float64)))
 
 
-def renormalize_x0(data, models: HIPTemplates, x0, full_coeffs=None):
+def renormalize_x0(data, models, x0, full_coeffs=None):
     """renormalize_x0 (src/fitting/utilities.jl:104-115): scale x0 by
-    sum(data) / sum(composite(full_coeffs))."""
+    sum(data) / sum(composite(full_coeffs)).  ``models`` is a HIPTemplates, a
+    flattened nbins x ntemplates matrix, or a list of 2-D templates with
+    ``data`` a matrix (the vector-of-matrices method, :117-122)."""
+    if not isinstance(models, HIPTemplates):
+        if isinstance(models, (list, tuple)):
+            models = stack_models(models)
+            data = np.asarray(data).ravel(order="F")
+        models = HIPTemplates(models)
     full = np.asarray(x0 if full_coeffs is None else full_coeffs, dtype=np.float64)
     comp = np.empty(models.nbins)
     composite_(comp, full, models)
@@ -99,15 +106,16 @@ def fit_templates(models, data, x0=None, **kws):
         raise ArgumentError("axes(coeffs,1) == axes(models,2)")
     x0 = np.log(renormalize_x0(data, tpl, x0))
     G = np.empty(T)
+    e, eg = driver_clamps(tpl, data)               # composite::Vector{S} (:173)
 
     def fg_map(logx):                              # :181-188
         x = np.exp(logx)
-        f = fg_(True, G, x, tpl, hd) - np.sum(logx)
+        f = fg_(True, G, x, tpl, hd, eps=e, eps_grad=eg) - np.sum(logx)
         return f, G * x - 1.0
 
     def fg_mle(logx):                              # :189-196
         x = np.exp(logx)
-        f = fg_(True, G, x, tpl, hd)
+        f = fg_(True, G, x, tpl, hd, eps=e, eps_grad=eg)
         return f, G * x
 
     r_map = _bfgs(fg_map, x0, kws)
@@ -128,10 +136,11 @@ def fit_templates_fast(models, data, x0=None, **kws):
     x0 = np.ones(T) if x0 is None else np.asarray(x0, dtype=np.float64)
     x0 = np.sqrt(renormalize_x0(data, tpl, x0))
     G = np.empty(T)
+    e, eg = driver_clamps(tpl, data)               # composite::Vector{S} (:250)
 
     def fg_mle(sq):                                # :257-264
         x = sq * sq
-        f = fg_(True, G, x, tpl, hd)
+        f = fg_(True, G, x, tpl, hd, eps=e, eps_grad=eg)
         return f, G * 2.0 * sq
 
     r = _bfgs(fg_mle, x0, kws)
@@ -147,9 +156,10 @@ def fit_templates_lbfgsb(models, data, x0=None, factr=1e-12, pgtol=1e-5, **kws):
     x0 = np.ones(T) if x0 is None else np.asarray(x0, dtype=np.float64)
     x0 = renormalize_x0(data, tpl, x0)
     G = np.empty(T)
+    e, eg = driver_clamps(tpl, data)               # composite::Vector{S} (:84)
 
     def fg(x):                                     # :88
-        f = fg_(True, G, x, tpl, hd)
+        f = fg_(True, G, x, tpl, hd, eps=e, eps_grad=eg)
         return f, G.copy()
 
     opts = {"ftol": factr * np.finfo(np.float64).eps, "gtol": pgtol, "maxiter": 15000, "maxfun": 15000}
@@ -164,26 +174,30 @@ class HMCModel:
 
     def __init__(self, models, data):
         self.models, self.data = _models_data(models, data)
+        # hmc_sample allocates composite::Vector{S}, S = eltype(models) (hmc_sample.jl:107)
+        self.eps, self.eps_grad = driver_clamps(self.models, data)
 
     def dimension(self):
         return self.models.ntemplates
 
     def logdensity(self, theta):                   # :11-21
         theta = np.asarray(theta, dtype=np.float64)
-        return loglikelihood(np.exp(theta), self.models, self.data) + theta.sum()
+        negll = fg_(True, None, np.exp(theta), self.models, self.data, eps=self.eps, eps_grad=self.eps_grad)
+        return -negll + theta.sum()
 
     def logdensity_and_gradient(self, logx):       # :24-37
         logx = np.asarray(logx, dtype=np.float64)
         x = np.exp(logx)
         G = np.empty_like(x)
-        ll = -fg_(True, G, x, self.models, self.data) + logx.sum()
+        ll = -fg_(True, G, x, self.models, self.data, eps=self.eps, eps_grad=self.eps_grad) + logx.sum()
         return ll, -G * x + 1.0
 
     def logdensity_and_gradient_batched(self, logX, precision="f64"):
         """The same for every column of logX (T x B) in one batched launch."""
         logX = np.asarray(logX, dtype=np.float64)
         X = np.exp(logX)
-        negll, G = fg_batched(X, self.models, self.data, precision=precision)
+        negll, G = fg_batched(X, self.models, self.data, precision=precision, eps=self.eps,
+                              eps_grad=self.eps_grad)
         return -negll + logX.sum(axis=0), -G * X + 1.0
 
 
@@ -193,12 +207,14 @@ class MCMCModel:
 
     def __init__(self, models, data):
         self.models, self.data = _models_data(models, data)
+        # C::Vector{T}, T = promote_type(eltype(models), eltype(data)) (mcmc_sample.jl:15,20)
+        self.eps = driver_clamps(self.models, data)[0]
 
     def __call__(self, theta):
         theta = np.asarray(theta, dtype=np.float64)
         if np.any(theta < 0):
             return -np.inf
-        return loglikelihood(theta, self.models, self.data)
+        return -fg_(True, None, theta, self.models, self.data, eps=self.eps)
 
     def batched(self, X, precision="f64"):
         """Log-likelihood of every column of X (T x B); -Inf for columns with a
@@ -206,7 +222,7 @@ class MCMCModel:
         X = np.asarray(X, dtype=np.float64)
         bad = np.any(X < 0, axis=0)
         Xs = np.where(bad[None, :], 1.0, X)
-        negll, _ = fg_batched(Xs, self.models, self.data, precision=precision, want_G=False)
+        negll, _ = fg_batched(Xs, self.models, self.data, precision=precision, want_G=False, eps=self.eps)
         ll = -negll
         ll[bad] = -np.inf
         return ll

@@ -19,7 +19,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._lib import ArgumentError
-from .core import HIPData, HIPTemplates, fg_
+from .core import HIPData, HIPTemplates, _eps_of, driver_clamps, fg_
 
 _LN10 = math.log(10.0)
 
@@ -309,7 +309,10 @@ def _fg_hierarchical(F, G, MH_model0, disp_model0, variables, models, data, logA
     Asum = np.bincount(jidx, weights=A, minlength=nR)
     coeffs = A * R[jidx] / Asum[jidx]
     Gc = np.empty(coeffs.size) if G is not None else None
-    nll = fg_(True, Gc, coeffs, models, data)
+    # composite = similar(data, S), S = promote_type(eltype(models), eltype(data))
+    # (generic_fitting.jl:97-98): both clamps are eps(S)
+    e = _eps_of(models, data)
+    nll = fg_(True, Gc, coeffs, models, data, eps=e, eps_grad=e)
     if G is None:
         return nll if F is not None else None
     if G.size != variables.size:
@@ -487,11 +490,12 @@ def fit_templates_fixed_amr(models, data, logAge, metallicities, relweights, x0=
         from .fitting import renormalize_x0
         x0 = renormalize_x0(np.asarray(data, dtype=np.float64), tpl, x0, rw * x0[jidx])
     Gc = np.empty(jidx.size)
+    e, eg = driver_clamps(tpl, data)               # composite::Vector{S} (fixed_amr.jl:50)
 
     def fg(xvec, mapj):
         x = np.exp(xvec)
         coeffs = rw * x[jidx]
-        nll = fg_(True, Gc, coeffs, tpl, hd)
+        nll = fg_(True, Gc, coeffs, tpl, hd, eps=e, eps_grad=eg)
         g = np.bincount(jidx, weights=Gc * coeffs, minlength=ula.size)          # :122, :145
         return (nll - np.sum(xvec), g - 1.0) if mapj else (nll, g)
 

@@ -102,24 +102,30 @@ class MultiFit:
         return out
 
     # ------------------------------------------------------------ fg!
-    def fg(self, coeffs, want_G: bool = True, eps: float = _f64eps):
+    def fg(self, coeffs, want_G: bool = True, eps: float = _f64eps, eps_grad: float | None = None):
         """fg!(true, G, coeffs_f, models_f, data_f) for every fit: returns
-        (-logL per fit, list of -∇logL per fit or None)."""
+        (-logL per fit, list of -∇logL per fit or None).  ``eps`` clamps the
+        log-likelihood, ``eps_grad`` (default ``eps``) the gradient, as sfh_fg."""
+        eps_grad = eps if eps_grad is None else eps_grad
         X = self.concat(coeffs)
         nl = np.empty(self.nfits)
         G = np.empty(X.size) if want_G else None
-        self.ctx.check(self.ctx.lib.sfh_multifit_fg(self.handle, X.ctypes.data, float(eps), nl.ctypes.data,
+        self.ctx.check(self.ctx.lib.sfh_multifit_fg(self.handle, X.ctypes.data, float(eps), float(eps_grad),
+                                                    nl.ctypes.data,
                                                     G.ctypes.data if want_G else None))
         return nl, (self.split(G) if want_G else None)
 
-    def fg_device(self, d_X, d_negLL, d_G=None, eps: float = _f64eps):
+    def fg_device(self, d_X, d_negLL, d_G=None, eps: float = _f64eps, eps_grad: float | None = None):
         """Asynchronous form on device buffers (concatenated X and G)."""
+        eps_grad = eps if eps_grad is None else eps_grad
         from .core import _ptr
-        self.ctx.check(self.ctx.lib.sfh_multifit_fg_device(self.handle, _ptr(d_X), float(eps), _ptr(d_negLL),
+        self.ctx.check(self.ctx.lib.sfh_multifit_fg_device(self.handle, _ptr(d_X), float(eps), float(eps_grad),
+                                                           _ptr(d_negLL),
                                                            _ptr(d_G)))
 
     # ------------------------------------------------------------ fits
-    def _fit(self, x0, transform, method, memory, g_abstol, iterations, steps_per_graph, eps, want_invH):
+    def _fit(self, x0, transform, method, memory, g_abstol, iterations, steps_per_graph, eps, want_invH,
+             eps_grad=None):
         opt = _lib.sfh_fit_options()
         self.ctx.lib.sfh_fit_options_default(C.byref(opt))
         opt.transform = transform
@@ -129,6 +135,7 @@ class MultiFit:
         opt.max_iterations = int(iterations)
         opt.steps_per_graph = int(steps_per_graph)
         opt.eps = float(eps)
+        opt.eps_grad = float(eps if eps_grad is None else eps_grad)
         nT = int(self.offsets[-1])
         nH = int(self.hoffsets[-1])
         mu_map = np.zeros(nT)
@@ -162,13 +169,17 @@ class MultiFit:
         return H[self.hoffsets[f]:self.hoffsets[f + 1]].reshape(T, T)
 
     def fit_templates(self, x0=None, method: str = "bfgs", memory: int = 10, g_abstol: float = 1e-8,
-                      iterations: int = 1000, steps_per_graph: int = 16, eps: float = _f64eps):
+                      iterations: int = 1000, steps_per_graph: int = 16, eps: float = _f64eps,
+                      eps_grad: float | None = None):
         """fit_templates (src/fitting/solvers.jl:172-221) for every fit:
         returns a list of {"map": LogTransformFTResult, "mle": ...}.  With
-        method="lbfgs" the inverse Hessian (and so σ) is not available."""
+        method="lbfgs" the inverse Hessian (and so σ) is not available.
+        ``eps`` / ``eps_grad`` are the log-likelihood's and the gradient's clamps
+        (eps(Float64) both by default; the reference's composite has the
+        models' element type, solvers.jl:173)."""
         m = {"bfgs": _lib.SFH_OPT_BFGS, "lbfgs": _lib.SFH_OPT_LBFGS}[method]
         mu_map, mu_mle, Hmap, Hmle, fmin, iters, evals, status = self._fit(
-            x0, _lib.SFH_FIT_LOG, m, memory, g_abstol, iterations, steps_per_graph, eps, True)
+            x0, _lib.SFH_FIT_LOG, m, memory, g_abstol, iterations, steps_per_graph, eps, True, eps_grad)
         out = []
         for f in range(self.nfits):
             sl = slice(self.offsets[f], self.offsets[f + 1])
@@ -182,11 +193,12 @@ class MultiFit:
         return out
 
     def fit_templates_fast(self, x0=None, method: str = "bfgs", memory: int = 10, g_abstol: float = 1e-8,
-                           iterations: int = 1000, steps_per_graph: int = 16, eps: float = _f64eps):
+                           iterations: int = 1000, steps_per_graph: int = 16, eps: float = _f64eps,
+                           eps_grad: float | None = None):
         """fit_templates_fast (src/fitting/solvers.jl:248-275) for every fit:
         returns a list of (coeffs, StageInfo) with coeffs = θ² at the MLE."""
         m = {"bfgs": _lib.SFH_OPT_BFGS, "lbfgs": _lib.SFH_OPT_LBFGS}[method]
         _, mu_mle, _, _, fmin, iters, evals, status = self._fit(
-            x0, _lib.SFH_FIT_SQRT, m, memory, g_abstol, iterations, steps_per_graph, eps, False)
+            x0, _lib.SFH_FIT_SQRT, m, memory, g_abstol, iterations, steps_per_graph, eps, False, eps_grad)
         return [(mu_mle[self.offsets[f]:self.offsets[f + 1]].copy(), self._info(1, f, fmin, iters, evals, status))
                 for f in range(self.nfits)]

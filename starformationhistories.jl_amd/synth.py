@@ -14,7 +14,9 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .templates import Martin2016_complete, Range, edges_range, exp_photerr
+from . import _lib
+from .templates import (ExpPhotErr, Martin2016Completeness, Martin2016_complete, Range, edges_range, exp_photerr,
+                        partial_cmd_smooth_batch)
 
 
 class Kroupa2001:
@@ -50,6 +52,26 @@ class Kroupa2001:
 
     def mean(self):
         return self._norm * self._integral(lambda a, lo, hi: self._pow_int(a, lo, hi), shift=1.0)
+
+    def as_fn(self, lo=None, hi=None):
+        """The device form: normalisation x coefs[k] m^-slopes[k] on the
+        segments of [mmin, mmax] split at 0.5 Msun."""
+        if self.mmin < 0.5 < self.mmax:
+            br, sl, co = [self.mmin, 0.5, self.mmax], [1.3, 2.3], [1.0, 0.5]
+        elif self.mmax <= 0.5:
+            br, sl, co = [self.mmin, self.mmax], [1.3], [1.0]
+        else:
+            br, sl, co = [self.mmin, self.mmax], [2.3], [0.5]
+        self._c = [np.ascontiguousarray(v, dtype=np.float64) for v in (br, sl, co)]
+        fn = _lib.sfh_fn(_lib.SFH_FN_POWERLAW)
+        fn.p[0] = self._norm
+        fn.powerlaw = _lib.sfh_imf(len(sl), *(v.ctypes.data for v in self._c))
+        return fn
+
+
+# the same error and completeness models as objects the device evaluates exactly
+PHOTERR = ExpPhotErr(math.e, 0.6, 28.0, 0.01)
+COMPLETENESS = Martin2016Completeness(1.0, 28.0, 0.7)
 
 
 def photerr(m):
@@ -154,6 +176,20 @@ def template_points(spec: TemplateSpec, npoints, edges, mmax_imf, dmod=24.0, nor
         r = float(correlation)
         cm = -r / math.sqrt(1.0 - r * r) * sp.color_err / sp.mag_err
     return sp, cm
+
+
+def templates_device(specs, npoints, edges, mmax_imf, dmod=24.0, normalize_value=1.0, correlation=0.0,
+                     output="device", return_info=False, ctx=None):
+    """The isochrones of ``specs`` built by partial_cmd_smooth_batch (the
+    preparation on the device): y = V, colour = B - V, the same error,
+    completeness and IMF as template_points.  The synthetic isochrones are
+    generated on the host first (inputs, not part of the template path)."""
+    imf = Kroupa2001(0.08, mmax_imf)
+    isos = [isochrone(sp.log_age, sp.mh, npoints, sp.seed, mmax_imf=mmax_imf) for sp in specs]
+    return partial_cmd_smooth_batch(isos, [PHOTERR, PHOTERR], 1, (0, 1), imf, [COMPLETENESS, COMPLETENESS],
+                                    dmod=dmod, normalize_value=normalize_value, mean_mass=imf.mean(),
+                                    correlation=correlation, edges=edges, output=output, return_info=return_info,
+                                    ctx=ctx)
 
 
 def poisson_data(composite, seed):

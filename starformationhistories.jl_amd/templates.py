@@ -619,3 +619,167 @@ def deposit_batch(batch: PointBatch, out=None, ctx: Context | None = None, retur
 def templates_from_device(out, nbins: int, ctx: Context | None = None) -> HIPTemplates:
     """Wrap deposited device templates as the likelihood path's HIPTemplates."""
     return HIPTemplates(ctx=ctx, device_ptr=out.data_ptr(), shape=(nbins, out.shape[0]))
+
+
+# ------------------------------------------ partial_cmd_smooth prepared on the device
+
+class ExpPhotErr:
+    """m -> exp_photerr(m, a, b, c, d) (src/utilities.jl:218) as a callable
+    the device evaluates exactly."""
+
+    def __init__(self, a, b, c, d):
+        self.a, self.b, self.c, self.d = float(a), float(b), float(c), float(d)
+
+    def __call__(self, m):
+        return exp_photerr(m, self.a, self.b, self.c, self.d)
+
+    def as_fn(self, lo=None, hi=None):
+        return _lib.sfh_fn(_lib.SFH_FN_EXP_PHOTERR, (C.c_double * 4)(self.a, self.b, self.c, self.d))
+
+
+class Martin2016Completeness:
+    """m -> Martin2016_complete(m, A, m50, rho) (src/utilities.jl:205) as a
+    callable the device evaluates exactly."""
+
+    def __init__(self, A, m50, rho):
+        self.A, self.m50, self.rho = float(A), float(m50), float(rho)
+
+    def __call__(self, m):
+        return Martin2016_complete(m, self.A, self.m50, self.rho)
+
+    def as_fn(self, lo=None, hi=None):
+        return _lib.sfh_fn(_lib.SFH_FN_MARTIN2016, (C.c_double * 4)(self.A, self.m50, self.rho, 0.0))
+
+
+class Constant:
+    """m -> value (the reference's default `one` / `zero` callables)."""
+
+    def __init__(self, value):
+        self.value = float(value)
+
+    def __call__(self, m):
+        return np.full(np.shape(m), self.value)
+
+    def as_fn(self, lo=None, hi=None):
+        return _lib.sfh_fn(_lib.SFH_FN_CONST, (C.c_double * 4)(self.value, 0.0, 0.0, 0.0))
+
+
+class Tabulated:
+    """Any callable f, tabulated on n points over [lo, hi] and interpolated
+    linearly by the device (clamped outside): the route for user callables the
+    device cannot evaluate directly.  The interpolation error is at most
+    h^2/8 max|f''| with h = (hi - lo)/(n - 1)."""
+
+    def __init__(self, f, lo, hi, n=16385):
+        if not (hi > lo and n >= 2):
+            raise ArgumentError("Tabulated needs hi > lo and n >= 2")
+        self.f, self.lo, self.hi, self.n = f, float(lo), float(hi), int(n)
+        self.step = (self.hi - self.lo) / (self.n - 1)
+        self.values = np.ascontiguousarray(_apply(f, self.lo + self.step * np.arange(self.n)), dtype=np.float64)
+
+    def __call__(self, m):
+        return np.interp(np.asarray(m, dtype=np.float64), self.lo + self.step * np.arange(self.n), self.values)
+
+    def as_fn(self, lo=None, hi=None):
+        fn = _lib.sfh_fn(_lib.SFH_FN_TABLE)
+        fn.table = _lib.sfh_table(self.lo, self.step, self.n, self.values.ctypes.data)
+        return fn
+
+
+def _fn_of(f, lo, hi, keep):
+    """The sfh_fn of callable ``f`` over arguments in [lo, hi]: its own form
+    when it has one (``as_fn``), else a table (kept alive in ``keep``)."""
+    if hasattr(f, "as_fn"):
+        keep.append(f)
+        return f.as_fn(lo, hi)
+    if not hi > lo:
+        hi = lo + 1.0
+    pad = 1e-9 * max(1.0, abs(lo), abs(hi))
+    t = Tabulated(f, lo - pad, hi + pad)
+    keep.append(t)
+    return t.as_fn()
+
+
+def partial_cmd_smooth_batch(isochrones, mag_err_funcs, y_index, color_indices, imf, completeness_funcs=None,
+                             bias_funcs=None, *, dmod=0.0, normalize_value=1.0, mean_mass=None, correlation=0.0,
+                             edges=None, xlim=None, ylim=None, nbins=None, xwidth=None, ywidth=None,
+                             output="device", return_info=False, ctx: Context | None = None):
+    """partial_cmd_smooth (src/StarFormationHistories.jl:829-893, NoBinaries)
+    for many isochrones in one call (sfh_partial_cmd_smooth_batch): the host
+    preparation of every isochrone runs on the device, then one deposition
+    launch.  ``isochrones``: a sequence of (m_ini, mags) with m_ini strictly
+    increasing and mags a sequence of per-filter magnitude arrays.  The
+    callables are shared by all isochrones; ``dmod``, ``normalize_value`` and
+    ``mean_mass`` may be scalars or one value per isochrone.  Callables with an
+    ``as_fn`` form (ExpPhotErr, Martin2016Completeness, Constant, Tabulated,
+    power-law IMFs) are evaluated exactly; any other callable is tabulated on
+    16385 points over the magnitudes (or masses) it will see.
+    ``correlation`` r != 0 replaces cov_mult by c = -r/sqrt(1-r^2) color_err/mag_err
+    per kernel point (a colour-magnitude correlation, not in the reference).
+    Returns a device tensor (ntemplates, nbins) (``output="device"``) or a host
+    array nbins x ntemplates (``"host"``); with ``return_info`` also
+    (kernel points, pairs)."""
+    import torch
+    ctx = ctx or default_context()
+    T = len(isochrones)
+    if T == 0:
+        raise ArgumentError("need at least one isochrone")
+    if len(color_indices) != 2:
+        raise ArgumentError("length(color_indices) == 2")
+    nf = len(isochrones[0][1])
+    if completeness_funcs is None:
+        completeness_funcs = [None] * nf
+    if bias_funcs is None:
+        bias_funcs = [None] * nf
+    if not (nf == len(mag_err_funcs) == len(completeness_funcs) == len(bias_funcs)):
+        raise ArgumentError("length(mags) == length(mag_err_funcs) == length(completeness_funcs) == "
+                            "length(bias_funcs)")
+    for i in (y_index, *color_indices):
+        if not 0 <= i < nf:
+            raise ArgumentError("y_index / color_indices out of range")
+    ex, ey = calculate_edges(edges, xlim, ylim, nbins, xwidth, ywidth)
+    m_cat = [np.asarray(m, dtype=np.float64).ravel() for m, _ in isochrones]
+    counts = np.array([m.size for m in m_cat], dtype=np.int64)
+    offsets = np.ascontiguousarray(np.concatenate([[0], np.cumsum(counts)]), dtype=np.int64)
+    m_all = np.ascontiguousarray(np.concatenate(m_cat))
+    mags_all = []
+    for f in range(nf):
+        arrs = [np.asarray(mg[f], dtype=np.float64).ravel() for _, mg in isochrones]
+        if any(a.size != n for a, n in zip(arrs, counts)):
+            raise ArgumentError("axes(m_ini) == axes(mags[i])")
+        mags_all.append(np.ascontiguousarray(np.concatenate(arrs)))
+    per = lambda v: np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (T,)))  # noqa: E731
+    dm = per(dmod)
+    if mean_mass is None:
+        mean_mass = imf_mean(imf)
+    nv, mm = per(normalize_value), per(mean_mass)
+    keep = [m_all, mags_all, dm, nv, mm, offsets]
+    # argument ranges the callables will see: the filters' magnitudes plus dmod
+    lo = [float(a.min() + dm.min()) if a.size else 0.0 for a in mags_all]
+    hi = [float(a.max() + dm.max()) if a.size else 1.0 for a in mags_all]
+    FnArr = _lib.sfh_fn * nf
+    err = FnArr(*[_fn_of(mag_err_funcs[f], lo[f], hi[f], keep) for f in range(nf)])
+    comp = FnArr(*[(_fn_of(completeness_funcs[f], lo[f], hi[f], keep) if completeness_funcs[f] is not None
+                    else Constant(1.0).as_fn()) for f in range(nf)])
+    bias = FnArr(*[(_fn_of(bias_funcs[f], lo[f], hi[f], keep) if bias_funcs[f] is not None
+                    else Constant(0.0).as_fn()) for f in range(nf)])
+    imf_fn = _fn_of(imf, float(m_all.min()) if m_all.size else 0.0, float(m_all.max()) if m_all.size else 1.0, keep)
+    mag_ptrs = (C.c_void_p * nf)(*[a.ctypes.data for a in mags_all])
+    nb = (ex.length - 1) * (ey.length - 1)
+    npts, pairs = C.c_int64(), C.c_int64()
+    if output == "device":
+        out = torch.empty((T, nb), dtype=torch.float32, device="cuda")
+        hptr, dptr = None, out.data_ptr()
+    elif output == "host":
+        out = np.empty((nb, T), dtype=np.float32, order="F")
+        hptr, dptr = out.ctypes.data, None
+    else:
+        raise ArgumentError('output must be "device" or "host"')
+    ctx.check(ctx.lib.sfh_partial_cmd_smooth_batch(
+        ctx.handle, T, offsets.ctypes.data, m_all.ctypes.data, nf, mag_ptrs, int(y_index), int(color_indices[0]),
+        int(color_indices[1]), err, comp, bias, C.byref(imf_fn), dm.ctypes.data, nv.ctypes.data, mm.ctypes.data,
+        float(correlation), ex.as_c(), ey.as_c(), hptr, dptr, nb, C.byref(npts), C.byref(pairs)))
+    del keep
+    if return_info:
+        return out, (npts.value, pairs.value)
+    return out

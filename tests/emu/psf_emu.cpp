@@ -23,41 +23,26 @@ extern "C" int64_t emu_bin_cmd_smooth(const double* colors, const double* mags, 
         pairs += asym ? prep_asymmetric(colors[p], mags[p], cerr[p], merr[p], weights[p], ex, ey, &desc[p])
                       : prep_covariant(colors[p], mags[p], cerr[p], merr[p], c, weights[p], ex, ey, &desc[p]);
     }
-    std::vector<double> acc((size_t)nx * ny, 0.0);
-    std::vector<float> part((size_t)nx * ny, 0.0f);
+    // the kernel's strips: a row's columns in runs of kCPT from column 0
+    const int ncg = (nx + kCPT - 1) / kCPT;
+    std::vector<double> acc((size_t)ncg * kCPT * ny, 0.0);
+    std::vector<float> part((size_t)ncg * kCPT * ny, 0.0f);
+    const size_t ld = (size_t)ncg * kCPT;
     for (int64_t pc = 0; pc < P; pc += 64) {
         const int64_t pe = pc + 64 < P ? pc + 64 : P;
         for (int64_t q = pc; q < pe; ++q) {
             const PointDesc& d = desc[q];
             if (d.xa > d.xb) continue;
-            const bool xser = d.ex < kSeriesEps;
-            const Series sx = series_coeffs(d.ex);
             for (int row = d.ya; row <= d.yb; ++row) {
-                if (asym) {
-                    const float v = fmaf((float)(row - d.ya), d.dv, d.ay);
-                    const float dy = d.ey < kSeriesEps ? d_series(v, series_coeffs(d.ey)) : d_direct(v, d.ey);
-                    const float fy = d.pref * dy;
-                    for (int col = d.xa; col <= d.xb; ++col) {
-                        const float u = fmaf((float)(col - d.xa), d.du, d.ax);
-                        const float dx = xser ? d_series(u, sx) : d_direct(u, d.ex);
-                        part[(size_t)row * nx + col] += fy * dx;
-                    }
-                } else {
-                    const float tj = fmaf((float)(row - d.ya), d.dv, d.ay);
-                    const float t0 = tj - d.ey, t2 = tj + d.ey;
-                    const float w0 = kW0 * fast_exp(-0.5f * t0 * t0);
-                    const float w1 = kW1 * fast_exp(-0.5f * tj * tj);
-                    const float w2 = kW0 * fast_exp(-0.5f * t2 * t2);
-                    const float h0 = d.g * t0, h1 = d.g * tj, h2 = d.g * t2;
-                    for (int col = d.xa; col <= d.xb; ++col) {
-                        const float u = fmaf((float)(col - d.xa), d.du, d.ax);
-                        float v;
-                        if (xser)
-                            v = w0 * d_series(u + h0, sx) + w1 * d_series(u + h1, sx) + w2 * d_series(u + h2, sx);
-                        else
-                            v = w0 * d_direct(u + h0, d.ex) + w1 * d_direct(u + h1, d.ex) + w2 * d_direct(u + h2, d.ex);
-                        part[(size_t)row * nx + col] += d.pref * v;
-                    }
+                float W[3], h[3] = {0.0f, 0.0f, 0.0f};
+                if (asym) W[0] = d.pref * y_factor(fmaf((float)(row - d.ya), d.dv, d.ay), d.ey);
+                else cov_row(d, row, W, h);
+                for (int cg = 0; cg < ncg; ++cg) {
+                    const int col0 = cg * kCPT;
+                    if (col0 > d.xb || col0 + kCPT - 1 < d.xa) continue;
+                    float* pp = &part[(size_t)row * ld + col0];
+                    if (asym) add_point<1, true>(d, W, h, col0, pp);
+                    else add_point<3, true>(d, W, h, col0, pp);
                 }
             }
         }
@@ -66,6 +51,7 @@ extern "C" int64_t emu_bin_cmd_smooth(const double* colors, const double* mags, 
             part[k] = 0.0f;
         }
     }
-    std::memcpy(out, acc.data(), acc.size() * sizeof(double));
+    for (int row = 0; row < ny; ++row)
+        for (int col = 0; col < nx; ++col) out[(size_t)row * nx + col] = acc[(size_t)row * ld + col];
     return pairs;
 }

@@ -65,6 +65,33 @@ def test_batch_200x300_shape(sfh, O, gpu_ctx):
     assert pairs == counts
 
 
+def test_batch_config1_exact_combination(sfh, O, gpu_ctx):
+    """The 700-template workload's exact combination: 200 x 300 grid,
+    2000-point isochrones, IMF 0.08-120 Msun, colour-magnitude correlation 0.6
+    (per-point covariance multipliers), on 3 templates (youngest, middle,
+    oldest) in one launch; max-normalised error <= 1e-5 and per-bin relative
+    error <= 2e-5 above 1e-6 of the maximum, pair counts equal."""
+    from sfh_amd import synth
+    edges = synth.edges_200x300()
+    specs = synth.specs_70x10()
+    specs = [specs[0], specs[345], specs[-1]]
+    pc = [synth.template_points(s, 2000, edges, 120.0, correlation=0.6) for s in specs]
+    batch = sfh.PointBatch([p for p, _ in pc], edges, cmult_list=[c for _, c in pc])
+    out, pairs = sfh.deposit_batch(batch, return_pairs=True)
+    out = out.cpu().numpy()
+    ex, ey = oracle_range(O, edges[0]), oracle_range(O, edges[1])
+    total = 0
+    for t, (sp, cm) in enumerate(pc):
+        ref, pr = O.bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, 1, sp.weights, ex, ey,
+                                   cmult=cm, nthreads=8)
+        total += pr
+        got = out[t].reshape(ref.shape, order="F")
+        assert max_norm_err(got, ref) <= TOL, t
+        big = ref > 1e-6 * ref.max()
+        assert np.max(np.abs(got[big] - ref[big]) / ref[big]) <= 2e-5, t
+    assert pairs == total
+
+
 def test_batch_real_valued_covariance_and_empty(sfh, O, gpu_ctx):
     from sfh_amd import synth
     edges = synth.edges_75x100()

@@ -78,3 +78,45 @@ def test_calculate_coeffs_invariants(H, kind):
     y = H.calculate_coeffs(model, disp, mstars[perm], np.repeat(ula[perm], umh.size), np.tile(umh, ula.size))
     yy = y.reshape(ula.size, umh.size)[np.argsort(-ula[perm], kind="stable")].ravel()
     assert np.allclose(x, yy, rtol=1e-12)
+
+
+def test_calculate_coeffs_hand_computed(H):
+    """Two ages x three metallicities, worked by hand from mzr.jl:58-84 /
+    amr.jl:56-77 and GaussianDispersion (dispersion_models.jl:91): the MZR's
+    cumulative mass runs from the OLDEST age bin, μ = MH0 + α (log10 M* - 6),
+    the AMR's μ = β + α (T_max - 10^(logAge-9)), and each age's weights
+    exp(-((x - μ)/σ)^2 / 2) are normalised to its stellar mass."""
+    logAge = [9.0, 9.0, 9.0, 10.0, 10.0, 10.0]
+    MH = [-1.0, -0.5, 0.0, -1.0, -0.5, 0.0]
+    mstars = [2e6, 1e6]                       # per unique age, in first-appearance order
+    sigma = 0.2
+
+    def weights(mu, m):
+        w = [math.exp(-((x - mu) / sigma) ** 2 / 2) for x in (-1.0, -0.5, 0.0)]
+        return [m * wi / sum(w) for wi in w]
+
+    # MZR: the 10.0 bin is oldest, so its cumulative mass is its own 1e6;
+    # the 9.0 bin's is 1e6 + 2e6
+    mu_old = -1.0 + 1.0 * (math.log10(1e6) - 6.0)        # = -1
+    mu_young = -1.0 + 1.0 * (math.log10(3e6) - 6.0)      # = -0.5229
+    expect = weights(mu_young, 2e6) + weights(mu_old, 1e6)
+    got = H.calculate_coeffs(H.PowerLawMZR(1.0, -1.0, 6.0), H.GaussianDispersion(sigma), mstars, logAge, MH)
+    assert np.allclose(got, expect, rtol=1e-14, atol=0)
+    # AMR with α = 0.1, β = -2, T_max = 13.7: μ(9) = -2 + 0.1 (13.7 - 1), μ(10) = -2 + 0.1 (13.7 - 10)
+    expect = weights(-2.0 + 0.1 * (13.7 - 1.0), 2e6) + weights(-2.0 + 0.1 * (13.7 - 10.0), 1e6)
+    got = H.calculate_coeffs(H.LinearAMR(0.1, -2.0), H.GaussianDispersion(sigma), mstars, logAge, MH)
+    assert np.allclose(got, expect, rtol=1e-14, atol=0)
+    # the same ages listed oldest first: masses follow unique(logAge) order
+    got2 = H.calculate_coeffs(H.PowerLawMZR(1.0, -1.0, 6.0), H.GaussianDispersion(sigma), [1e6, 2e6],
+                              logAge[3:] + logAge[:3], MH[3:] + MH[:3])
+    assert np.allclose(got2, weights(mu_old, 1e6) + weights(mu_young, 2e6), rtol=1e-14, atol=0)
+
+
+def test_stack_models(sfh):
+    """stack_models (src/fitting/utilities.jl:12-13): reduce(hcat, vec.(models)),
+    each template column-major."""
+    a = np.arange(6.0).reshape(2, 3)
+    b = 10 + np.arange(6.0).reshape(2, 3)
+    s = sfh.stack_models([a, b])
+    assert s.shape == (6, 2) and s.flags.f_contiguous
+    assert np.array_equal(s[:, 0], [0, 3, 1, 4, 2, 5]) and np.array_equal(s[:, 1], [10, 13, 11, 14, 12, 15])

@@ -58,6 +58,29 @@ def test_emulation_real_valued_covariance(sfh, emu, O):
     assert max_norm_err(mat, ref) < 1e-5
 
 
+@pytest.mark.parametrize("la,mh,cov_mult", [(6.6, -2.5, 1), (10.05, 0.0, 1), (8.0, -1.0, 0)])
+def test_emulation_config1_templates(sfh, emu, O, la, mh, cov_mult):
+    """The kernel arithmetic at the 700-template workload's shape: a 200 x 300
+    grid, 2000-point isochrones, IMF 0.08-120 Msun, colour-magnitude
+    correlation 0.6 (covariant) or the pixel-space asymmetric kernel; every
+    x form (degree-1/2/3 series, erfc differences) occurs.  Max-normalised
+    error <= 1e-5 and per-bin relative error <= 2e-5 on bins above 1e-6 of the
+    maximum."""
+    from sfh_amd import synth
+    edges = synth.edges_200x300()
+    sp, cm = synth.template_points(synth.TemplateSpec(la, mh, 7), 2000, edges, 120.0, correlation=0.6)
+    if cov_mult == 0:
+        cm = None
+    ex, ey = oracle_range(O, edges[0]), oracle_range(O, edges[1])
+    ref, pr = O.bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, cov_mult, sp.weights, ex, ey,
+                               cmult=cm, nthreads=8)
+    mat, pe = _emu(emu, O, sp.colors, sp.mags, sp.color_err, sp.mag_err, sp.weights, cov_mult, ex, ey, cmult=cm)
+    assert pe == pr
+    assert max_norm_err(mat, ref) <= 1e-5
+    big = ref > 1e-6 * ref.max()
+    assert np.max(np.abs(mat[big] - ref[big]) / ref[big]) <= 2e-5
+
+
 def test_host_prep_matches_oracle_prep(sfh, O):
     """The product's vectorised partial_cmd_smooth preparation equals the
     oracle's loop restatement."""
