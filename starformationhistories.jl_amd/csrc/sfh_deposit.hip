@@ -87,33 +87,49 @@ __global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) 
         acc[c * kDepBlock] = 0.0;
         part[c] = 0.0f;
     }
-    // descriptors are uniform: scalar loads, the next one in flight while a
-    // point is evaluated (32-bit indices: scalar compares)
+    // Each chunk of 64 points: every lane reads one point's footprint (the
+    // descriptor's first 16 bytes), a ballot keeps the points whose rows meet
+    // this wave's strips, and only those descriptors are loaded -- two at a
+    // time through scalar loads (uniform across the wave, one wait per pair).
     const PointDesc* __restrict__ td = desc + p0;
     const int np = (int)(p1 - p0);
-    PointDesc dn{};
-    if (np > 0) dn = td[0];
+    const int lane = threadIdx.x & 63;
+    auto footprint = [&](int pc) {
+        const int p = pc + lane;
+        return p < np ? *reinterpret_cast<const int4*>(td + p) : make_int4(1, 0, 1, 0);
+    };
+    auto add = [&](const PointDesc& d) {
+        if (!valid || row < d.ya || row > d.yb || col0 > d.xb || col0 + kCPT - 1 < d.xa) return;
+        // full width: every column of the grid is in the footprint, and the
+        // strips' columns past the grid are never written, so no mask
+        const bool full = d.xa == 0 && d.xb == nx - 1;
+        if constexpr (kAsym) {
+            float W[1], h[1] = {0.0f};
+            W[0] = d.pref * psf::y_factor(fmaf((float)(row - d.ya), d.dv, d.ay), d.ey);
+            if (full) psf::add_point<1, false>(d, W, h, col0, part);
+            else psf::add_point<1, true>(d, W, h, col0, part);
+        } else {
+            float W[3], h[3];
+            psf::cov_row(d, row, W, h);
+            if (full) psf::add_point<3, false>(d, W, h, col0, part);
+            else psf::add_point<3, true>(d, W, h, col0, part);
+        }
+    };
+    int4 fn = footprint(0);
     for (int pc = 0; pc < np; pc += kChunk) {
-        const int pe = (np - pc) < kChunk ? np : pc + kChunk;
-        for (int p = pc; p < pe; ++p) {
-            const PointDesc d = dn;
-            dn = td[p + 1 < np ? p + 1 : p];
-            if (d.ya > row_hi || d.yb < row_lo || d.xa > d.xb) continue;
-            if (!valid || row < d.ya || row > d.yb || col0 > d.xb || col0 + kCPT - 1 < d.xa) continue;
-            // full width: every column of the grid is in the footprint, and the
-            // strips' columns past the grid are never written, so no mask
-            const bool full = d.xa == 0 && d.xb == nx - 1;
-            if constexpr (kAsym) {
-                float W[1], h[1] = {0.0f};
-                W[0] = d.pref * psf::y_factor(fmaf((float)(row - d.ya), d.dv, d.ay), d.ey);
-                if (full) psf::add_point<1, false>(d, W, h, col0, part);
-                else psf::add_point<1, true>(d, W, h, col0, part);
-            } else {
-                float W[3], h[3];
-                psf::cov_row(d, row, W, h);
-                if (full) psf::add_point<3, false>(d, W, h, col0, part);
-                else psf::add_point<3, true>(d, W, h, col0, part);
-            }
+        const int4 f = fn;
+        fn = footprint(pc + kChunk);                           // next chunk's footprints in flight
+        uint64_t hits = __ballot(f.x <= f.y && f.z <= row_hi && f.w >= row_lo);
+        while (hits) {
+            const int i = __builtin_ctzll(hits);
+            hits &= hits - 1;
+            const bool two = hits != 0;
+            const int j = two ? __builtin_ctzll(hits) : i;
+            if (two) hits &= hits - 1;
+            const PointDesc d0 = td[pc + i];
+            const PointDesc d1 = td[pc + j];
+            add(d0);
+            if (two) add(d1);
         }
 #pragma unroll
         for (int c = 0; c < kCPT; ++c) {

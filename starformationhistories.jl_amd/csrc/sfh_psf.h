@@ -34,6 +34,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 namespace sfh {
 namespace psf {
@@ -323,6 +324,32 @@ __host__ __device__ inline void cov_row(const PointDesc& d, int row, float* W, f
     h[2] = d.g * t2;
 }
 
+// c du + a for a compile-time column offset c.  On the device the offset is a
+// literal of v_fmamk_f32 (D = S0 K + S1, fused): written as a plain fmaf the
+// compiler keeps every column's offset in a register of its own (VOP3 takes no
+// literal), which costs the kernel sixteen registers.
+template <int C>
+__host__ __device__ __forceinline__ float col_fma(float du, float a)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    float r;
+    asm("v_fmamk_f32 %0, %1, %2, %3" : "=v"(r) : "v"(du), "i"(__builtin_bit_cast(int, (float)C)), "v"(a));
+    return r;
+#else
+    return fmaf((float)C, du, a);
+#endif
+}
+
+// f(integral_constant<int, C>) for C = I .. N-1, in order
+template <int I, int N, typename F>
+__host__ __device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
 // Adds the contributions of one point to the kCPT columns col0 .. col0+kCPT-1
 // of one row: part[c] += sum_k W[k] X(a_k(c)), a_k(c) the x coordinate of
 // column col0 + c shifted by h[k] (NODES = 3: covariant GL nodes; NODES = 1:
@@ -336,25 +363,26 @@ __host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const flo
 {
     const float base = fmaf((float)(col0 - d.xa), d.du, d.ax);
     const int lo = d.xa - col0, hi = d.xb - col0;
+    const float du = d.du;
     if constexpr (DEG == kDirect) {
         // erf at the column edges b(c) = a(c) - ex, c = 0..kCPT, as erfc(|b|) so
         // that D(c) = erf(b(c+1)) - erf(b(c)) keeps its relative accuracy in
         // the tails: both edges >= 0: erfc(b0) - erfc(b1); both < 0:
         // erfc(-b1) - erfc(-b0); across 0: 2 - erfc(b1) - erfc(-b0)
-        float prev[NODES];
+        float prev[NODES], b0[NODES];
         bool pneg[NODES];
 #pragma unroll
         for (int k = 0; k < NODES; ++k) {
-            const float b0 = base + h[k] - d.ex;
-            pneg[k] = b0 < 0.0f;
-            prev[k] = erfc_pos(fabsf(b0));
+            b0[k] = base + h[k] - d.ex;
+            pneg[k] = b0[k] < 0.0f;
+            prev[k] = erfc_pos(fabsf(b0[k]));
         }
-#pragma unroll
-        for (int c = 0; c < kCPT; ++c) {
+        static_for<0, kCPT>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
             float v = part[c];
 #pragma unroll
             for (int k = 0; k < NODES; ++k) {
-                const float b1 = fmaf((float)(c + 1), d.du, base + h[k] - d.ex);
+                const float b1 = col_fma<c + 1>(du, b0[k]);
                 const bool neg = b1 < 0.0f;
                 const float e1 = erfc_pos(fabsf(b1));
                 const float dd = neg ? e1 - prev[k] : (pneg[k] ? 2.0f - e1 - prev[k] : prev[k] - e1);
@@ -364,7 +392,12 @@ __host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const flo
             }
             if (MASK) part[c] = (c >= lo && c <= hi) ? v : part[c];
             else part[c] = v;
-        }
+#if defined(__HIP_DEVICE_COMPILE__)
+            // one column at a time: interleaving the columns' erfc sequences
+            // would raise the register count of the whole kernel
+            __builtin_amdgcn_sched_barrier(0);
+#endif
+        });
     } else {
         float a0[NODES], c0[NODES], c1[NODES], c2[NODES], c3[NODES];
 #pragma unroll
@@ -375,12 +408,12 @@ __host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const flo
             c2[k] = DEG >= 2 ? W[k] * d.s2 : 0.0f;
             c3[k] = DEG >= 3 ? W[k] * d.s3 : 0.0f;
         }
-#pragma unroll
-        for (int c = 0; c < kCPT; ++c) {
+        static_for<0, kCPT>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
             float v = part[c];
 #pragma unroll
             for (int k = 0; k < NODES; ++k) {
-                const float a = fmaf((float)c, d.du, a0[k]);
+                const float a = col_fma<c>(du, a0[k]);
                 const float w = a * a;
                 float S;
                 if constexpr (DEG == 1) S = fmaf(c1[k], w, c0[k]);
@@ -390,7 +423,12 @@ __host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const flo
             }
             if (MASK) part[c] = (c >= lo && c <= hi) ? v : part[c];
             else part[c] = v;
-        }
+#if defined(__HIP_DEVICE_COMPILE__)
+            // columns in groups of four: enough independent work per wave, without
+            // every column's temporaries live at once
+            if constexpr ((c & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+#endif
+        });
     }
 }
 

@@ -694,8 +694,10 @@ def _fn_of(f, lo, hi, keep):
         return f.as_fn(lo, hi)
     if not hi > lo:
         hi = lo + 1.0
-    pad = 1e-9 * max(1.0, abs(lo), abs(hi))
-    t = Tabulated(f, lo - pad, hi + pad)
+    # exactly over [lo, hi]: arguments are clamped to the table, and a padded
+    # range would sample functions with hard limits (an IMF's mass range) outside
+    # their support
+    t = Tabulated(f, lo, hi)
     keep.append(t)
     return t.as_fn()
 

@@ -58,10 +58,10 @@ def test_fg_clamp_types(sfh, O, gpu_ctx, N, T):
     assert np.max(np.abs(G - rG64)) <= 1e-10 * np.max(np.abs(rG64))
     # ∇loglikelihood! alone clamps with eps(eltype(composite)) (:277)
     c32 = np.asarray(models.astype(np.float64) @ x, dtype=np.float32)
+    cc = c32.astype(np.float64)                 # before the call clobbers c32
     G3 = np.empty(T)
     sfh.grad_loglikelihood_(G3, c32, tpl, data)
     Gref = np.empty(T)
-    cc = c32.astype(np.float64)
     O.grad_inplace(Gref, cc, models, data, eps=E32)
     assert np.max(np.abs(G3 - Gref)) <= 1e-10 * np.max(np.abs(Gref))
 
