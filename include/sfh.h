@@ -158,6 +158,22 @@ int sfh_fg_device(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
                   const double* d_coeffs, int want_F, double eps, double eps_grad,
                   double* d_negLL, double* d_G, double* d_composite);
 
+/* No counterpart in the reference: sfh_fg_device for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same kernels and results, bit for
+ * bit, with the partial sums in a static device buffer of
+ * SFH_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead of
+ * context scratch: no call allocates or frees anything.  Limits: 1..704
+ * templates (the single-pass kernel; SFH_ERR_ARGUMENT otherwise) and
+ * partials (one LL value + ntemplates gradient values per work item, two
+ * items per compute unit) within the buffer (SFH_ERR_ARGUMENT otherwise).
+ * Every call on one GPU shares that GPU's buffer: calls that overlap in time
+ * on one GPU (several contexts or streams) race. */
+#define SFH_NOALLOC_SCRATCH_BYTES (4 * 1024 * 1024)
+int sfh_fg_device_noalloc(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data,
+                          const double* d_coeffs, int want_F, double eps, double eps_grad,
+                          double* d_negLL, double* d_G, double* d_composite);
+
 /* --------------------------------------- P2: many coefficient vectors (new) */
 
 /* B independent fg! evaluations, one per column of X (ntemplates x B), the

@@ -13,7 +13,7 @@ _lib.load()
 
 from ._lib import ArgumentError, DeviceError, DomainError, InexactError  # noqa: E402
 from .core import (Context, HIPData, HIPTemplates, composite_, default_context, driver_clamps, fg_,  # noqa: E402
-                   fg_batched, fg_batched_device, fg_device, grad_loglikelihood,
+                   fg_batched, fg_batched_device, fg_device, fg_device_noalloc, grad_loglikelihood,
                    grad_loglikelihood_, loglikelihood, stack_models)
 from .fitting import (HMCModel, LogTransformFTResult, MCMCModel, construct_x0,  # noqa: E402
                       fit_templates, fit_templates_fast, fit_templates_lbfgsb, mcmc_sample,
@@ -38,6 +38,7 @@ from .templates import (Histogram, NoBinaries, PointBatch, RandomBinaryPairs, Ra
 __all__ = [
     "ArgumentError", "DeviceError", "InexactError", "Context", "HIPData", "HIPTemplates",
     "composite_", "default_context", "driver_clamps", "fg_", "fg_batched", "fg_batched_device", "fg_device",
+    "fg_device_noalloc",
     "grad_loglikelihood", "grad_loglikelihood_", "loglikelihood", "stack_models", "Histogram",
     "PointBatch", "Range", "bin_cmd_smooth", "calculate_edges", "calculate_weights",
     "deposit_batch", "edges_range", "exp_photerr", "histogram_data", "histogram_pix",

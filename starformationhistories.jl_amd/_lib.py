@@ -125,6 +125,7 @@ SIGNATURES = [
     ("sfh_grad_loglikelihood_inplace", _int, [_vp, _vp, _vp, _vp, _dbl, _vp]),
     ("sfh_fg", _int, [_vp, _vp, _vp, _vp, _int, _dbl, _dbl, _pd, _vp, _vp]),
     ("sfh_fg_device", _int, [_vp, _vp, _vp, _vp, _int, _dbl, _dbl, _vp, _vp, _vp]),
+    ("sfh_fg_device_noalloc", _int, [_vp, _vp, _vp, _vp, _int, _dbl, _dbl, _vp, _vp, _vp]),
     ("sfh_fg_batched", _int, [_vp, _vp, _vp, _vp, _i64, _int, _dbl, _dbl, _vp, _vp]),
     ("sfh_fg_batched_device", _int, [_vp, _vp, _vp, _vp, _i64, _int, _dbl, _dbl, _vp, _vp]),
     ("sfh_multifit_create", _int, [_vp, _i64, _vp, _vp, C.POINTER(_vp)]),

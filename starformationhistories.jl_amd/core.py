@@ -394,6 +394,17 @@ def fg_device(models: HIPTemplates, data: HIPData, d_coeffs, d_negLL=None, d_G=N
                                                   _ptr(d_G), _ptr(d_composite)))
 
 
+def fg_device_noalloc(models: HIPTemplates, data: HIPData, d_coeffs, d_negLL=None, d_G=None, d_composite=None,
+                      want_F: bool = True, eps: float = _f64eps, eps_grad: float | None = None):
+    """fg_device through sfh_fg_device_noalloc: the same kernels and results
+    with the partial sums in the library's static device buffer, so the call
+    allocates nothing (graph capture, serving loops).  At most 704 templates;
+    calls that overlap in time on one GPU race."""
+    eg = eps if eps_grad is None else eps_grad
+    models.ctx.check(models.ctx.lib.sfh_fg_device_noalloc(models.ctx.handle, models.handle, data.handle,
+                                                          _ptr(d_coeffs), 1 if want_F else 0, eps, eg,
+                                                          _ptr(d_negLL), _ptr(d_G), _ptr(d_composite)))
+
 
 def fg_batched(X, models: HIPTemplates, data, precision: str = "f64", want_G: bool = True,
                eps: float = _f64eps, eps_grad: float | None = None):

@@ -471,6 +471,12 @@ int sfh_ctx_create(int device, void* stream, sfh_ctx** out)
         }
         c->own_stream = true;
     }
+    if (int rc = noalloc_init(device)) {
+        if (c->own_stream) hipStreamDestroy(c->stream);
+        delete c;
+        g_create_error = "noalloc_init: hipGetSymbolAddress of the static fg! buffer failed";
+        return rc;
+    }
     *out = c;
     return SFH_OK;
 }

@@ -131,6 +131,10 @@ int single_pass_fg(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_data,
                    double eps_g, int want_ll, double* d_negLL, double* d_ll, double* d_G, double gsign, double* d_comp_out,
                    int clobber_with_R);
 
+// Resolves, once per device, the address of the static buffer that
+// sfh_fg_device_noalloc uses (that device current).  Called by sfh_ctx_create.
+int noalloc_init(int device);
+
 // Deposition gather over descriptors already built on the device (template t
 // owns desc[koff[t] .. koff[t+1]), koff on the device); FP32 out, column t at
 // out32 + t*ld_out.

@@ -1008,6 +1008,35 @@ int single_pass_fg(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_data,
                               d_comp_out, clobber_with_R, (double*)llp, (double*)gp, true);
 }
 
+// ------------------------------------------ fg! that allocates nothing
+
+// The partial sums of sfh_fg_device_noalloc live in this fixed device array
+// (SFH_NOALLOC_SCRATCH_BYTES; every GPU the library runs on holds its own copy
+// until the process ends).  Host code never uses the array's name as an
+// address: noalloc_init resolves it per device with hipGetSymbolAddress and
+// the entry point passes that pointer to the kernels.  Nothing in it needs
+// clearing: every partial the reduction reads is written earlier in the same
+// call.  Calls that overlap in time on one device share it and race.
+constexpr int64_t kNoAllocDoubles = SFH_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double noalloc_scratch[kNoAllocDoubles];
+static double* g_noalloc_ptr[64] = {};
+static std::mutex g_noalloc_mu;
+
+int noalloc_init(int device)
+{
+    if (device < 0 || device >= 64) return SFH_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return SFH_OK;
+    // the caller (sfh_ctx_create) has made `device` current
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)) != hipSuccess || !p) return SFH_ERR_DEVICE;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return SFH_ERR_DEVICE;
+    __atomic_store_n(&g_cus_of[device], cus, __ATOMIC_RELAXED);
+    g_noalloc_ptr[device] = (double*)p;
+    return SFH_OK;
+}
+
 }  // namespace sfh
 
 using namespace sfh;
@@ -1537,6 +1566,35 @@ int sfh_multifit_fit(sfh_multifit* mf, const double* x0, const sfh_fit_options* 
         }
     }
     return SFH_OK;
+}
+
+int sfh_fg_device_noalloc(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data, const double* d_coeffs,
+                          int want_F, double eps, double eps_grad, double* d_negLL, double* d_G, double* d_composite)
+{
+    if (!ctx) return SFH_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (int rc = bind(ctx)) return rc;
+    SFH_ARG_CHECK(ctx, tpl && data && d_coeffs, "null pointer");
+    SFH_ARG_CHECK(ctx, tpl->nbins == data->nbins, "axes(models,1) == axes(data,1) (%lld vs %lld)",
+                  (long long)tpl->nbins, (long long)data->nbins);
+    SFH_ARG_CHECK(ctx, !want_F || d_negLL, "d_negLL must be given when want_F");
+    SFH_ARG_CHECK(ctx, single_pass_covers(tpl),
+                  "sfh_fg_device_noalloc: 1 to %d templates and (ntemplates-1)*ld + 32 < 2^31 (got %lld x %lld)",
+                  8 * mf::kSingleNW * mf::kSingleMaxK, (long long)tpl->nbins, (long long)tpl->ntemplates);
+    if (ctx->device < 0 || ctx->device >= 64 || !g_noalloc_ptr[ctx->device])
+        return set_error(ctx, SFH_ERR_DEVICE, "sfh_fg_device_noalloc: static buffer not initialised on device %d",
+                         ctx->device);
+    const int cus = __atomic_load_n(&g_cus_of[ctx->device], __ATOMIC_RELAXED);
+    int64_t cap = 0;
+    const int64_t need = single_pass_partials(tpl, cus, d_G != nullptr, &cap);
+    SFH_ARG_CHECK(ctx, need <= kNoAllocDoubles,
+                  "sfh_fg_device_noalloc: %lld partial sums exceed the static buffer (%lld doubles)",
+                  (long long)need, (long long)kNoAllocDoubles);
+    if (!want_F && !d_G && !d_composite) return SFH_OK;
+    double* base = g_noalloc_ptr[ctx->device];
+    return single_pass_launch(ctx, tpl, cus, data->d_data, d_coeffs, eps, eps_grad, want_F ? 1 : 0, d_negLL,
+                              nullptr, d_G, 1.0, d_composite, d_G ? 1 : 0, base, d_G ? base + cap : nullptr,
+                              false);
 }
 
 
