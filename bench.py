@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--fits", type=int, default=256, help="independent fits in the uneven multi-fit run")
     ap.add_argument("--no-multifit", action="store_true")
+    ap.add_argument("--multifit-cpu-seconds", type=float, default=60.0,
+                    help="host time budget of the multi-fit convergence comparison")
     ap.add_argument("--surrogate-steps", type=int, default=40000,
                     help="training steps (batch 32) of the template surrogate on 20,000 templates")
     ap.add_argument("--surrogate-stars", type=float, default=100.0,
@@ -50,9 +52,11 @@ def parse():
                     help="birth mass (Msun) of the mock population sampled, observed and binned on the GPU")
     ap.add_argument("--no-mock", action="store_true")
     ap.add_argument("--no-binaries", action="store_true")
-    ap.add_argument("--section", default="all", choices=["all", "templates", "plain"],
+    ap.add_argument("--no-config0", action="store_true")
+    ap.add_argument("--section", default="all", choices=["all", "templates", "plain", "config0"],
                     help="templates: only the deposition of the 700 templates; plain: deposition, then only the "
-                         "single-walker fg! loop (short runs for the profiler's counter passes)")
+                         "single-walker fg! loop (short runs for the profiler's counter passes); config0: only the "
+                         "50-template plain-method workload (templates + fit_templates)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates HBM counters)")
     return ap.parse_args()
@@ -94,15 +98,24 @@ class Timer:
         self.ms = self.s.elapsed_time(self.e)
 
 
-def uneven_fits(torch, nfits, seed=20261203):
-    """Seeded uneven fits: ntemplates ~ U[50, 700], nbins ~ U[5k, 60k] (a
-    colour x magnitude grid of aspect 2:3), distance modulus ~ U[24, 30], star
-    total log-uniform in [1e4, 1e6].  Template j of a fit is a main-sequence-like
-    ridge (position set by its age / metallicity index and shifted with the
-    distance modulus, width by a photometric-error scale) with a luminosity
-    function falling to faint magnitudes, unit sum, FP32, generated on the GPU.
-    Data: Poisson draws of the composite at log-normal coefficients."""
+def uneven_fits(sfh, torch, ctx, nfits, seed=20261203):
+    """Seeded uneven fits built like the 50-template workload: ntemplates ~
+    U[50, 700] isochrones picked without repetition from the 70 x 10 (logAge
+    6.6-10.05, [M/H] -2.5-0.0) grid of 500-point synthetic isochrones, Kroupa
+    IMF over 0.08-2 Msun, the exponential photometric error and logistic
+    completeness, deposited by partial_cmd_smooth_batch at the fit's distance
+    modulus ~ U[24, 30] on a colour -0.5..2.5 x magnitude dmod-6..dmod+6 grid
+    of nbins ~ U[5k, 60k] (aspect 2:3); star total log-uniform in [1e4, 1e6].
+    Data: Poisson draws of the composite at log-normal coefficients.  Returns
+    [(templates (T, N) FP32 on the device, data (N,) FP64 on the device, T, N,
+    dmod, total)]."""
+    from sfh_amd import synth
+    from sfh_amd.templates import edges_range
     rng = np.random.default_rng(seed)
+    specs = synth.specs_70x10()
+    imf = synth.Kroupa2001(0.08, 2.0)
+    isos = [synth.isochrone(s.log_age, s.mh, 500, s.seed, mmax_imf=2.0) for s in specs]
+    errs, comps = [synth.PHOTERR] * 2, [synth.COMPLETENESS] * 2
     fits = []
     for f in range(nfits):
         T = int(rng.integers(50, 701))
@@ -110,34 +123,27 @@ def uneven_fits(torch, nfits, seed=20261203):
         nx = max(8, int(round(np.sqrt(N * 2.0 / 3.0))))
         ny = max(8, N // nx)
         N = nx * ny
-        dmod = rng.uniform(24.0, 30.0)
-        total = 10 ** rng.uniform(4.0, 6.0)
-        g = torch.Generator(device="cuda").manual_seed(int(seed + f))
-        na = max(1, int(round(np.sqrt(T * 2))))
-        j = torch.arange(T, device="cuda", dtype=torch.float32)
-        age = (j % na) / max(na - 1, 1)
-        met = torch.floor(j / na) / max((T + na - 1) // na - 1, 1)
-        u = torch.linspace(0.0, 1.0, nx, device="cuda")[None, :, None]
-        v = torch.linspace(0.0, 1.0, ny, device="cuda")[None, None, :]
-        c0 = (0.15 + 0.5 * age + 0.08 * (dmod - 24.0) / 6.0)[:, None, None]
-        c1 = (0.5 + 0.3 * met)[:, None, None]
-        w = (0.01 + 0.03 * torch.rand(T, generator=g, device="cuda"))[:, None, None]
-        ridge = c0 + c1 * (u - 0.3) ** 2
-        C = torch.exp(-0.5 * ((v - ridge) / w) ** 2) * torch.exp(-2.5 * v) + 1e-7
-        C = (C / C.sum(dim=(1, 2), keepdim=True)).reshape(T, N).contiguous()
+        dmod = float(rng.uniform(24.0, 30.0))
+        total = float(10 ** rng.uniform(4.0, 6.0))
+        pick = np.sort(rng.choice(len(specs), T, replace=False))
+        edges = (edges_range(-0.5, 2.5, nx + 1), edges_range(dmod - 6.0, dmod + 6.0, ny + 1))
+        C = sfh.partial_cmd_smooth_batch([isos[k] for k in pick], errs, 1, (0, 1), imf, comps, dmod=dmod,
+                                         normalize_value=1.0, mean_mass=imf.mean(), edges=edges, ctx=ctx)
         x = torch.from_numpy(rng.lognormal(0.0, 1.0, T)).cuda()
         comp = x @ C.double()
         comp *= total / comp.sum()
+        g = torch.Generator(device="cuda").manual_seed(int(seed + f))
         data = torch.poisson(comp, generator=g)
-        fits.append((C, data, T, N))
+        fits.append((C, data, T, N, dmod, total))
     return fits
 
 
-def cpu_fit_templates(O, models, data, nthreads):
+def cpu_fit_templates(O, models, data, nthreads, maxiter=10000):
     """fit_templates (src/fitting/solvers.jl:172-221) on the host: SciPy BFGS
     (inf-norm gradient tolerance 1e-8, as Optim's g_abstol) around the C FP64
-    oracle's fg!, MAP then MLE in log-coefficients.  Returns MAP/MLE (mu, fmin)
-    and the fg! evaluation count."""
+    oracle's fg!, MAP then MLE in log-coefficients, at most ``maxiter``
+    iterations per stage.  Returns MAP/MLE (mu, fmin, status: "converged",
+    "max_iterations" or SciPy's message) and the fg! evaluation count."""
     from scipy.optimize import minimize
     T = models.shape[1]
     x0 = np.ones(T)
@@ -150,10 +156,15 @@ def cpu_fit_templates(O, models, data, nthreads):
         f, G, _ = O.fg(models, data, x, nthreads=nthreads)
         return (f, G * x) if mle else (f - th.sum(), G * x - 1.0)
 
-    opts = {"gtol": 1e-8, "maxiter": 10000, "norm": np.inf}
+    opts = {"gtol": 1e-8, "maxiter": maxiter, "norm": np.inf}
     r1 = minimize(obj, np.log(x0), jac=True, method="BFGS", args=(False,), options=opts)
     r2 = minimize(obj, r1.x, jac=True, method="BFGS", args=(True,), options=opts)
-    return {"map": (np.exp(r1.x), float(r1.fun)), "mle": (np.exp(r2.x), float(r2.fun)), "evals": nev[0]}
+
+    def status(r):
+        return {0: "converged", 1: "max_iterations"}.get(r.status, str(r.message))
+
+    return {"map": (np.exp(r1.x), float(r1.fun), status(r1)), "mle": (np.exp(r2.x), float(r2.fun), status(r2)),
+            "evals": nev[0]}
 
 
 def multifit_bench(sfh, torch, ctx, O, args, ncpu):
@@ -163,16 +174,11 @@ def multifit_bench(sfh, torch, ctx, O, args, ncpu):
     fits checked against the same fits done on the host."""
     from sfh_amd.multifit import MultiFit
     t0 = time.time()
-    fits = uneven_fits(torch, args.fits)
-    tpls = [sfh.HIPTemplates(device_ptr=C.data_ptr(), shape=(N, T), ctx=ctx) for C, _, T, N in fits]
-    datas = [sfh.HIPData(device_ptr=d.data_ptr(), nbins=N, ctx=ctx) for _, d, _, N in fits]
+    fits = uneven_fits(sfh, torch, ctx, args.fits)
+    tpls = [sfh.HIPTemplates(device_ptr=C.data_ptr(), shape=(N, T), ctx=ctx) for C, _, T, N, _, _ in fits]
+    datas = [sfh.HIPData(device_ptr=d.data_ptr(), nbins=N, ctx=ctx) for _, d, _, N, _, _ in fits]
     ctx.synchronize()
-    sizes = [(T, N) for _, _, T, N in fits]
-    # host copies of the smallest fits, for the host-side sample
-    order = [int(f) for f in np.argsort([T * N for T, N in sizes])[:16]]
-    host = {f: (fits[f][0].cpu().numpy().T, fits[f][1].cpu().numpy()) for f in order}
-    del fits
-    torch.cuda.empty_cache()
+    sizes = [(T, N) for _, _, T, N, _, _ in fits]
     mf = MultiFit(tpls, datas, ctx=ctx)
     gen_s = time.time() - t0
     elems = float(sum(T * N for T, N in sizes))
@@ -223,39 +229,133 @@ def multifit_bench(sfh, torch, ctx, O, args, ncpu):
            "graph_ms_total": gms, "fg_evaluations": evals, "status_counts": st,
            "iterations_median": float(np.median(iters)), "iterations_max": int(iters.max())}
     if not args.no_cpu:
-        # host fit_templates on the smallest fits, bounded to about cpu_seconds
+        # host fit_templates, 1000 iterations per stage as on the device, on
+        # the three smallest fits and the smallest fits that reached the
+        # iteration cap on the device, bounded to about multifit_cpu_seconds
+        by_size = [int(f) for f in np.argsort([T * N for T, N in sizes])]
+        capped = [f for f in by_size
+                  if "max_iterations" in (res[f]["map"].result.status, res[f]["mle"].result.status)]
+        order = by_size[:3] + [f for f in capped if f not in by_size[:3]][:3]
         sample, cpu_s, cpu_work = [], 0.0, 0.0
         for f in order:
-            if cpu_s > args.cpu_seconds:
+            if cpu_s > args.multifit_cpu_seconds:
                 break
-            models, data = host[f]
+            models = np.asfortranarray(fits[f][0].cpu().numpy().T)
+            data = fits[f][1].cpu().numpy()
             t1 = time.time()
-            ref = cpu_fit_templates(O, np.asfortranarray(models, dtype=np.float32), data, ncpu)
+            ref = cpu_fit_templates(O, models, data, ncpu, maxiter=1000)
             dt = time.time() - t1
             cpu_s += dt
             cpu_work += ref["evals"] * sizes[f][0] * sizes[f][1]
-            row = {"fit": int(f), "templates": sizes[f][0], "bins": sizes[f][1], "cpu_s": dt}
+            row = {"fit": int(f), "templates": sizes[f][0], "bins": sizes[f][1], "dmod": fits[f][4],
+                   "stars": fits[f][5], "cpu_s": dt, "cpu_evals": ref["evals"]}
             for key in ("map", "mle"):
-                mu_ref, f_ref = ref[key]
-                mu = res[f][key].mu
+                mu_ref, f_ref, st_ref = ref[key]
+                r = res[f][key]
                 big = mu_ref > 1e-3 * mu_ref.max()
-                row[f"{key}_mu_rel_err"] = float(np.max(np.abs(mu[big] - mu_ref[big]) / mu_ref[big]))
-                row[f"{key}_fmin_rel_diff"] = float((res[f][key].result.minimum - f_ref) / abs(f_ref))
-                row[f"{key}_status"] = res[f][key].result.status
+                row[f"{key}_mu_rel_err"] = float(np.max(np.abs(r.mu[big] - mu_ref[big]) / mu_ref[big]))
+                row[f"{key}_fmin_rel_diff"] = float((r.result.minimum - f_ref) / abs(f_ref))
+                row[f"{key}_status"] = r.result.status
+                row[f"{key}_cpu_status"] = st_ref
+                row[f"{key}_iterations"] = r.result.iterations
+                # coefficients the optimiser drives towards zero in log space
+                row[f"{key}_tiny_coeffs"] = int(np.sum(r.mu < 1e-6 * r.mu.max()))
             sample.append(row)
         # the host's time per (template element x fg! evaluation), projected onto the whole group
-        proj = cpu_s / cpu_work * work
+        proj = cpu_s / max(cpu_work, 1.0) * work
         out["cpu_sample"] = sample
         out["cpu_projected_seconds_all_fits"] = proj
         out["gpu_over_cpu_projected"] = proj / fit_s
-        log(f"multifit host sample: {len(sample)} fits in {cpu_s:.1f} s on {ncpu} threads; "
-            f"projected host time for all fits {proj:.0f} s ({proj / fit_s:.0f}x the GPU)")
-    del host
+        log(f"multifit host sample: {len(sample)} fits ({sum(1 for f in order[:len(sample)] if f in capped)} "
+            f"capped on the device) in {cpu_s:.1f} s on {ncpu} threads; projected host time for all fits "
+            f"{proj:.0f} s ({proj / fit_s:.0f}x the GPU)")
+        for row in sample:
+            log(f"  fit {row['fit']} T={row['templates']} N={row['bins']}: device {row['map_status']}/"
+                f"{row['mle_status']}, host {row['map_cpu_status']}/{row['mle_cpu_status']}, fmin rel diff "
+                f"{row['map_fmin_rel_diff']:.1e}/{row['mle_fmin_rel_diff']:.1e}")
+    del fits
     mf.close()
     for t in tpls:
         t.close()
     for d in datas:
         d.close()
+    return out
+
+
+def config0_bench(sfh, torch, ctx, O, args, ncpu):
+    """The plain method end to end on the 50-template workload (10 log ages x
+    5 [M/H], 500-point isochrones, 75 x 100 grid, IMF 0.08-2 Msun, ~1e5 stars):
+    templates by partial_cmd_smooth_batch, then fit_templates (MAP + MLE) by
+    the host BFGS around the GPU fg! and by the on-device BFGS, against the
+    host path (the oracle's partial_cmd_smooth and SciPy BFGS around its
+    fg!)."""
+    from sfh_amd import synth
+    from sfh_amd.multifit import MultiFit
+    edges = synth.edges_75x100()
+    specs = synth.specs_10x5()
+    imf = synth.Kroupa2001(0.08, 2.0)
+    isos = [synth.isochrone(s.log_age, s.mh, 500, s.seed, mmax_imf=2.0) for s in specs]
+    kw = dict(dmod=24.0, normalize_value=1.0, mean_mass=imf.mean(), edges=edges, ctx=ctx)
+    errs, comps = [synth.PHOTERR] * 2, [synth.COMPLETENESS] * 2
+    sfh.partial_cmd_smooth_batch(isos, errs, 1, (0, 1), imf, comps, output="host", **kw)
+    t0 = time.time()
+    models = sfh.partial_cmd_smooth_batch(isos, errs, 1, (0, 1), imf, comps, output="host", **kw)
+    tpl_s = time.time() - t0
+    models = np.asfortranarray(models)
+    rng = np.random.default_rng(20261203)
+    x_true = 10.0 ** (0.3 * rng.standard_normal(len(specs)))
+    x_true *= 1e5 / (models.astype(np.float64) @ x_true).sum()
+    data = synth.poisson_data(models.astype(np.float64) @ x_true, seed=5)
+    sfh.fit_templates(models, data)                       # warm-up (first-call set-up)
+    t0 = time.time()
+    host_loop = sfh.fit_templates(models, data)
+    host_loop_s = time.time() - t0
+    mf = MultiFit([models], [data], ctx=ctx)
+    mf.fit_templates()
+    t0 = time.time()
+    dev = mf.fit_templates()[0]
+    dev_s = time.time() - t0
+    mf.close()
+    out = {"templates": len(specs), "bins": models.shape[0], "stars": float(data.sum()),
+           "templates_s": tpl_s,
+           "fit_host_bfgs_gpu_fg_s": host_loop_s,
+           "fit_host_bfgs_evals": int(host_loop["map"].result.nfev + host_loop["mle"].result.nfev),
+           "fit_device_bfgs_s": dev_s,
+           "device_status": [dev["map"].result.status, dev["mle"].result.status],
+           "end_to_end_s": tpl_s + dev_s}
+    log(f"config0: 50 templates in {tpl_s * 1e3:.1f} ms; fit_templates host-BFGS/GPU-fg! {host_loop_s:.2f} s "
+        f"({out['fit_host_bfgs_evals']} fg!), device BFGS {dev_s:.3f} s ({out['device_status']})")
+    if not args.no_cpu:
+        f = lambda v: float(synth.photerr(v))           # noqa: E731
+        c = lambda v: float(synth.completeness(v))      # noqa: E731
+        ex, ey = (O.Range(e.first, e.step, e.length) for e in edges)
+        t0 = time.time()
+        ref = []
+        for m, mags in isos:
+            r, _ = O.partial_cmd_smooth(m, mags, [f, f], 1, (0, 1), lambda v: float(imf(v)), [c, c], dmod=24.0,
+                                        normalize_value=1.0, mean_mass=imf.mean(), ex=ex, ey=ey, nthreads=ncpu)
+            ref.append(r.ravel(order="F"))
+        ref = np.asfortranarray(np.stack(ref, axis=1), dtype=np.float32)
+        cpu_tpl_s = time.time() - t0
+        t0 = time.time()
+        cfit = cpu_fit_templates(O, ref, data, ncpu)
+        cpu_fit_s = time.time() - t0
+        same = cpu_fit_templates(O, models, data, ncpu)
+        out["template_max_norm_err_vs_oracle"] = float(max(
+            np.max(np.abs(models[:, t] - ref[:, t])) / np.max(np.abs(ref[:, t])) for t in range(len(specs))))
+        for key in ("map", "mle"):
+            out[f"{key}_fmin_rel_diff_same_templates"] = {
+                "host_bfgs_gpu_fg": float((host_loop[key].result.fun - same[key][1]) / abs(same[key][1])),
+                "device_bfgs": float((dev[key].result.minimum - same[key][1]) / abs(same[key][1]))}
+        out["mle_fmin_rel_diff_vs_oracle_templates"] = float(
+            (dev["mle"].result.minimum - cfit["mle"][1]) / abs(cfit["mle"][1]))
+        out["cpu_baseline"] = {"value": cpu_tpl_s + cpu_fit_s, "unit": "s end to end", "cores": ncpu,
+                               "kind": "port", "templates_s": cpu_tpl_s, "fit_s": cpu_fit_s,
+                               "sample": "the whole workload: the oracle's partial_cmd_smooth (Python "
+                                         "preparation + C FP64 deposition) for 50 templates, then SciPy BFGS "
+                                         "(MAP + MLE) around the C FP64 fg!"}
+        log(f"config0 host path: templates {cpu_tpl_s:.2f} s + fit {cpu_fit_s:.2f} s on {ncpu} threads; "
+            f"template err {out['template_max_norm_err_vs_oracle']:.2e}")
     return out
 
 
@@ -298,14 +398,18 @@ def surrogate_bench(sfh, torch, ctx, O, args, ncpu):
                 "exact_templates_per_s": 20000 / gen_s, "loss_history": hist,
                 "heldout_KL_per_star_q50_q90_q99_max": q,
                 "heldout_sum_ratio_q01_q50_q99": [float(v) for v in np.quantile(ss / se, [0.01, 0.5, 0.99])],
-                "heldout_fail_frac": {str(n): float((dl * n > 0.5).mean()) for n in (10, 100, 1000)}})
+                "heldout_fail_frac": {str(n): float((dl * n > 0.5).mean())
+                                      for n in (10, 100, 1000, 10000, 100000, 1000000)}})
     log(f"surrogate held-out (500 over the whole box): KL/star q50/q90/q99/max {q}; "
         f"dlnL > 0.5 at 100 stars: {out['heldout_fail_frac']['100']:.3f}")
     # the fill
     la = np.round(6.6 + 0.01 * np.arange(351), 2)
     mh = np.linspace(-2.5, 0.3, 34)
     net.predict(np.array([[8.0, -1.0, 27.0, 1.0]]))        # warm-up of the inference shapes
-    for nstars in (args.surrogate_stars, 10.0 * args.surrogate_stars):
+    # the fallback rate rises with the star count (dlnL = N KL): fills at
+    # 1e2 .. 1e5 stars
+    for nstars in (args.surrogate_stars, 10.0 * args.surrogate_stars, 100.0 * args.surrogate_stars,
+                   1000.0 * args.surrogate_stars):
         out[f"fill_{nstars:g}_stars"] = _fill_bench(S, torch, ctx, O, args, ncpu, net, la, mh, nstars,
                                                     20000 / gen_s)
     return out
@@ -599,12 +703,15 @@ def main():
     ctx = sfh.Context(0, stream=stream.cuda_stream)
     T_, B = 700, args.walkers
     res = {}
+    O = ge.load_oracle()
+    ncpu = os.cpu_count() or 1
+    if args.section == "config0":
+        print(json.dumps({"section": "config0", **config0_bench(sfh, torch, ctx, O, args, ncpu)}), flush=True)
+        return
 
     # ---------------------------------------------------- templates (deposition)
     edges = synth.edges_200x300()
     nbins = (edges[0].length - 1) * (edges[1].length - 1)
-    O = ge.load_oracle()
-    ncpu = os.cpu_count() or 1
     res["templates"], out = templates_bench(sfh, torch, ctx, O, args, ncpu)
     if args.section == "templates":
         print(json.dumps({"section": "templates", **res["templates"]}), flush=True)
@@ -747,6 +854,8 @@ def main():
                          f"fg! (OpenMP over bins and templates), {dt:.1f} s"}
         log(f"cpu oracle: {n / dt:.3g} evals/s on {ncpu} threads")
 
+    if not args.no_extras and not args.no_config0:
+        extras["config0"] = config0_bench(sfh, torch, ctx, O, args, ncpu)
     if not args.no_extras and not args.no_multifit:
         extras["multifit"] = multifit_bench(sfh, torch, ctx, O, args, ncpu)
     if not args.no_extras and not args.no_surrogate:

@@ -207,7 +207,8 @@ int sfh_multifit_destroy(sfh_multifit* mf);
 
 /* Work decomposition of the grouped fg! launch: about item_elems template
  * elements per work item (whole 32-bin blocks of one fit), workgroups of
- * 64*waves threads (waves in {4, 8, 16}) and wg_per_cu persistent workgroups
+ * 64*waves threads (waves in {4, 8, 16}; 16 only when no fit has more than 512
+ * templates) and wg_per_cu persistent workgroups
  * per compute unit. */
 int sfh_multifit_get_shape(const sfh_multifit* mf, int64_t* item_elems, int* waves, int* wg_per_cu);
 int sfh_multifit_set_shape(sfh_multifit* mf, int64_t item_elems, int waves, int wg_per_cu);

@@ -815,7 +815,9 @@ __global__ __launch_bounds__(kWG) void k_datasum(const FitDesc* __restrict__ fit
 int kmax_for(int64_t maxT, int nw)
 {
     const int need = (int)cdiv(maxT, 8 * nw);
-    const int cap = nw == 4 ? 32 : nw == 8 ? 16 : 8;
+    // 16-wave workgroups (128 registers per lane) stop at 4 columns per
+    // thread: 8 columns' two register sets and gradient sums spill
+    const int cap = nw == 4 ? 32 : nw == 8 ? 16 : 4;
     for (int v : {2, 4, 8, 11, 16, 32})
         if (need <= v && v <= cap) return v;
     return 0;
@@ -1137,7 +1139,7 @@ void launch_fg(sfh_multifit* mf, const double* d_X, const int32_t* active, doubl
         if (km <= 2) SFH_L(8, 2); else if (km <= 4) SFH_L(8, 4); else if (km <= 8) SFH_L(8, 8);
         else if (km <= 11) SFH_L(8, 11); else SFH_L(8, 16);
     } else {
-        if (km <= 2) SFH_L(16, 2); else if (km <= 4) SFH_L(16, 4); else SFH_L(16, 8);
+        if (km <= 2) SFH_L(16, 2); else SFH_L(16, 4);
     }
 #undef SFH_L
 }
@@ -1298,6 +1300,8 @@ int sfh_multifit_set_shape(sfh_multifit* mf, int64_t item_elems, int waves, int 
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     SFH_ARG_CHECK(ctx, item_elems >= kSB && (waves == 4 || waves == 8 || waves == 16) && wg_per_cu >= 1 && wg_per_cu <= 8,
                   "sfh_multifit_set_shape: item_elems >= 32, waves in {4, 8, 16}, 1 <= wg_per_cu <= 8");
+    SFH_ARG_CHECK(ctx, kmax_for(mf->maxT, waves) > 0,
+                  "sfh_multifit_set_shape: 16 waves per workgroup need at most 512 templates per fit");
     if (int rc = bind(ctx)) return rc;
     mf->item_elems = item_elems;
     mf->nw = waves;
@@ -1336,7 +1340,7 @@ int sfh_multifit_autotune(sfh_multifit* mf, int reps, int64_t* item_elems, int* 
         for (int wg : cand_wg) {
             if (rc) break;
             // no more than 32 waves per compute unit
-            if (nw * wg > 32) continue;
+            if (nw * wg > 32 || kmax_for(mf->maxT, nw) == 0) continue;
             mf->item_elems = ie;
             mf->nw = nw;
             mf->wg_per_cu = wg;

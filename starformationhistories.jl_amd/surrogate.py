@@ -31,6 +31,8 @@ serve only as device memory; all arithmetic runs in libsfh_hip.so.
 from __future__ import annotations
 
 import ctypes as C
+import math
+
 import numpy as np
 
 from . import _lib
@@ -481,12 +483,17 @@ def sample_parameters(n, seed=0, lo=PARAM_LO, hi=PARAM_HI):
 def exact_templates(params, edges=None, npoints=2000, correlation=0.6, mmax_imf=120.0, chunk=1024, out=None,
                     ctx: Context | None = None):
     """partial_cmd_smooth of each (logAge, [M/H], dmod, error scale) row on
-    the 200 x 300 grid: synthetic isochrone (seed ISOCHRONE_SEED), host point
-    preparation and the GPU deposition kernel.  Returns an n x nbins FP32
-    device tensor (bin = ix + nx iy, per unit stellar mass)."""
+    the 200 x 300 grid: synthetic isochrone (seed ISOCHRONE_SEED), point
+    preparation and the GPU deposition kernel.  Rows that share their error
+    scale with at least ``min_group - 1`` others (a fill grid) are prepared on
+    the device (partial_cmd_smooth_batch: the scale s enters the photometric
+    error exactly, s (0.01 + e^(0.6 (m - 28))) = 0.01 s + e^(0.6 (m - 28 +
+    ln(s) / 0.6))); the others (a sampled design, every scale different) by
+    the host preparation.  Returns an n x nbins FP32 device tensor (bin = ix +
+    nx iy, per unit stellar mass)."""
     import torch
     from . import synth
-    from .templates import PointBatch, deposit_batch
+    from .templates import PointBatch, deposit_batch, partial_cmd_smooth_batch, ExpPhotErr
     ctx = ctx or default_context()
     edges = edges or synth.edges_200x300()
     p = np.atleast_2d(np.asarray(params, dtype=np.float64))
@@ -494,14 +501,39 @@ def exact_templates(params, edges=None, npoints=2000, correlation=0.6, mmax_imf=
     nbins = (edges[0].length - 1) * (edges[1].length - 1)
     if out is None:
         out = torch.empty((n, nbins), dtype=torch.float32, device="cuda")
-    for c0 in range(0, n, chunk):
-        c1 = min(n, c0 + chunk)
-        pts = [synth.template_points(synth.TemplateSpec(r[0], r[1], ISOCHRONE_SEED), npoints, edges, mmax_imf,
-                                     dmod=r[2], correlation=correlation, err_scale=r[3]) for r in p[c0:c1]]
+    min_group = 8
+    scales, inv, counts = np.unique(p[:, 3], return_inverse=True, return_counts=True)
+    imf = synth.Kroupa2001(0.08, mmax_imf)
+    for k in np.flatnonzero(counts >= min_group):
+        s = float(scales[k])
+        err = ExpPhotErr(math.e, 0.6, 28.0 - math.log(s) / 0.6, 0.01 * s)
+        rows = np.flatnonzero(inv == k)
+        for c0 in range(0, rows.size, chunk):
+            r = rows[c0:c0 + chunk]
+            isos = [synth.isochrone(q[0], q[1], npoints, ISOCHRONE_SEED, mmax_imf=mmax_imf) for q in p[r]]
+            d = partial_cmd_smooth_batch(isos, [err, err], 1, (0, 1), imf,
+                                         [synth.COMPLETENESS, synth.COMPLETENESS], dmod=[float(v) for v in p[r, 2]],
+                                         normalize_value=1.0, mean_mass=imf.mean(),
+                                         correlation=0.0 if correlation is None else correlation, edges=edges,
+                                         ctx=ctx)
+            out.index_copy_(0, torch.from_numpy(r.astype(np.int64)).to(out.device), d)
+            del d
+    host = np.flatnonzero(counts[inv] < min_group)
+    for c0 in range(0, host.size, chunk):
+        r = host[c0:c0 + chunk]
+        pts = [synth.template_points(synth.TemplateSpec(q[0], q[1], ISOCHRONE_SEED), npoints, edges, mmax_imf,
+                                     dmod=q[2], correlation=correlation, err_scale=q[3]) for q in p[r]]
         batch = PointBatch([q for q, _ in pts], edges,
                            cmult_list=[c for _, c in pts] if correlation is not None else None)
-        deposit_batch(batch, out=out[c0:c1], ctx=ctx)
+        if r.size and r[-1] - r[0] == r.size - 1:
+            deposit_batch(batch, out=out[int(r[0]):int(r[-1]) + 1], ctx=ctx)
+        else:
+            tmp = torch.empty((r.size, nbins), dtype=torch.float32, device=out.device)
+            deposit_batch(batch, out=tmp, ctx=ctx)
+            out.index_copy_(0, torch.from_numpy(r.astype(np.int64)).to(out.device), tmp)
+            del tmp
         ctx.synchronize()
+    ctx.synchronize()
     return out
 
 
@@ -546,6 +578,28 @@ def _lattice(n, stride):
     return np.asarray(idx)
 
 
+def _segments(idx):
+    """Consecutive pairs of checked indices along one axis; an axis with a
+    single checked index is one degenerate segment."""
+    if idx.size == 1:
+        return [(int(idx[0]), int(idx[0]))]
+    return [(int(idx[k]), int(idx[k + 1])) for k in range(idx.size - 1)]
+
+
+def fallback_mask(dgrid, ia, im, threshold):
+    """Grid points to compute exactly: every point of a lattice cell (a
+    rectangle between consecutive checked ages and metallicities, or a segment
+    where an axis has a single grid point) with a checked corner whose dlnL
+    exceeds ``threshold``."""
+    bad = np.zeros(dgrid.shape, dtype=bool)
+    for a0, a1 in _segments(np.asarray(ia)):
+        for b0, b1 in _segments(np.asarray(im)):
+            c = dgrid[np.ix_([a0, a1], [b0, b1])]
+            if np.any(c > threshold):
+                bad[a0:a1 + 1, b0:b1 + 1] = True
+    return bad
+
+
 def fill_templates(surrogate: TemplateSurrogate, log_ages, mhs, dmod, err_scale, n_stars, threshold=0.5,
                    check_stride=(5, 4), batch=256, exact_kws=None):
     """Templates of the grid log_ages x mhs at distance modulus ``dmod`` and
@@ -572,12 +626,7 @@ def fill_templates(surrogate: TemplateSurrogate, log_ages, mhs, dmod, err_scale,
     dl, _, _ = template_delta_lnL(E, T[cidx], n_stars, ctx=surrogate.ctx)
     dgrid = np.full((na, nm), np.nan)
     dgrid.ravel()[cidx] = dl
-    bad = np.zeros((na, nm), dtype=bool)
-    for a in range(ia.size - 1):
-        for b in range(im.size - 1):
-            c = dgrid[np.ix_(ia[a:a + 2], im[b:b + 2])]
-            if np.any(c > threshold):
-                bad[ia[a]:ia[a + 1] + 1, im[b]:im[b + 1] + 1] = True
+    bad = fallback_mask(dgrid, ia, im, threshold)
     T[cidx] = E
     del E
     fb = np.flatnonzero(bad.ravel() & ~checked.ravel())

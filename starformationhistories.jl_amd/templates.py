@@ -771,6 +771,9 @@ def partial_cmd_smooth_batch(isochrones, mag_err_funcs, y_index, color_indices, 
     npts, pairs = C.c_int64(), C.c_int64()
     if output == "device":
         out = torch.empty((T, nb), dtype=torch.float32, device="cuda")
+        # the allocator may hand back memory that work still queued on torch's
+        # stream reads; the library writes it on its own stream
+        torch.cuda.current_stream().synchronize()
         hptr, dptr = None, out.data_ptr()
     elif output == "host":
         out = np.empty((nb, T), dtype=np.float32, order="F")

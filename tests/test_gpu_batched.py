@@ -1,16 +1,20 @@
 """GPU parity of the batched fg! (many walkers per launch, MFMA) against the
-FP64 oracle, for each precision of the ladder.  Tolerances are the documented
-bounds of each form (FP64 final reduction in all of them):
-  f64    : LL rel-err <= 1e-9        gradient norm-rel-err <= 1e-4
-  bf16x2 : LL rel-err <= 5e-5        gradient norm-rel-err <= 1e-3
-  fp32   : LL rel-err <= 1e-5        gradient norm-rel-err <= 1e-4
+FP64 oracle, for each precision of the ladder (FP64 final reduction in all of
+them).  The log-likelihood bound of the f64 form is the 1e-9 target; the other
+bounds are 3-6x the largest error measured over these cases on an MI355X, so
+that a regression of the GEMM kernels shows:
+  form     LL rel-err   gradient norm-rel-err   (extreme-value case)   measured max
+  f64      1e-9         2e-6                     1e-5                 2.1e-14 / 3.7e-7 (2.1e-6)
+  bf16x2   5e-5         1e-4                     2e-5                 1.4e-5 / 3.3e-5 (4.5e-6)
+  fp32     1e-5         2e-5                     2e-6                 3.1e-6 / 4.3e-6 (3.5e-7)
 """
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-BOUNDS = {"f64": (1e-9, 1e-4), "bf16x2": (5e-5, 1e-3), "fp32": (1e-5, 1e-4)}
+BOUNDS = {"f64": (1e-9, 2e-6), "bf16x2": (5e-5, 1e-4), "fp32": (1e-5, 2e-5)}
+EXTREME_GRAD = {"f64": 1e-5, "bf16x2": 2e-5, "fp32": 2e-6}
 
 
 def _problem(N, T, B, seed, total=1e6, sigma_dex=0.3):
@@ -46,6 +50,26 @@ def test_batched_against_oracle(sfh, O, gpu_ctx, prec, N, T, B):
 
 
 @pytest.mark.parametrize("prec", ["f64", "bf16x2", "fp32"])
+def test_batched_full_config_4096_walkers(sfh, O, gpu_ctx, prec):
+    """The full workload in one launch: 4096 walkers x 700 templates x 60,000
+    bins; the oracle checks 17 walkers spread over the batch (the first, and
+    the last of every 256-walker tile)."""
+    B = 4096
+    models, data, X = _problem(60000, 700, B, seed=4096)
+    tpl = sfh.HIPTemplates(models)
+    hd = sfh.HIPData(data)
+    negll, G = sfh.fg_batched(X, tpl, hd, precision=prec)
+    cols = np.unique(np.r_[0, np.arange(255, B, 256), B - 1])
+    rnl, rG = O.fg_batched(models, data, X[:, cols], nthreads=8)
+    ll_err, g_err = _errors(negll[cols], G[:, cols], rnl, rG)
+    print(f"{prec} 60000 x 700 x 4096: LL rel-err {ll_err:.2e}, grad norm-rel-err {g_err:.2e}")
+    lb, gb = BOUNDS[prec]
+    assert np.all(np.isfinite(negll)) and np.all(np.isfinite(G))
+    assert ll_err <= lb
+    assert g_err <= gb
+
+
+@pytest.mark.parametrize("prec", ["f64", "bf16x2", "fp32"])
 def test_batched_extreme_values(sfh, O, gpu_ctx, prec):
     """Empty bins, counts up to 1e7, an all-zero template, composites clamped
     at eps (all-zero and 1e-30 walkers) and huge coefficients: the
@@ -66,10 +90,10 @@ def test_batched_extreme_values(sfh, O, gpu_ctx, prec):
     rnl, rG = O.fg_batched(models, data, X, nthreads=8)
     ll_err, g_err = _errors(negll, G, rnl, rG)
     print(f"{prec} extreme values: LL rel-err {ll_err:.2e}, grad norm-rel-err {g_err:.2e}")
-    lb, gb = BOUNDS[prec]
+    lb, _ = BOUNDS[prec]
     assert np.all(np.isfinite(negll)) and np.all(np.isfinite(G))
     assert ll_err <= lb
-    assert g_err <= gb
+    assert g_err <= EXTREME_GRAD[prec]
 
 
 def test_batched_without_gradient_and_device(sfh, O, gpu_ctx):
@@ -96,7 +120,7 @@ def test_batched_single_walker_matches_plain(sfh, gpu_ctx):
     Gp = np.empty(64)
     nl = sfh.fg_(True, Gp, X[:, 0], tpl, data)
     assert abs(negll[0] - nl) <= 1e-9 * abs(nl)
-    assert np.linalg.norm(G[:, 0] - Gp) <= 1e-4 * np.linalg.norm(Gp)
+    assert np.linalg.norm(G[:, 0] - Gp) <= 2e-6 * np.linalg.norm(Gp)
 
 
 def test_batched_argument_errors(sfh, gpu_ctx):
