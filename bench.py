@@ -623,16 +623,17 @@ def templates_bench(sfh, torch, ctx, O, args, ncpu):
     dep_ms = kms["deposit_cov"]
     prep_ms = kms["pcs_count"] + kms["pcs_points"] + kms["pcs_desc"]
     pairs_per_s = pairs / (dep_ms * 1e-3)
-    # VALU work per pair of the gather's dominant form (degree-1 series, full
-    # width): per GL node and bin 5 VALU instructions (column coordinate, its
-    # square, the series polynomial, exp2, the accumulation); 3 nodes per pair
+    # VALU issue slots per pair of the gather's dominant form (the Gaussian
+    # form of the narrowest bins, full width): per GL node and bin the column
+    # coordinate (FMA), its square, exp2 (a transcendental: two issue slots)
+    # and the accumulation (FMA); 3 nodes per pair
     ops_per_pair = 15.0
     achieved = pairs_per_s * ops_per_pair
     roof = {"bound": "valu", "kernel": "k_dep_gather<false>", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
-            "unit": "T VALU lane-instructions/s", "frac": achieved / VALU_PEAK, "traffic": None,
-            "algorithmic": f"{ops_per_pair:g} VALU instructions per (point, bin) pair (3 GL nodes x (FMA coordinate, "
-                           f"square, degree-1 series FMA, exp2, accumulate FMA)) x {pairs:.4g} pairs per launch; "
-                           f"peak = 256 CUs x 64 lanes x 2.4 GHz"}
+            "unit": "T VALU lane-slots/s", "frac": achieved / VALU_PEAK, "traffic": None,
+            "algorithmic": f"{ops_per_pair:g} VALU issue slots per (point, bin) pair (3 GL nodes x (coordinate FMA, "
+                           f"square, exp2 = 2 slots, accumulate FMA)) x {pairs:.4g} pairs per launch; "
+                           f"peak = 256 CUs x 64 lanes x 2.4 GHz lane-slots/s"}
     # the same templates through the host (NumPy) preparation, for comparison
     t0 = time.time()
     pc = [synth.template_points(s, 2000, edges, 120.0, correlation=0.6) for s in specs]

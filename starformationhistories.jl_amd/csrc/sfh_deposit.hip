@@ -88,38 +88,65 @@ __global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) 
         part[c] = 0.0f;
     }
     // Each chunk of 64 points: every lane reads one point's footprint (the
-    // descriptor's first 16 bytes), a ballot keeps the points whose rows meet
-    // this wave's strips, and only those descriptors are loaded -- two at a
-    // time through scalar loads (uniform across the wave, one wait per pair).
+    // descriptor's first 16 bytes) and form, a ballot keeps the points whose
+    // rows meet this wave's strips, and only those descriptors are loaded --
+    // through scalar loads (uniform across the wave).  The hits are taken in
+    // two passes: first the common case (the Gaussian form with a footprint
+    // spanning the whole grid width), two descriptors per wait, then every
+    // other point.  Each pass has one code path, so the strip's registers
+    // carry through it without copies; a bin sums the chunk's common-case
+    // points, then the others, each in point order (deterministic).
     const PointDesc* __restrict__ td = desc + p0;
     const int np = (int)(p1 - p0);
     const int lane = threadIdx.x & 63;
+    struct Foot {
+        int4 f;
+        int kind;
+    };
     auto footprint = [&](int pc) {
         const int p = pc + lane;
-        return p < np ? *reinterpret_cast<const int4*>(td + p) : make_int4(1, 0, 1, 0);
+        Foot r;
+        if (p < np) {
+            r.f = *reinterpret_cast<const int4*>(td + p);
+            r.kind = td[p].kind;
+        } else {
+            r.f = make_int4(1, 0, 1, 0);
+            r.kind = psf::kDirect;
+        }
+        return r;
+    };
+    auto in_rows = [&](const PointDesc& d) { return valid && row >= d.ya && row <= d.yb; };
+    auto add_common = [&](const PointDesc& d) {
+        if (!in_rows(d)) return;
+        if constexpr (kAsym) {
+            float M[1], L[1] = {0.0f}, h[1] = {0.0f};
+            M[0] = d.pref * psf::y_factor(fmaf((float)(row - d.ya), d.dv, d.ay), d.ey);
+            psf::gauss_strip<1, false>(d, M, L, h, col0, part);
+        } else {
+            float M[3], L[3], h[3];
+            psf::cov_row_log(d, row, M, L, h);
+            psf::gauss_strip<3, false>(d, M, L, h, col0, part);
+        }
     };
     auto add = [&](const PointDesc& d) {
-        if (!valid || row < d.ya || row > d.yb || col0 > d.xb || col0 + kCPT - 1 < d.xa) return;
-        // full width: every column of the grid is in the footprint, and the
-        // strips' columns past the grid are never written, so no mask
-        const bool full = d.xa == 0 && d.xb == nx - 1;
+        if (!in_rows(d) || col0 > d.xb || col0 + kCPT - 1 < d.xa) return;
         if constexpr (kAsym) {
             float W[1], h[1] = {0.0f};
             W[0] = d.pref * psf::y_factor(fmaf((float)(row - d.ya), d.dv, d.ay), d.ey);
-            if (full) psf::add_point<1, false>(d, W, h, col0, part);
-            else psf::add_point<1, true>(d, W, h, col0, part);
+            psf::add_point<1, true>(d, W, h, col0, part);
         } else {
             float W[3], h[3];
             psf::cov_row(d, row, W, h);
-            if (full) psf::add_point<3, false>(d, W, h, col0, part);
-            else psf::add_point<3, true>(d, W, h, col0, part);
+            psf::add_point<3, true>(d, W, h, col0, part);
         }
     };
-    int4 fn = footprint(0);
+    Foot fn = footprint(0);
     for (int pc = 0; pc < np; pc += kChunk) {
-        const int4 f = fn;
+        const Foot f = fn;
         fn = footprint(pc + kChunk);                           // next chunk's footprints in flight
-        uint64_t hits = __ballot(f.x <= f.y && f.z <= row_hi && f.w >= row_lo);
+        const bool hit = f.f.x <= f.f.y && f.f.z <= row_hi && f.f.w >= row_lo;
+        const bool common = f.kind == psf::kGauss && f.f.x == 0 && f.f.y == nx - 1;
+        uint64_t hits = __ballot(hit && common);
         while (hits) {
             const int i = __builtin_ctzll(hits);
             hits &= hits - 1;
@@ -128,8 +155,14 @@ __global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) 
             if (two) hits &= hits - 1;
             const PointDesc d0 = td[pc + i];
             const PointDesc d1 = td[pc + j];
-            add(d0);
-            if (two) add(d1);
+            add_common(d0);
+            if (two) add_common(d1);
+        }
+        hits = __ballot(hit && !common);
+        while (hits) {
+            const int i = __builtin_ctzll(hits);
+            hits &= hits - 1;
+            add(td[pc + i]);
         }
 #pragma unroll
         for (int c = 0; c < kCPT; ++c) {

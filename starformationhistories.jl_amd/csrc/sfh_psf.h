@@ -23,10 +23,16 @@
 // value at |u| <= 4 the dropped terms are below ~1e-7 for DEG = 1 when
 // e < 0.0077, DEG = 2 when e < 0.026 and DEG = 3 when e < 0.1 (the terms of
 // order e^2n |H_2n(4)| / (2n+1)! are ~10 e^2, 28 e^4, 30 e^6, 15 e^8).
+// For e < 0.0077 the degree-1 factor is kept in the exponent instead:
+// s0 + s1 w = s0 exp(b w) (1 + O(b^2 w^2)), b = s1 / s0 = 2e^2/3 + O(e^4), so
+// D = (4 e / sqrt(pi)) s0 exp(-(1 - b) w): one Gaussian of a rescaled
+// coordinate.  ln S(w) = ln s0 + b w - (4/45) e^4 w^2 + ..., so the dropped
+// term, (4/45) e^4 w^2 <= 8e-8 at |u| = 4, is below the polynomial's own
+// truncation error (2/15) e^4 w^2.
 //
-// The series forms run in scaled coordinates a' = a sqrt(log2 e), so
-// exp(-a^2) = exp2(-a'^2) is one hardware exp2, with the coefficients of S
-// rescaled to w' = a'^2.  The direct form shares erf values between adjacent
+// The series forms run in scaled coordinates a' = a sqrt(log2 e) (the
+// Gaussian form: a sqrt((1 - b) log2 e)), so exp(-a^2) = exp2(-a'^2) is one
+// hardware exp2, with the coefficients of S rescaled to w' = a'^2.  The direct form shares erf values between adjacent
 // columns: the right edge of column c is the left edge of column c + 1.
 #pragma once
 
@@ -49,8 +55,9 @@ constexpr double kLog2e = 1.4426950408889634;
 constexpr double kSqrtLog2e = 1.2011224087864498;
 constexpr int kCPT = 16;                        // consecutive columns evaluated per thread
 
-// x evaluation of a point: the erf difference or the series of degree 1..3
-enum : int32_t { kDirect = 0, kSeries1 = 1, kSeries2 = 2, kSeries3 = 3 };
+// x evaluation of a point: the erf difference, the Gaussian form, or the
+// series of degree 2..3
+enum : int32_t { kDirect = 0, kGauss = 1, kSeries2 = 2, kSeries3 = 3 };
 
 // One prepared kernel: the clipped footprint and its FP32 parameters.
 // Footprint indices are 0-based and inclusive; xa > xb marks "nothing to
@@ -66,9 +73,10 @@ struct alignas(16) PointDesc {
     float dv;     // t (or v) step per row
     float ey;     // covariant: GL node offset xi*hy/sy; asymmetric: half row width in v
     float pref;   // amplitude prefactor (see header), with the series form factors
-    float s0, s1, s2, s3;   // x series coefficients in w' (series kinds)
-    int32_t kind;           // kDirect or kSeriesN for the x factor
+    float s0, s1, s2, s3;   // x series coefficients in w' (series kinds; kGauss: s0 = 1)
+    int32_t kind;           // kDirect, kGauss or kSeriesN for the x factor
     int32_t pad0, pad1, pad2;
+    float q[8];             // kGauss: q[j] = 2^-(du (j + 1/2))^2, the strip's column factors
 };
 
 struct Range {
@@ -133,17 +141,37 @@ __host__ __device__ inline void set_x_form(PointDesc* d, float e)
         d->kind = kDirect;
         return;
     }
-    d->kind = e < kDeg1Eps ? kSeries1 : e < kDeg2Eps ? kSeries2 : kSeries3;
+    d->kind = e < kDeg1Eps ? kGauss : e < kDeg2Eps ? kSeries2 : kSeries3;
     const Series s = series_coeffs(e);
     const double l1 = 1.0 / kLog2e, l2 = l1 * l1, l3 = l2 * l1;
-    d->s0 = s.s0;
-    d->s1 = (float)(s.s1 * l1);
-    d->s2 = (float)(s.s2 * l2);
-    d->s3 = (float)(s.s3 * l3);
-    // coordinates of the series forms in a' = a sqrt(log2 e)
-    d->ax = (float)((double)d->ax * kSqrtLog2e);
-    d->du = (float)((double)d->du * kSqrtLog2e);
-    d->g = (float)((double)d->g * kSqrtLog2e);
+    double scale = kSqrtLog2e;
+    if (d->kind == kGauss) {
+        // s0 exp(-(1 - b) w), b = s1 / s0: the coordinates carry sqrt(1 - b),
+        // the amplitude s0
+        const double b = (double)s.s1 / (double)s.s0;
+        scale *= sqrt(1.0 - b);
+        d->pref = (float)((double)d->pref * (double)s.s0);
+        d->s0 = 1.0f;
+        d->s1 = d->s2 = d->s3 = 0.0f;
+    } else {
+        d->s0 = s.s0;
+        d->s1 = (float)(s.s1 * l1);
+        d->s2 = (float)(s.s2 * l2);
+        d->s3 = (float)(s.s3 * l3);
+    }
+    // coordinates of the series forms in a' = a sqrt(log2 e) (times sqrt(1 - b))
+    d->ax = (float)((double)d->ax * scale);
+    d->du = (float)((double)d->du * scale);
+    d->g = (float)((double)d->g * scale);
+    if (d->kind == kGauss) {
+        // the column factors of the strip recurrence (gauss_strip), from the
+        // rounded step the kernel uses
+        const double du = (double)d->du;
+        for (int j = 0; j < 8; ++j) {
+            const double sj = du * ((double)j + 0.5);
+            d->q[j] = (float)exp2(-sj * sj);
+        }
+    }
 }
 
 // Covariant kernel centred at (x0, y0) (real units) -- addstar! :366-408.
@@ -324,6 +352,25 @@ __host__ __device__ inline void cov_row(const PointDesc& d, int row, float* W, f
     h[2] = d.g * t2;
 }
 
+// The covariant row in the Gaussian form's terms: M_k = pref w_k, the log2 of
+// the node's y weight L_k = -(log2 e / 2) t_k^2 (it joins the exponent of the
+// strip recurrence, so no exp2 of its own) and the x shifts h_k = g t_k.
+__host__ __device__ inline void cov_row_log(const PointDesc& d, int row, float* M, float* L, float* h)
+{
+    const float tj = fmaf((float)(row - d.ya), d.dv, d.ay);
+    const float t0 = tj - d.ey, t2 = tj + d.ey;
+    const float hl = (float)(-0.5 * kLog2e);
+    M[0] = d.pref * kW0;
+    M[1] = d.pref * kW1;
+    M[2] = M[0];
+    L[0] = hl * t0 * t0;
+    L[1] = hl * tj * tj;
+    L[2] = hl * t2 * t2;
+    h[0] = d.g * t0;
+    h[1] = d.g * tj;
+    h[2] = d.g * t2;
+}
+
 // c du + a for a compile-time column offset c.  On the device the offset is a
 // literal of v_fmamk_f32 (D = S0 K + S1, fused): written as a plain fmaf the
 // compiler keeps every column's offset in a register of its own (VOP3 takes no
@@ -350,17 +397,64 @@ __host__ __device__ __forceinline__ void static_for(F&& f)
     }
 }
 
+// The Gaussian form over one row's strip: part[c] += sum_k M[k] 2^(L[k]) E_k(c),
+// E_k(c) = 2^-(a_k(c))^2, by a product recurrence from the strip's middle.
+// With m the x coordinate midway between columns 7 and 8 and s = c - 7.5,
+//   (m + s du)^2 = m^2 + 2 s m du + s^2 du^2,
+// so E(8 + j) = 2^-(m^2 + m du) B^j q[j] and E(7 - j) = 2^-(m^2 - m du) B^-j q[j],
+// B = 2^(-2 m du), q[j] = 2^-(du (j + 1/2))^2 (prepared per point).  Four exp2
+// per node instead of one per column (the transcendental unit is the bound:
+// an exp2 issues ~7x slower than an FMA); the multiplier's rounding grows
+// over at most 7 steps.  The Gaussian form has du < 0.019 (half bin width
+// e < 0.0077), so B stays within [2^-1, 2^1] for |m| < 27.  Each column adds
+// the nodes in order k = 0, 1, 2.
+template <int NODES, bool MASK>
+__host__ __device__ __forceinline__ void gauss_strip(const PointDesc& d, const float* M, const float* L,
+                                                     const float* h, int col0, float* part)
+{
+    static_assert(kCPT == 16, "the recurrence runs 8 columns each way from the strip's middle");
+    const float base = fmaf((float)(col0 - d.xa), d.du, d.ax);
+    const int lo = d.xa - col0, hi = d.xb - col0;
+    const float du = d.du;
+#pragma unroll
+    for (int k = 0; k < NODES; ++k) {
+        const float m = fmaf(7.5f, du, base + h[k]);
+        float pf = M[k] * fast_exp2(L[k] - m * (m + du));
+        float pb = M[k] * fast_exp2(L[k] - m * (m - du));
+        const float md2 = 2.0f * m * du;
+        const float B = fast_exp2(-md2), Bi = fast_exp2(md2);
+        static_for<0, 8>([&](auto jj) {
+            constexpr int j = decltype(jj)::value;
+            constexpr int cf = 8 + j, cb = 7 - j;
+            const float vf = fmaf(pf, d.q[j], part[cf]);
+            const float vb = fmaf(pb, d.q[j], part[cb]);
+            if (MASK) {
+                part[cf] = (cf >= lo && cf <= hi) ? vf : part[cf];
+                part[cb] = (cb >= lo && cb <= hi) ? vb : part[cb];
+            } else {
+                part[cf] = vf;
+                part[cb] = vb;
+            }
+            if constexpr (j < 7) {
+                pf *= B;
+                pb *= Bi;
+            }
+        });
+    }
+}
+
 // Adds the contributions of one point to the kCPT columns col0 .. col0+kCPT-1
 // of one row: part[c] += sum_k W[k] X(a_k(c)), a_k(c) the x coordinate of
 // column col0 + c shifted by h[k] (NODES = 3: covariant GL nodes; NODES = 1:
 // asymmetric, W[0] = pref * y factor).  DEG: the x form (kDirect or a series
-// degree).  MASK: only columns inside [xa, xb] receive anything (without it
-// every column does: the footprint spans the whole grid and columns past the
-// grid are never written).
+// degree 2, 3).  MASK: only columns inside [xa, xb] receive anything (without
+// it every column does: the footprint spans the whole grid and columns past
+// the grid are never written).  Each column adds the nodes in order.
 template <int NODES, int DEG, bool MASK>
 __host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const float* W, const float* h, int col0,
                                                    float* part)
 {
+    static_assert(DEG == kDirect || DEG == 2 || DEG == 3, "the Gaussian form is gauss_strip");
     const float base = fmaf((float)(col0 - d.xa), d.du, d.ax);
     const int lo = d.xa - col0, hi = d.xb - col0;
     const float du = d.du;
@@ -368,36 +462,32 @@ __host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const flo
         // erf at the column edges b(c) = a(c) - ex, c = 0..kCPT, as erfc(|b|) so
         // that D(c) = erf(b(c+1)) - erf(b(c)) keeps its relative accuracy in
         // the tails: both edges >= 0: erfc(b0) - erfc(b1); both < 0:
-        // erfc(-b1) - erfc(-b0); across 0: 2 - erfc(b1) - erfc(-b0)
-        float prev[NODES], b0[NODES];
-        bool pneg[NODES];
+        // erfc(-b1) - erfc(-b0); across 0: 2 - erfc(b1) - erfc(-b0).  Node by
+        // node (each column still adds the nodes in order), so that one
+        // node's edge values are live at a time.
 #pragma unroll
         for (int k = 0; k < NODES; ++k) {
-            b0[k] = base + h[k] - d.ex;
-            pneg[k] = b0[k] < 0.0f;
-            prev[k] = erfc_pos(fabsf(b0[k]));
-        }
-        static_for<0, kCPT>([&](auto cc) {
-            constexpr int c = decltype(cc)::value;
-            float v = part[c];
-#pragma unroll
-            for (int k = 0; k < NODES; ++k) {
-                const float b1 = col_fma<c + 1>(du, b0[k]);
+            const float b0 = base + h[k] - d.ex;
+            bool pneg = b0 < 0.0f;
+            float prev = erfc_pos(fabsf(b0));
+            static_for<0, kCPT>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                const float b1 = col_fma<c + 1>(du, b0);
                 const bool neg = b1 < 0.0f;
                 const float e1 = erfc_pos(fabsf(b1));
-                const float dd = neg ? e1 - prev[k] : (pneg[k] ? 2.0f - e1 - prev[k] : prev[k] - e1);
-                v = fmaf(W[k], dd, v);
-                prev[k] = e1;
-                pneg[k] = neg;
-            }
-            if (MASK) part[c] = (c >= lo && c <= hi) ? v : part[c];
-            else part[c] = v;
+                const float dd = neg ? e1 - prev : (pneg ? 2.0f - e1 - prev : prev - e1);
+                const float v = fmaf(W[k], dd, part[c]);
+                if (MASK) part[c] = (c >= lo && c <= hi) ? v : part[c];
+                else part[c] = v;
+                prev = e1;
+                pneg = neg;
 #if defined(__HIP_DEVICE_COMPILE__)
-            // one column at a time: interleaving the columns' erfc sequences
-            // would raise the register count of the whole kernel
-            __builtin_amdgcn_sched_barrier(0);
+                // one column at a time: interleaving the columns' erfc sequences
+                // would raise the register count of the whole kernel
+                __builtin_amdgcn_sched_barrier(0);
 #endif
-        });
+            });
+        }
     } else {
         float a0[NODES], c0[NODES], c1[NODES], c2[NODES], c3[NODES];
 #pragma unroll
@@ -405,7 +495,7 @@ __host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const flo
             a0[k] = base + h[k];
             c0[k] = W[k] * d.s0;
             c1[k] = W[k] * d.s1;
-            c2[k] = DEG >= 2 ? W[k] * d.s2 : 0.0f;
+            c2[k] = W[k] * d.s2;
             c3[k] = DEG >= 3 ? W[k] * d.s3 : 0.0f;
         }
         static_for<0, kCPT>([&](auto cc) {
@@ -416,8 +506,7 @@ __host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const flo
                 const float a = col_fma<c>(du, a0[k]);
                 const float w = a * a;
                 float S;
-                if constexpr (DEG == 1) S = fmaf(c1[k], w, c0[k]);
-                else if constexpr (DEG == 2) S = fmaf(fmaf(c2[k], w, c1[k]), w, c0[k]);
+                if constexpr (DEG == 2) S = fmaf(fmaf(c2[k], w, c1[k]), w, c0[k]);
                 else S = fmaf(fmaf(fmaf(c3[k], w, c2[k]), w, c1[k]), w, c0[k]);
                 v = fmaf(S, fast_exp2(-w), v);
             }
@@ -438,7 +527,11 @@ __host__ __device__ __forceinline__ void add_point(const PointDesc& d, const flo
                                                    float* part)
 {
     switch (d.kind) {
-    case kSeries1: add_strip<NODES, 1, MASK>(d, W, h, col0, part); break;
+    case kGauss: {
+        const float L[NODES] = {};
+        gauss_strip<NODES, MASK>(d, W, L, h, col0, part);
+        break;
+    }
     case kSeries2: add_strip<NODES, 2, MASK>(d, W, h, col0, part); break;
     case kSeries3: add_strip<NODES, 3, MASK>(d, W, h, col0, part); break;
     default: add_strip<NODES, kDirect, MASK>(d, W, h, col0, part); break;

@@ -1,8 +1,10 @@
 // psf_emu.cpp -- host build of the deposition kernel's arithmetic (the
 // __host__ __device__ functions of sfh_psf.h) arranged exactly like
-// k_dep_gather: FP32 per-point contributions summed per 64-point chunk, then
-// flushed into FP64.  Lets the CPU test-suite check the FP32 formulation and
-// the footprint index math against the FP64 oracle without a GPU.
+// k_dep_gather: FP32 per-point contributions summed per 64-point chunk -- the
+// chunk's common-case points (Gaussian form, footprint spanning the grid
+// width) first, then the others, each in point order -- then flushed into
+// FP64.  Lets the CPU test-suite check the FP32 formulation and the footprint
+// index math against the FP64 oracle without a GPU.
 #include "../../starformationhistories.jl_amd/csrc/sfh_psf.h"
 
 #include <cstring>
@@ -30,19 +32,29 @@ extern "C" int64_t emu_bin_cmd_smooth(const double* colors, const double* mags, 
     const size_t ld = (size_t)ncg * kCPT;
     for (int64_t pc = 0; pc < P; pc += 64) {
         const int64_t pe = pc + 64 < P ? pc + 64 : P;
-        for (int64_t q = pc; q < pe; ++q) {
-            const PointDesc& d = desc[q];
-            if (d.xa > d.xb) continue;
-            for (int row = d.ya; row <= d.yb; ++row) {
-                float W[3], h[3] = {0.0f, 0.0f, 0.0f};
-                if (asym) W[0] = d.pref * y_factor(fmaf((float)(row - d.ya), d.dv, d.ay), d.ey);
-                else cov_row(d, row, W, h);
-                for (int cg = 0; cg < ncg; ++cg) {
-                    const int col0 = cg * kCPT;
-                    if (col0 > d.xb || col0 + kCPT - 1 < d.xa) continue;
-                    float* pp = &part[(size_t)row * ld + col0];
-                    if (asym) add_point<1, true>(d, W, h, col0, pp);
-                    else add_point<3, true>(d, W, h, col0, pp);
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int64_t q = pc; q < pe; ++q) {
+                const PointDesc& d = desc[q];
+                if (d.xa > d.xb) continue;
+                const bool common = d.kind == kGauss && d.xa == 0 && d.xb == nx - 1;
+                if (common != (pass == 0)) continue;
+                for (int row = d.ya; row <= d.yb; ++row) {
+                    float W[3], L[3] = {0.0f, 0.0f, 0.0f}, h[3] = {0.0f, 0.0f, 0.0f};
+                    if (asym) W[0] = d.pref * y_factor(fmaf((float)(row - d.ya), d.dv, d.ay), d.ey);
+                    else if (common) cov_row_log(d, row, W, L, h);
+                    else cov_row(d, row, W, h);
+                    for (int cg = 0; cg < ncg; ++cg) {
+                        const int col0 = cg * kCPT;
+                        float* pp = &part[(size_t)row * ld + col0];
+                        if (common) {
+                            if (asym) gauss_strip<1, false>(d, W, L, h, col0, pp);
+                            else gauss_strip<3, false>(d, W, L, h, col0, pp);
+                            continue;
+                        }
+                        if (col0 > d.xb || col0 + kCPT - 1 < d.xa) continue;
+                        if (asym) add_point<1, true>(d, W, h, col0, pp);
+                        else add_point<3, true>(d, W, h, col0, pp);
+                    }
                 }
             }
         }
