@@ -623,17 +623,21 @@ def templates_bench(sfh, torch, ctx, O, args, ncpu):
     dep_ms = kms["deposit_cov"]
     prep_ms = kms["pcs_count"] + kms["pcs_points"] + kms["pcs_desc"]
     pairs_per_s = pairs / (dep_ms * 1e-3)
-    # VALU issue slots per pair of the gather's dominant form (the Gaussian
-    # form of the narrowest bins, full width): per GL node and bin the column
-    # coordinate (FMA), its square, exp2 (a transcendental: two issue slots)
-    # and the accumulation (FMA); 3 nodes per pair
-    ops_per_pair = 15.0
+    # VALU issue slots per pair of the gather's dominant path (the Gaussian
+    # form over a full-width row strip of 16 bins, 94% of the pairs): per GL
+    # node and bin one FMA and one recurrence multiply (6 per pair), plus the
+    # strip's set-up amortised over its 16 bins (137 VALU instructions in all
+    # per strip in the compiled kernel, 5 of them exp2 at two issue slots:
+    # 142 slots / 16 bins)
+    ops_per_pair = 142.0 / 16.0
     achieved = pairs_per_s * ops_per_pair
     roof = {"bound": "valu", "kernel": "k_dep_gather<false>", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
             "unit": "T VALU lane-slots/s", "frac": achieved / VALU_PEAK, "traffic": None,
-            "algorithmic": f"{ops_per_pair:g} VALU issue slots per (point, bin) pair (3 GL nodes x (coordinate FMA, "
-                           f"square, exp2 = 2 slots, accumulate FMA)) x {pairs:.4g} pairs per launch; "
-                           f"peak = 256 CUs x 64 lanes x 2.4 GHz lane-slots/s"}
+            "algorithmic": f"{ops_per_pair:g} VALU issue slots per (point, bin) pair (the Gaussian-form strip: "
+                           f"3 GL nodes x (accumulate FMA + recurrence multiply) per bin, plus the strip's set-up "
+                           f"of 5 exp2 (2 slots each) and 42 other instructions over 16 bins) x {pairs:.4g} pairs "
+                           f"per launch; peak = 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-slots/s (a wave64 "
+                           f"FP32 instruction issues every 4 cycles)"}
     # the same templates through the host (NumPy) preparation, for comparison
     t0 = time.time()
     pc = [synth.template_points(s, 2000, edges, 120.0, correlation=0.6) for s in specs]

@@ -20,15 +20,18 @@
 //   D = (4 e / sqrt(pi)) exp(-u^2) S(u^2),  S(w) = sum_n H_2n(sqrt w) e^2n / (2n+1)!
 // (the mean of exp(-(u+s)^2) over |s| < e; Hermite polynomials H_2n), whose
 // coefficients in powers of w are kept up to w^DEG.  Relative to the local
-// value at |u| <= 4 the dropped terms are below ~1e-7 for DEG = 1 when
-// e < 0.0077, DEG = 2 when e < 0.026 and DEG = 3 when e < 0.1 (the terms of
-// order e^2n |H_2n(4)| / (2n+1)! are ~10 e^2, 28 e^4, 30 e^6, 15 e^8).
-// For e < 0.0077 the degree-1 factor is kept in the exponent instead:
-// s0 + s1 w = s0 exp(b w) (1 + O(b^2 w^2)), b = s1 / s0 = 2e^2/3 + O(e^4), so
-// D = (4 e / sqrt(pi)) s0 exp(-(1 - b) w): one Gaussian of a rescaled
-// coordinate.  ln S(w) = ln s0 + b w - (4/45) e^4 w^2 + ..., so the dropped
-// term, (4/45) e^4 w^2 <= 8e-8 at |u| = 4, is below the polynomial's own
-// truncation error (2/15) e^4 w^2.
+// value at |u| <= 4 the dropped terms are below ~1e-7 for DEG = 2 when
+// e < 0.026 and DEG = 3 when e < 0.1 (the terms of order e^2n |H_2n(4)| /
+// (2n+1)! are ~10 e^2, 28 e^4, 30 e^6, 15 e^8).
+// For e < 0.016 the factor is kept in the exponent instead (the Gaussian
+// form): s0 + s1 w = s0 exp(b w) (1 + O(b^2 w^2)), b = s1 / s0 = 2e^2/3 +
+// O(e^4), so D = (4 e / sqrt(pi)) s0 exp(-(1 - b) w), one Gaussian of a
+// rescaled coordinate.  ln S(w) = ln s0 + b w - (4/45) e^4 w^2 + ..., so the
+// dropped term is (4/45) e^4 w^2 relative to the local value: <= 1.5e-6 at
+// |u| = 4 for e < 0.016 (below 8e-8 for e < 0.0077, and always below the
+// degree-1 polynomial's own truncation error (2/15) e^4 w^2); relative to
+// the kernel's peak it is below 4e-9.  On the 700-template workload 94% of
+// the pairs take this form.
 //
 // The series forms run in scaled coordinates a' = a sqrt(log2 e) (the
 // Gaussian form: a sqrt((1 - b) log2 e)), so exp(-a^2) = exp2(-a'^2) is one
@@ -46,7 +49,7 @@ namespace sfh {
 namespace psf {
 
 constexpr float kSeriesEps = 0.1f;
-constexpr float kDeg1Eps = 0.0077f;
+constexpr float kDeg1Eps = 0.016f;    // the Gaussian form below this half width
 constexpr float kDeg2Eps = 0.026f;
 constexpr double kGLNode = 0.7745966692414834;   // :303
 constexpr float kW0 = 0.5555555555555556f;      // :304
@@ -75,7 +78,9 @@ struct alignas(16) PointDesc {
     float pref;   // amplitude prefactor (see header), with the series form factors
     float s0, s1, s2, s3;   // x series coefficients in w' (series kinds; kGauss: s0 = 1)
     int32_t kind;           // kDirect, kGauss or kSeriesN for the x factor
-    int32_t pad0, pad1, pad2;
+    float gf, gfi;          // kGauss, covariant: 2^(2 g ey du) and its inverse (the
+                            // outer GL nodes' strip multipliers relative to the middle node's)
+    int32_t pad0;
     float q[8];             // kGauss: q[j] = 2^-(du (j + 1/2))^2, the strip's column factors
 };
 
@@ -164,13 +169,16 @@ __host__ __device__ inline void set_x_form(PointDesc* d, float e)
     d->du = (float)((double)d->du * scale);
     d->g = (float)((double)d->g * scale);
     if (d->kind == kGauss) {
-        // the column factors of the strip recurrence (gauss_strip), from the
-        // rounded step the kernel uses
+        // the column factors of the strip recurrence (gauss_strip) and the
+        // outer nodes' multiplier ratios, from the rounded values the kernel uses
         const double du = (double)d->du;
         for (int j = 0; j < 8; ++j) {
             const double sj = du * ((double)j + 0.5);
             d->q[j] = (float)exp2(-sj * sj);
         }
+        const double r = 2.0 * (double)d->g * (double)d->ey * du;
+        d->gf = (float)exp2(r);
+        d->gfi = (float)exp2(-r);
     }
 }
 
@@ -401,33 +409,52 @@ __host__ __device__ __forceinline__ void static_for(F&& f)
 // E_k(c) = 2^-(a_k(c))^2, by a product recurrence from the strip's middle.
 // With m the x coordinate midway between columns 7 and 8 and s = c - 7.5,
 //   (m + s du)^2 = m^2 + 2 s m du + s^2 du^2,
-// so E(8 + j) = 2^-(m^2 + m du) B^j q[j] and E(7 - j) = 2^-(m^2 - m du) B^-j q[j],
-// B = 2^(-2 m du), q[j] = 2^-(du (j + 1/2))^2 (prepared per point).  Four exp2
-// per node instead of one per column (the transcendental unit is the bound:
-// an exp2 issues ~7x slower than an FMA); the multiplier's rounding grows
-// over at most 7 steps.  The Gaussian form has du < 0.019 (half bin width
-// e < 0.0077), so B stays within [2^-1, 2^1] for |m| < 27.  Each column adds
-// the nodes in order k = 0, 1, 2.
+// so E(8 + j) = P B^j q[j] and E(7 - j) = P B^-(j+1) q[j], P = 2^-(m^2 + m du),
+// B = 2^(-2 m du), q[j] = 2^-(du (j + 1/2))^2 (prepared per point).  For the
+// three GL nodes (m_k - m_1 = -+g ey) the outer nodes' B are the middle node's
+// times 2^(+-2 g ey du) (prepared per point), so a row takes 5 exp2 (3 for the
+// asymmetric kernel) instead of one per node and column (an exp2 takes two
+// VALU issue slots, an FMA or multiply one), and a column costs one FMA and
+// one multiply per node.  The multiplier's rounding grows over at most 8
+// steps.  The Gaussian form has du < 0.039 (half bin width e < 0.016), so B
+// stays within [2^-2, 2^2] for |m| < 25.  Each column adds the nodes in order
+// k = 0, 1, 2.
 template <int NODES, bool MASK>
 __host__ __device__ __forceinline__ void gauss_strip(const PointDesc& d, const float* M, const float* L,
                                                      const float* h, int col0, float* part)
 {
     static_assert(kCPT == 16, "the recurrence runs 8 columns each way from the strip's middle");
+    static_assert(NODES == 1 || NODES == 3, "one node (asymmetric) or three GL nodes (covariant)");
     const float base = fmaf((float)(col0 - d.xa), d.du, d.ax);
     const int lo = d.xa - col0, hi = d.xb - col0;
     const float du = d.du;
+    constexpr int mid = NODES / 2;
+    float pf[NODES], B[NODES], Bi[NODES];
 #pragma unroll
     for (int k = 0; k < NODES; ++k) {
         const float m = fmaf(7.5f, du, base + h[k]);
-        float pf = M[k] * fast_exp2(L[k] - m * (m + du));
-        float pb = M[k] * fast_exp2(L[k] - m * (m - du));
-        const float md2 = 2.0f * m * du;
-        const float B = fast_exp2(-md2), Bi = fast_exp2(md2);
+        pf[k] = M[k] * fast_exp2(L[k] - m * (m + du));
+        if (k == mid) {
+            const float md2 = 2.0f * m * du;
+            B[k] = fast_exp2(-md2);
+            Bi[k] = fast_exp2(md2);
+        }
+    }
+    if constexpr (NODES == 3) {
+        B[0] = B[1] * d.gf;
+        Bi[0] = Bi[1] * d.gfi;
+        B[2] = B[1] * d.gfi;
+        Bi[2] = Bi[1] * d.gf;
+    }
+#pragma unroll
+    for (int k = 0; k < NODES; ++k) {
+        float f = pf[k], b = pf[k] * Bi[k];
+        const float Bk = B[k], Bik = Bi[k];
         static_for<0, 8>([&](auto jj) {
             constexpr int j = decltype(jj)::value;
             constexpr int cf = 8 + j, cb = 7 - j;
-            const float vf = fmaf(pf, d.q[j], part[cf]);
-            const float vb = fmaf(pb, d.q[j], part[cb]);
+            const float vf = fmaf(f, d.q[j], part[cf]);
+            const float vb = fmaf(b, d.q[j], part[cb]);
             if (MASK) {
                 part[cf] = (cf >= lo && cf <= hi) ? vf : part[cf];
                 part[cb] = (cb >= lo && cb <= hi) ? vb : part[cb];
@@ -436,8 +463,8 @@ __host__ __device__ __forceinline__ void gauss_strip(const PointDesc& d, const f
                 part[cb] = vb;
             }
             if constexpr (j < 7) {
-                pf *= B;
-                pb *= Bi;
+                f *= Bk;
+                b *= Bik;
             }
         });
     }
