@@ -841,6 +841,22 @@ def main():
                                         f"+ FP64 data); the 168 MB template set fits the 256 MB Infinity Cache, "
                                         f"so repeated evaluations can be served above the HBM rate"},
             "ll_rel_err": float(abs(n1.item() - rnl[0]) / abs(rnl[0]))}
+        # the drop-in fg! through the host-pointer C ABI (sfh_fg): coefficients
+        # in, F and G out, with and without downloading the N-element composite
+        x1 = X[:, 0].copy()
+        Gh = np.empty(T_)
+        comp_h = np.empty(nbins, dtype=np.float32)
+        host = {}
+        for label, comp_arg in (("no_composite", None), ("with_composite_f32", comp_h)):
+            for _ in range(5):
+                sfh.fg_(True, Gh, x1, tpl, hd, comp_arg)
+            t0 = time.perf_counter()
+            for _ in range(nplain):
+                sfh.fg_(True, Gh, x1, tpl, hd, comp_arg)
+            host[label] = (time.perf_counter() - t0) / nplain * 1e3
+        extras["plain"]["host_abi_ms_per_eval"] = host
+        log(f"plain through the host C ABI: {host['no_composite']:.4f} ms/eval, {host['with_composite_f32']:.4f} ms "
+            f"with the composite downloaded")
         extras["batched_over_plain"] = value / extras["plain"]["evals_per_s"]
         log(f"plain: {pms:.4f} ms/eval ({1e3 / pms:.4g} evals/s); batched/plain = {extras['batched_over_plain']:.1f}x")
 

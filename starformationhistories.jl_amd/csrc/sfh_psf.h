@@ -395,6 +395,17 @@ __host__ __device__ __forceinline__ float col_fma(float du, float a)
 #endif
 }
 
+// part = in ? v : part, as a select: v is computed for every lane first (the
+// register barrier keeps the compiler from sinking its computation into a
+// branch per column, which serialises the columns' dependency chains)
+__host__ __device__ __forceinline__ void masked_set(float& part, float v, bool in)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    __asm__ volatile("" : "+v"(v));
+#endif
+    part = in ? v : part;
+}
+
 // f(integral_constant<int, C>) for C = I .. N-1, in order
 template <int I, int N, typename F>
 __host__ __device__ __forceinline__ void static_for(F&& f)
@@ -456,8 +467,8 @@ __host__ __device__ __forceinline__ void gauss_strip(const PointDesc& d, const f
             const float vf = fmaf(f, d.q[j], part[cf]);
             const float vb = fmaf(b, d.q[j], part[cb]);
             if (MASK) {
-                part[cf] = (cf >= lo && cf <= hi) ? vf : part[cf];
-                part[cb] = (cb >= lo && cb <= hi) ? vb : part[cb];
+                masked_set(part[cf], vf, cf >= lo && cf <= hi);
+                masked_set(part[cb], vb, cb >= lo && cb <= hi);
             } else {
                 part[cf] = vf;
                 part[cb] = vb;
@@ -504,7 +515,7 @@ __host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const flo
                 const float e1 = erfc_pos(fabsf(b1));
                 const float dd = neg ? e1 - prev : (pneg ? 2.0f - e1 - prev : prev - e1);
                 const float v = fmaf(W[k], dd, part[c]);
-                if (MASK) part[c] = (c >= lo && c <= hi) ? v : part[c];
+                if (MASK) masked_set(part[c], v, c >= lo && c <= hi);
                 else part[c] = v;
                 prev = e1;
                 pneg = neg;
@@ -537,7 +548,7 @@ __host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const flo
                 else S = fmaf(fmaf(fmaf(c3[k], w, c2[k]), w, c1[k]), w, c0[k]);
                 v = fmaf(S, fast_exp2(-w), v);
             }
-            if (MASK) part[c] = (c >= lo && c <= hi) ? v : part[c];
+            if (MASK) masked_set(part[c], v, c >= lo && c <= hi);
             else part[c] = v;
 #if defined(__HIP_DEVICE_COMPILE__)
             // columns in groups of four: enough independent work per wave, without
