@@ -60,7 +60,8 @@ __global__ __launch_bounds__(256) void k_dep_prep(
 
 // The FP64 accumulators live in LDS (16 per thread, 32 KiB per workgroup),
 // touched once per 64 points, so that the FP32 strip and the per-point
-// constants fit four waves per SIMD.
+// constants fit four waves per SIMD (a fifth, at 95 registers with the
+// series' coefficient products formed per column, measured slower).
 template <bool kAsym>
 __global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dep_gather(
     const PointDesc* __restrict__ desc, const int64_t* __restrict__ offsets, int nx, int ny,
@@ -69,7 +70,11 @@ __global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) 
 {
     __shared__ double sacc[kCPT][kDepBlock];
     const int64_t t = blockIdx.x / nblk_per_tpl;
-    const int blk = blockIdx.x - (int)(t * nblk_per_tpl);
+    // Row band of this workgroup, rotated by the template index: consecutive
+    // workgroups go to the 8 XCDs in turn, so without the rotation every
+    // template's band j would land on XCD j mod 8 and the populated bands of a
+    // diagram would load some XCDs far more than others.
+    const int blk = (int)((blockIdx.x - t * nblk_per_tpl + t) % nblk_per_tpl);
     const int64_t p0 = offsets[t], p1 = offsets[t + 1];
     const int slot = blk * kDepBlock + threadIdx.x;
     const int row = slot / ncg;
