@@ -632,6 +632,9 @@ def templates_bench(sfh, torch, ctx, O, args, ncpu):
     ops_per_pair = 142.0 / 16.0
     achieved = pairs_per_s * ops_per_pair
     roof = {"bound": "valu", "kernel": "k_dep_gather<false>", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
+            "peak_note": "non-packed FP32 issue rate; the published FP32 vector figure (157.3 TFLOP/s = 78.6 T "
+                         "FMA lane-ops/s) counts packed FP32, which this build does not use; against it the "
+                         f"fraction is {achieved / 78.6e12:.2f}",
             "unit": "T VALU lane-slots/s", "frac": achieved / VALU_PEAK, "traffic": None,
             "algorithmic": f"{ops_per_pair:g} VALU issue slots per (point, bin) pair (the Gaussian-form strip: "
                            f"3 GL nodes x (accumulate FMA + recurrence multiply) per bin, plus the strip's set-up "
