@@ -599,8 +599,9 @@ def templates_bench(sfh, torch, ctx, O, args, ncpu):
     T = len(specs)
     imf = synth.Kroupa2001(0.08, 120.0)
     t0 = time.time()
-    isos = [synth.isochrone(s.log_age, s.mh, 2000, s.seed, mmax_imf=120.0) for s in specs]
-    gen_s = time.time() - t0               # synthetic inputs, not part of the path
+    isos = sfh.IsochroneBatch.from_list([synth.isochrone(s.log_age, s.mh, 2000, s.seed, mmax_imf=120.0)
+                                         for s in specs])
+    gen_s = time.time() - t0               # synthetic inputs (packed host arrays), not part of the path
     kw = dict(dmod=24.0, normalize_value=1.0, mean_mass=imf.mean(), correlation=0.6, edges=edges)
     errs, comps = [synth.PHOTERR] * 2, [synth.COMPLETENESS] * 2
     out, (npts, pairs) = sfh.partial_cmd_smooth_batch(isos, errs, 1, (0, 1), imf, comps, return_info=True,
@@ -649,10 +650,11 @@ def templates_bench(sfh, torch, ctx, O, args, ncpu):
            "end_to_end_s": e2e_s, "templates_per_s_end_to_end": T / e2e_s,
            "kernel_ms": kms, "device_prep_ms": prep_ms, "deposit_ms": dep_ms,
            "templates_per_s_kernels": T / ((prep_ms + dep_ms) * 1e-3), "pairs_per_s": pairs_per_s,
-           "hbm_GBps_algorithmic": (nbins * T * 4 + npts * 80) / (dep_ms * 1e-3) / 1e9,
+           "hbm_GBps_algorithmic": (nbins * T * 4 + npts * 112) / (dep_ms * 1e-3) / 1e9,
            "host_numpy_prep_s": host_prep_s, "synthetic_isochrones_s": gen_s, "roofline": roof,
-           "note": "end-to-end = host isochrone arrays in, templates resident on the device out (upload, device "
-                   "preparation, deposition); the isochrones themselves are synthetic inputs generated beforehand"}
+           "note": "end-to-end = host isochrone arrays in (packed: one array per quantity with offsets, "
+                   "IsochroneBatch), templates resident on the device out (upload, device preparation, "
+                   "deposition); the isochrones themselves are synthetic inputs generated beforehand"}
     log(f"templates: {T} in {e2e_s * 1e3:.1f} ms end to end ({T / e2e_s:.4g}/s); kernels: prep {prep_ms:.2f} ms, "
         f"deposit {dep_ms:.2f} ms ({pairs_per_s:.3g} pairs/s, VALU frac {roof['frac']:.2f}); host NumPy prep "
         f"would take {host_prep_s:.2f} s")

@@ -33,7 +33,7 @@ from .templates import (Histogram, NoBinaries, PointBatch, RandomBinaryPairs, Ra
                         histogram_data, histogram_pix, interpolate_mini, Martin2016_complete,
                         midpoints, mini_spacing, partial_cmd_smooth, partial_cmd_smooth_points,
                         templates_from_device, partial_cmd_smooth_batch, ExpPhotErr, Martin2016Completeness,
-                        Constant, Tabulated)
+                        Constant, Tabulated, IsochroneBatch)
 
 __all__ = [
     "ArgumentError", "DeviceError", "InexactError", "Context", "HIPData", "HIPTemplates",
@@ -52,5 +52,5 @@ __all__ = [
     "generate_stars_mag_composite", "generate_stars_mass", "generate_stars_mass_composite", "ingest_mags",
     "mass_limits", "model_cmd", "recycling_fraction", "recycling_mass_fraction", "sort_ingested",
     "surviving_fraction", "surviving_mass_fraction", "partial_cmd_smooth_batch", "ExpPhotErr",
-    "Martin2016Completeness", "Constant", "Tabulated",
+    "Martin2016Completeness", "Constant", "Tabulated", "IsochroneBatch",
 ]

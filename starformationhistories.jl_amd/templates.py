@@ -702,6 +702,45 @@ def _fn_of(f, lo, hi, keep):
     return t.as_fn()
 
 
+class IsochroneBatch:
+    """Many isochrones in the packed form sfh_partial_cmd_smooth_batch takes:
+    isochrone t owns entries offsets[t] .. offsets[t+1]-1 of m_ini and of each
+    filter's magnitude array.  Building it once lets repeated batch calls skip
+    the per-call concatenation of per-isochrone arrays."""
+
+    def __init__(self, m_ini, mags, offsets):
+        self.m_ini = np.ascontiguousarray(m_ini, dtype=np.float64).ravel()
+        self.mags = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in mags]
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64).ravel()
+        n = self.m_ini.size
+        if self.offsets.size < 2 or self.offsets[0] != 0 or self.offsets[-1] != n or np.any(np.diff(self.offsets) < 0):
+            raise ArgumentError("offsets must run from 0 to length(m_ini), non-decreasing")
+        if any(a.size != n for a in self.mags):
+            raise ArgumentError("axes(m_ini) == axes(mags[i])")
+
+    @classmethod
+    def from_list(cls, isochrones):
+        """From a sequence of (m_ini, mags) pairs."""
+        m_cat = [np.asarray(m, dtype=np.float64).ravel() for m, _ in isochrones]
+        counts = np.array([m.size for m in m_cat], dtype=np.int64)
+        nf = len(isochrones[0][1]) if len(isochrones) else 0
+        mags = []
+        for f in range(nf):
+            arrs = [np.asarray(mg[f], dtype=np.float64).ravel() for _, mg in isochrones]
+            if any(a.size != n for a, n in zip(arrs, counts)):
+                raise ArgumentError("axes(m_ini) == axes(mags[i])")
+            mags.append(np.concatenate(arrs))
+        return cls(np.concatenate(m_cat) if m_cat else np.zeros(0), mags,
+                   np.concatenate([[0], np.cumsum(counts)]))
+
+    def __len__(self):
+        return self.offsets.size - 1
+
+    @property
+    def nfilters(self):
+        return len(self.mags)
+
+
 def partial_cmd_smooth_batch(isochrones, mag_err_funcs, y_index, color_indices, imf, completeness_funcs=None,
                              bias_funcs=None, *, dmod=0.0, normalize_value=1.0, mean_mass=None, correlation=0.0,
                              edges=None, xlim=None, ylim=None, nbins=None, xwidth=None, ywidth=None,
@@ -710,7 +749,8 @@ def partial_cmd_smooth_batch(isochrones, mag_err_funcs, y_index, color_indices, 
     for many isochrones in one call (sfh_partial_cmd_smooth_batch): the host
     preparation of every isochrone runs on the device, then one deposition
     launch.  ``isochrones``: a sequence of (m_ini, mags) with m_ini strictly
-    increasing and mags a sequence of per-filter magnitude arrays.  The
+    increasing and mags a sequence of per-filter magnitude arrays, or an
+    IsochroneBatch of the same in packed form.  The
     callables are shared by all isochrones; ``dmod``, ``normalize_value`` and
     ``mean_mass`` may be scalars or one value per isochrone.  Callables with an
     ``as_fn`` form (ExpPhotErr, Martin2016Completeness, Constant, Tabulated,
@@ -723,12 +763,13 @@ def partial_cmd_smooth_batch(isochrones, mag_err_funcs, y_index, color_indices, 
     (kernel points, pairs)."""
     import torch
     ctx = ctx or default_context()
-    T = len(isochrones)
+    batch = isochrones if isinstance(isochrones, IsochroneBatch) else IsochroneBatch.from_list(isochrones)
+    T = len(batch)
     if T == 0:
         raise ArgumentError("need at least one isochrone")
     if len(color_indices) != 2:
         raise ArgumentError("length(color_indices) == 2")
-    nf = len(isochrones[0][1])
+    nf = batch.nfilters
     if completeness_funcs is None:
         completeness_funcs = [None] * nf
     if bias_funcs is None:
@@ -740,16 +781,7 @@ def partial_cmd_smooth_batch(isochrones, mag_err_funcs, y_index, color_indices, 
         if not 0 <= i < nf:
             raise ArgumentError("y_index / color_indices out of range")
     ex, ey = calculate_edges(edges, xlim, ylim, nbins, xwidth, ywidth)
-    m_cat = [np.asarray(m, dtype=np.float64).ravel() for m, _ in isochrones]
-    counts = np.array([m.size for m in m_cat], dtype=np.int64)
-    offsets = np.ascontiguousarray(np.concatenate([[0], np.cumsum(counts)]), dtype=np.int64)
-    m_all = np.ascontiguousarray(np.concatenate(m_cat))
-    mags_all = []
-    for f in range(nf):
-        arrs = [np.asarray(mg[f], dtype=np.float64).ravel() for _, mg in isochrones]
-        if any(a.size != n for a, n in zip(arrs, counts)):
-            raise ArgumentError("axes(m_ini) == axes(mags[i])")
-        mags_all.append(np.ascontiguousarray(np.concatenate(arrs)))
+    offsets, m_all, mags_all = batch.offsets, batch.m_ini, batch.mags
     per = lambda v: np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (T,)))  # noqa: E731
     dm = per(dmod)
     if mean_mass is None:

@@ -101,6 +101,10 @@ def test_device_prep_outputs_and_tabulated_callables(sfh, gpu_ctx):
     h = sfh.partial_cmd_smooth_batch(isos, [synth.PHOTERR] * 2, 1, (0, 1), imf, [synth.COMPLETENESS] * 2,
                                      output="host", **kw)
     assert np.array_equal(d.cpu().numpy().T, h)
+    # the packed input form gives the same templates bit for bit
+    packed = sfh.partial_cmd_smooth_batch(sfh.IsochroneBatch.from_list(isos), [synth.PHOTERR] * 2, 1, (0, 1), imf,
+                                          [synth.COMPLETENESS] * 2, output="host", **kw)
+    assert np.array_equal(packed, h)
     f = lambda v: 0.01 + np.exp(0.6 * (np.asarray(v) - 28.0))           # noqa: E731
     c = lambda v: 1.0 / (1.0 + np.exp((np.asarray(v) - 28.0) / 0.7))    # noqa: E731
     tab = sfh.partial_cmd_smooth_batch(isos, [f] * 2, 1, (0, 1), lambda m: imf(m), [c] * 2, output="host", **kw)

@@ -146,3 +146,21 @@ def test_errors_without_gpu_are_loud(sfh):
         pytest.skip("GPU present")
     with pytest.raises(Exception):
         sfh.Context(0)
+
+
+def test_isochrone_batch_packing(sfh):
+    """The packed isochrone form: concatenation order, offsets and the
+    argument checks (no GPU needed)."""
+    a = (np.array([0.1, 0.2, 0.3]), [np.array([1.0, 2.0, 3.0]), np.array([4.0, 5.0, 6.0])])
+    b = (np.array([0.5, 0.6]), [np.array([7.0, 8.0]), np.array([9.0, 10.0])])
+    ib = sfh.IsochroneBatch.from_list([a, b])
+    assert len(ib) == 2 and ib.nfilters == 2
+    assert list(ib.offsets) == [0, 3, 5]
+    assert list(ib.m_ini) == [0.1, 0.2, 0.3, 0.5, 0.6]
+    assert list(ib.mags[1]) == [4.0, 5.0, 6.0, 9.0, 10.0]
+    with pytest.raises(sfh.ArgumentError):
+        sfh.IsochroneBatch(np.zeros(3), [np.zeros(3)], [0, 2])          # offsets must end at length(m_ini)
+    with pytest.raises(sfh.ArgumentError):
+        sfh.IsochroneBatch(np.zeros(3), [np.zeros(2)], [0, 3])          # filter length
+    with pytest.raises(sfh.ArgumentError):
+        sfh.IsochroneBatch(np.zeros(3), [np.zeros(3)], [0, 2, 1, 3])    # decreasing offsets

@@ -53,3 +53,4 @@ def test_lattice_includes_ends(sfh):
     assert list(_lattice(11, 5)) == [0, 5, 10]
     assert list(_lattice(12, 5)) == [0, 5, 10, 11]
     assert list(_lattice(1, 5)) == [0]
+
