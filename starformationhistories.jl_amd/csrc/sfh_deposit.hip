@@ -95,10 +95,10 @@ __global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) 
     // Each chunk of 64 points: every lane reads one point's footprint (the
     // descriptor's first 16 bytes) and form, a ballot keeps the points whose
     // rows meet this wave's strips, and only those descriptors are loaded --
-    // through scalar loads (uniform across the wave).  The hits are taken in
-    // two passes: first the common case (the Gaussian form with a footprint
-    // spanning the whole grid width), two descriptors per wait, then every
-    // other point.  Each pass has one code path, so the strip's registers
+    // through scalar loads (uniform across the wave), two descriptors per
+    // wait.  The hits are taken in two passes: first the common case (the
+    // Gaussian form with a footprint spanning the whole grid width), then
+    // every other point.  Each pass has one code path, so the strip's registers
     // carry through it without copies; a bin sums the chunk's common-case
     // points, then the others, each in point order (deterministic).
     const PointDesc* __restrict__ td = desc + p0;
@@ -167,7 +167,13 @@ __global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) 
         while (hits) {
             const int i = __builtin_ctzll(hits);
             hits &= hits - 1;
-            add(td[pc + i]);
+            const bool two = hits != 0;
+            const int j = two ? __builtin_ctzll(hits) : i;
+            if (two) hits &= hits - 1;
+            const PointDesc d0 = td[pc + i];
+            const PointDesc d1 = td[pc + j];
+            add(d0);
+            if (two) add(d1);
         }
 #pragma unroll
         for (int c = 0; c < kCPT; ++c) {
