@@ -618,11 +618,11 @@ def templates_bench(sfh, torch, ctx, O, args, ncpu):
     for _ in range(nrep):
         out = sfh.partial_cmd_smooth_batch(isos, errs, 1, (0, 1), imf, comps, ctx=ctx, **kw)
     ctx.synchronize()
-    kern = {k: ctx.kernel_stats(k) for k in ("pcs_count", "pcs_points", "pcs_desc", "deposit_cov")}
+    kern = {k: ctx.kernel_stats(k) for k in ("pcs_count", "pcs_points", "pcs_desc", "deposit_order", "deposit_cov")}
     ctx.set_timing(False)
     kms = {k: v[0] / max(v[1], 1) for k, v in kern.items()}
     dep_ms = kms["deposit_cov"]
-    prep_ms = kms["pcs_count"] + kms["pcs_points"] + kms["pcs_desc"]
+    prep_ms = kms["pcs_count"] + kms["pcs_points"] + kms["pcs_desc"] + kms["deposit_order"]
     pairs_per_s = pairs / (dep_ms * 1e-3)
     # VALU issue slots per pair of the gather's dominant path (the Gaussian
     # form over a full-width row strip of 16 bins, 94% of the pairs): per GL

@@ -62,19 +62,30 @@ __global__ __launch_bounds__(256) void k_dep_prep(
 // touched once per 64 points, so that the FP32 strip and the per-point
 // constants fit four waves per SIMD (a fifth, at 95 registers with the
 // series' coefficient products formed per column, measured slower).
+// Workgroup w of the gather takes (template, row band) order[w] = t * nblk + band
+// when an order is given: the (template, band) pairs by decreasing cost, so
+// the longest workgroups start first and the launch ends with short ones.
+// Without one, template t's band j is (j + t) mod nblk: consecutive workgroups
+// go to the 8 XCDs in turn, so without the rotation every template's band j
+// would land on XCD j mod 8 and the populated bands of a diagram would load
+// some XCDs far more than others.
 template <bool kAsym>
 __global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dep_gather(
-    const PointDesc* __restrict__ desc, const int64_t* __restrict__ offsets, int nx, int ny,
-    int ncg, int nblk_per_tpl, float* __restrict__ out32, double* __restrict__ out64,
+    const PointDesc* __restrict__ desc, const int64_t* __restrict__ offsets, const int32_t* __restrict__ order,
+    int nx, int ny, int ncg, int nblk_per_tpl, float* __restrict__ out32, double* __restrict__ out64,
     int64_t ld_out)
 {
     __shared__ double sacc[kCPT][kDepBlock];
-    const int64_t t = blockIdx.x / nblk_per_tpl;
-    // Row band of this workgroup, rotated by the template index: consecutive
-    // workgroups go to the 8 XCDs in turn, so without the rotation every
-    // template's band j would land on XCD j mod 8 and the populated bands of a
-    // diagram would load some XCDs far more than others.
-    const int blk = (int)((blockIdx.x - t * nblk_per_tpl + t) % nblk_per_tpl);
+    int64_t t;
+    int blk;
+    if (order) {
+        const int id = order[blockIdx.x];
+        t = id / nblk_per_tpl;
+        blk = id - (int)t * nblk_per_tpl;
+    } else {
+        t = blockIdx.x / nblk_per_tpl;
+        blk = (int)((blockIdx.x - t * nblk_per_tpl + t) % nblk_per_tpl);
+    }
     const int64_t p0 = offsets[t], p1 = offsets[t + 1];
     const int slot = blk * kDepBlock + threadIdx.x;
     const int row = slot / ncg;
@@ -194,6 +205,135 @@ __global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) 
     }
 }
 
+// Relative cost per row strip of each x form (issue slots of the form's strip
+// code over the Gaussian form's).
+__device__ inline unsigned form_cost(int kind, bool full)
+{
+    if (kind == psf::kGauss) return full ? 4u : 6u;
+    if (kind == psf::kSeries2 || kind == psf::kSeries3) return 12u;
+    return 32u;
+}
+
+// Estimated cost of every (template, row band) workgroup of the gather: per
+// point, the rows x strips of its footprint inside each band it meets, times
+// its form's cost.  Integer atomics: the sums do not depend on the order.
+__global__ __launch_bounds__(256) void k_dep_cost(const PointDesc* __restrict__ desc,
+                                                  const int64_t* __restrict__ offsets, int64_t ntemplates,
+                                                  int64_t npoints, int nx, int ncg, int nblk,
+                                                  unsigned long long* __restrict__ cost)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npoints) return;
+    const int4 f = *reinterpret_cast<const int4*>(desc + p);
+    if (f.x > f.y) return;
+    int64_t lo = 0, hi = ntemplates - 1;    // template t: offsets[t] <= p < offsets[t + 1]
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) / 2;
+        if (offsets[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    const bool full = f.x == 0 && f.y == nx - 1;
+    const unsigned w = form_cost(desc[p].kind, full);
+    const int nstrips = f.y / kCPT - f.x / kCPT + 1;
+    const int b0 = (int)(((int64_t)f.z * ncg) / kDepBlock);
+    int b1 = (int)(((int64_t)f.w * ncg + ncg - 1) / kDepBlock);
+    if (b1 > nblk - 1) b1 = nblk - 1;
+    for (int b = b0; b <= b1; ++b) {
+        const int rlo = (int)(((int64_t)b * kDepBlock) / ncg), rhi = (int)(((int64_t)b * kDepBlock + kDepBlock - 1) / ncg);
+        const int r0 = f.z > rlo ? f.z : rlo, r1 = f.w < rhi ? f.w : rhi;
+        if (r1 >= r0)
+            atomicAdd(&cost[lo * nblk + b], (unsigned long long)(r1 - r0 + 1) * nstrips * w);
+    }
+}
+
+constexpr int kOrderMax = 16384;   // workgroups one ordering launch sorts (its LDS: 64 KiB)
+
+// order[] = the n workgroup ids by decreasing cost (ties: increasing id), by a
+// bitonic sort of 32-bit keys in one workgroup's LDS: the cost scaled to 18
+// bits in the high bits, 2^14 - 1 - id in the low 14 (n <= 2^14), so that a
+// descending sort orders ties by id; padding keys are 0 and sort last.
+__global__ __launch_bounds__(1024) void k_dep_order(const unsigned long long* __restrict__ cost, int n,
+                                                    int32_t* __restrict__ order)
+{
+    extern __shared__ uint32_t keys[];
+    __shared__ unsigned long long smax[16];
+    int m = 1;
+    while (m < n) m <<= 1;
+    unsigned long long mx = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) mx = cost[i] > mx ? cost[i] : mx;
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) mx = smax[w] > mx ? smax[w] : mx;
+    int shift = 0;
+    while ((mx >> shift) >= (1ull << 18)) ++shift;
+    for (int i = threadIdx.x; i < m; i += blockDim.x)
+        keys[i] = i < n ? ((uint32_t)(cost[i] >> shift) << 14) | (uint32_t)(0x3FFF - i) : 0u;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t a = keys[i], b = keys[l];
+                    // the block of i descends when (i & k) == 0: all of it at k = m
+                    if ((i & k) == 0 ? (a < b) : (a > b)) {
+                        keys[i] = b;
+                        keys[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) order[i] = 0x3FFF - (int32_t)(keys[i] & 0x3FFF);
+}
+
+// The gather over descriptors of npoints points (template t owns
+// desc[offsets[t] .. offsets[t+1])), with the workgroups in decreasing
+// estimated cost when the launch is small enough to order in one pass.
+static int launch_gather(sfh_ctx* ctx, const PointDesc* desc, const int64_t* d_offsets, int64_t ntemplates,
+                         int64_t npoints, bool asym, int nx, int ny, float* out32, double* out64, int64_t ld_out)
+{
+    const int ncg = (int)cdiv(nx, kCPT);
+    const int nblk = (int)cdiv((int64_t)ncg * ny, kDepBlock);
+    const int64_t grid = (int64_t)nblk * ntemplates;
+    SFH_ARG_CHECK(ctx, grid < (int64_t)1 << 31, "too many templates for one launch");
+    if (grid == 0) return SFH_OK;
+    const int32_t* order = nullptr;
+    if (ntemplates > 1 && grid <= kOrderMax && npoints > 0) {
+        void *pc = nullptr, *po = nullptr;
+        if (int rc = scratch(ctx, "dep_cost", (size_t)grid * sizeof(unsigned long long), &pc)) return rc;
+        if (int rc = scratch(ctx, "dep_order", (size_t)grid * sizeof(int32_t), &po)) return rc;
+        SFH_HIP_TRY(ctx, hipMemsetAsync(pc, 0, (size_t)grid * sizeof(unsigned long long), ctx->stream));
+        {
+            TimedLaunch tl(ctx, "deposit_order");
+            hipLaunchKernelGGL(k_dep_cost, dim3((unsigned)cdiv(npoints, 256)), dim3(256), 0, ctx->stream, desc,
+                               d_offsets, ntemplates, npoints, nx, ncg, nblk, (unsigned long long*)pc);
+            int m = 1;
+            while (m < grid) m <<= 1;
+            hipLaunchKernelGGL(k_dep_order, dim3(1), dim3(1024), (size_t)m * sizeof(uint32_t),
+                               ctx->stream, (const unsigned long long*)pc, (int)grid, (int32_t*)po);
+        }
+        if (int rc = check_launch(ctx, "k_dep_order")) return rc;
+        order = (const int32_t*)po;
+    }
+    if (asym) {
+        TimedLaunch tl(ctx, "deposit_asym");
+        hipLaunchKernelGGL(k_dep_gather<true>, dim3((unsigned)grid), dim3(kDepBlock), 0, ctx->stream, desc, d_offsets,
+                           order, nx, ny, ncg, nblk, out32, out64, ld_out);
+    } else {
+        TimedLaunch tl(ctx, "deposit_cov");
+        hipLaunchKernelGGL(k_dep_gather<false>, dim3((unsigned)grid), dim3(kDepBlock), 0, ctx->stream, desc,
+                           d_offsets, order, nx, ny, ncg, nblk, out32, out64, ld_out);
+    }
+    return check_launch(ctx, "k_dep_gather");
+}
+
 static int validate_ranges(sfh_ctx* ctx, sfh_range xe, sfh_range ye)
 {
     SFH_ARG_CHECK(ctx, xe.length >= 2 && ye.length >= 2, "edge ranges need at least 2 edges");
@@ -230,22 +370,9 @@ static int deposit(sfh_ctx* ctx, const double* colors, const double* mags, const
                            asym ? 1 : 0, (PointDesc*)pdesc, d_pairs, d_bad);
     }
     if (int rc = check_launch(ctx, "k_dep_prep")) return rc;
-    const int ncg = (int)cdiv(nx, kCPT);
-    const int nblk = (int)cdiv((int64_t)ncg * ny, kDepBlock);
-    const int64_t grid = (int64_t)nblk * ntemplates;
-    SFH_ARG_CHECK(ctx, grid < (int64_t)1 << 31, "too many templates for one launch");
-    if (grid > 0) {
-        if (asym) {
-            TimedLaunch tl(ctx, "deposit_asym");
-            hipLaunchKernelGGL(k_dep_gather<true>, dim3((unsigned)grid), dim3(kDepBlock), 0, ctx->stream,
-                               (const PointDesc*)pdesc, d_offsets, nx, ny, ncg, nblk, out32, out64, ld_out);
-        } else {
-            TimedLaunch tl(ctx, "deposit_cov");
-            hipLaunchKernelGGL(k_dep_gather<false>, dim3((unsigned)grid), dim3(kDepBlock), 0, ctx->stream,
-                               (const PointDesc*)pdesc, d_offsets, nx, ny, ncg, nblk, out32, out64, ld_out);
-        }
-    }
-    if (int rc = check_launch(ctx, "k_dep_gather")) return rc;
+    if (int rc = launch_gather(ctx, (const PointDesc*)pdesc, d_offsets, ntemplates, npoints_total, asym, nx, ny,
+                               out32, out64, ld_out))
+        return rc;
     // Non-finite inputs: the reference's round(Int, NaN) throws InexactError.
     int bad = 0;
     unsigned long long pairs = 0;
@@ -259,26 +386,14 @@ static int deposit(sfh_ctx* ctx, const double* colors, const double* mags, const
 
 // The gather of descriptors already built (by the device preparation of
 // partial_cmd_smooth): template t owns desc[koff[t] .. koff[t+1]).
-int deposit_from_desc(sfh_ctx* ctx, const void* desc, const int64_t* d_koff, int64_t ntemplates, bool asym,
-                      sfh_range xe, sfh_range ye, float* out32, int64_t ld_out)
+int deposit_from_desc(sfh_ctx* ctx, const void* desc, const int64_t* d_koff, int64_t ntemplates, int64_t npoints,
+                      bool asym, sfh_range xe, sfh_range ye, float* out32, int64_t ld_out)
 {
     if (int rc = validate_ranges(ctx, xe, ye)) return rc;
     const int nx = (int)(xe.length - 1), ny = (int)(ye.length - 1);
     SFH_ARG_CHECK(ctx, ld_out >= (int64_t)nx * ny, "ld_out < nx*ny");
-    const int ncg = (int)cdiv(nx, kCPT);
-    const int nblk = (int)cdiv((int64_t)ncg * ny, kDepBlock);
-    const int64_t grid = (int64_t)nblk * ntemplates;
-    SFH_ARG_CHECK(ctx, grid < (int64_t)1 << 31, "too many templates for one launch");
-    if (asym) {
-        TimedLaunch tl(ctx, "deposit_asym");
-        hipLaunchKernelGGL(k_dep_gather<true>, dim3((unsigned)grid), dim3(kDepBlock), 0, ctx->stream,
-                           (const PointDesc*)desc, d_koff, nx, ny, ncg, nblk, out32, (double*)nullptr, ld_out);
-    } else {
-        TimedLaunch tl(ctx, "deposit_cov");
-        hipLaunchKernelGGL(k_dep_gather<false>, dim3((unsigned)grid), dim3(kDepBlock), 0, ctx->stream,
-                           (const PointDesc*)desc, d_koff, nx, ny, ncg, nblk, out32, (double*)nullptr, ld_out);
-    }
-    return check_launch(ctx, "k_dep_gather");
+    return launch_gather(ctx, (const PointDesc*)desc, d_koff, ntemplates, npoints, asym, nx, ny, out32, nullptr,
+                         ld_out);
 }
 
 }  // namespace sfh

@@ -138,8 +138,8 @@ int noalloc_init(int device);
 // Deposition gather over descriptors already built on the device (template t
 // owns desc[koff[t] .. koff[t+1]), koff on the device); FP32 out, column t at
 // out32 + t*ld_out.
-int deposit_from_desc(sfh_ctx* ctx, const void* desc, const int64_t* d_koff, int64_t ntemplates, bool asym,
-                      sfh_range xe, sfh_range ye, float* out32, int64_t ld_out);
+int deposit_from_desc(sfh_ctx* ctx, const void* desc, const int64_t* d_koff, int64_t ntemplates, int64_t npoints,
+                      bool asym, sfh_range xe, sfh_range ye, float* out32, int64_t ld_out);
 
 // Launchers implemented per translation unit.
 int launch_composite(sfh_ctx* ctx, const sfh_templates* tpl, const double* d_coeffs,

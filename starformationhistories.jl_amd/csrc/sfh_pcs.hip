@@ -562,8 +562,8 @@ int sfh_partial_cmd_smooth_batch(sfh_ctx* ctx, int64_t ntemplates, const int64_t
         if (int rc = scratch(ctx, "pcs_out", (size_t)nbins * ntemplates * sizeof(float), &p)) return rc;
         dst = (float*)p;
     }
-    if (int rc = deposit_from_desc(ctx, ddesc, (const int64_t*)dk, ntemplates, a.cov_mult == 0.0, xedges, yedges,
-                                   dst, ld))
+    if (int rc = deposit_from_desc(ctx, ddesc, (const int64_t*)dk, ntemplates, K, a.cov_mult == 0.0, xedges,
+                                   yedges, dst, ld))
         return rc;
     unsigned long long pairs = 0;
     SFH_HIP_TRY(ctx, hipMemcpyAsync(&bad, a.bad, sizeof(int), hipMemcpyDeviceToHost, s));
