@@ -115,6 +115,7 @@ __global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) 
     const PointDesc* __restrict__ td = desc + p0;
     const int np = (int)(p1 - p0);
     const int lane = threadIdx.x & 63;
+    const int wave_id = (int)(threadIdx.x >> 6);
     struct Foot {
         int4 f;
         int kind;
@@ -156,6 +157,40 @@ __global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) 
             psf::add_point<3, true>(d, W, h, col0, part);
         }
     };
+    // A point in the erf-difference form (a narrow kernel: its footprint is at
+    // most 55 columns wide) with the whole wave cooperating, instead of the
+    // strip code in which only the few lanes whose strips meet the footprint
+    // would work: lane l takes footprint column xa + l, the point's rows inside
+    // this wave's strips one after another; lanes 0 .. nc evaluate erfc at
+    // their column's left edge and a column's right edge is the next lane's.
+    // Each bin's value (nodes summed in order, in FP32) is added to its
+    // owner's FP64 accumulator in LDS: the bins are distinct, so no two lanes
+    // write one word, and the owner's own flush comes later in program order.
+    auto direct = [&](const PointDesc& d) {
+        constexpr int NODES = kAsym ? 1 : 3;
+        const int nc = d.xb - d.xa + 1;
+        const int r0 = d.ya > row_lo ? d.ya : row_lo, r1 = d.yb < row_hi ? d.yb : row_hi;
+        for (int r = r0; r <= r1; ++r) {
+            float W[3], h[3] = {0.0f, 0.0f, 0.0f};
+            if constexpr (kAsym) W[0] = d.pref * psf::y_factor(fmaf((float)(r - d.ya), d.dv, d.ay), d.ey);
+            else psf::cov_row(d, r, W, h);
+            float v = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NODES; ++k) {
+                const float b = psf::direct_edge(d, lane, h[k]);
+                const float e = psf::erfc_pos(fabsf(b));
+                const int n = b < 0.0f ? 1 : 0;
+                const float e1 = __shfl_down(e, 1, 64);
+                const int n1 = __shfl_down(n, 1, 64);
+                v = fmaf(W[k], psf::erf_diff(e, n != 0, e1, n1 != 0), v);
+            }
+            if (lane < nc) {
+                const int col = d.xa + lane;
+                const int o = r * ncg + col / kCPT - wslot;
+                if (o >= 0 && o < 64) sacc[col % kCPT][wave_id * 64 + o] += (double)v;
+            }
+        }
+    };
     Foot fn = footprint(0);
     for (int pc = 0; pc < np; pc += kChunk) {
         const Foot f = fn;
@@ -183,8 +218,13 @@ __global__ __launch_bounds__(kDepBlock) __attribute__((amdgpu_waves_per_eu(4))) 
             if (two) hits &= hits - 1;
             const PointDesc d0 = td[pc + i];
             const PointDesc d1 = td[pc + j];
-            add(d0);
-            if (two) add(d1);
+            // the narrow erf-difference kernels go to the wave-cooperative path
+            if (d0.kind == psf::kDirect && d0.xb - d0.xa < 63) direct(d0);
+            else add(d0);
+            if (two) {
+                if (d1.kind == psf::kDirect && d1.xb - d1.xa < 63) direct(d1);
+                else add(d1);
+            }
         }
 #pragma unroll
         for (int c = 0; c < kCPT; ++c) {

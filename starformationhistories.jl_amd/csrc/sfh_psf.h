@@ -330,6 +330,13 @@ __host__ __device__ inline float erfc_pos(float x)
     return t * fast_exp(fmaf(-x, x, p));
 }
 
+// erf(b1) - erf(b0) of a column from e = erfc(|b|) and the sign of its two
+// edges: both >= 0: e0 - e1; both < 0: e1 - e0; across 0: 2 - e1 - e0
+__host__ __device__ inline float erf_diff(float e0, bool n0, float e1, bool n1)
+{
+    return n1 ? e1 - e0 : (n0 ? 2.0f - e1 - e0 : e0 - e1);
+}
+
 // erf(u + e) - erf(u - e) through erfc of non-negative arguments (D is even
 // in u), so that it keeps its relative accuracy in the tails
 __host__ __device__ inline float d_direct(float u, float e)
@@ -513,7 +520,7 @@ __host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const flo
                 const float b1 = col_fma<c + 1>(du, b0);
                 const bool neg = b1 < 0.0f;
                 const float e1 = erfc_pos(fabsf(b1));
-                const float dd = neg ? e1 - prev : (pneg ? 2.0f - e1 - prev : prev - e1);
+                const float dd = erf_diff(prev, pneg, e1, neg);
                 const float v = fmaf(W[k], dd, part[c]);
                 if (MASK) masked_set(part[c], v, c >= lo && c <= hi);
                 else part[c] = v;
@@ -557,6 +564,14 @@ __host__ __device__ __forceinline__ void add_strip(const PointDesc& d, const flo
 #endif
         });
     }
+}
+
+// Left edge of footprint column l of a narrow kernel in the erf-difference
+// form, for GL node shift h: x0 + l du + h, x0 = ax - ex (the wave-cooperative
+// path: one column per lane, the right edge being the next column's left).
+__host__ __device__ inline float direct_edge(const PointDesc& d, int l, float h)
+{
+    return fmaf((float)l, d.du, d.ax - d.ex) + h;
 }
 
 // One point's contributions to one row's strip, dispatched on its form.

@@ -3,7 +3,7 @@
 // k_dep_gather: FP32 per-point contributions summed per 64-point chunk -- the
 // chunk's common-case points (Gaussian form, footprint spanning the grid
 // width) first, then the others, each in point order -- then flushed into
-// FP64.  Lets the CPU test-suite check the FP32 formulation and the footprint
+// FP64; narrow erf-difference kernels add each bin straight into FP64.  Lets the CPU test-suite check the FP32 formulation and the footprint
 // index math against the FP64 oracle without a GPU.
 #include "../../starformationhistories.jl_amd/csrc/sfh_psf.h"
 
@@ -38,11 +38,27 @@ extern "C" int64_t emu_bin_cmd_smooth(const double* colors, const double* mags, 
                 if (d.xa > d.xb) continue;
                 const bool common = d.kind == kGauss && d.xa == 0 && d.xb == nx - 1;
                 if (common != (pass == 0)) continue;
+                // narrow erf-difference kernels: the kernel's wave-cooperative
+                // path, one column at a time, each bin straight into FP64
+                const bool coop = d.kind == kDirect && d.xb - d.xa < 63;
                 for (int row = d.ya; row <= d.yb; ++row) {
                     float W[3], L[3] = {0.0f, 0.0f, 0.0f}, h[3] = {0.0f, 0.0f, 0.0f};
                     if (asym) W[0] = d.pref * y_factor(fmaf((float)(row - d.ya), d.dv, d.ay), d.ey);
                     else if (common) cov_row_log(d, row, W, L, h);
                     else cov_row(d, row, W, h);
+                    if (coop) {
+                        const int nodes = asym ? 1 : 3;
+                        for (int l = 0; l <= d.xb - d.xa; ++l) {
+                            float v = 0.0f;
+                            for (int k = 0; k < nodes; ++k) {
+                                const float b0 = direct_edge(d, l, h[k]), b1 = direct_edge(d, l + 1, h[k]);
+                                v = fmaf(W[k], erf_diff(erfc_pos(fabsf(b0)), b0 < 0.0f, erfc_pos(fabsf(b1)), b1 < 0.0f),
+                                         v);
+                            }
+                            acc[(size_t)row * ld + d.xa + l] += (double)v;
+                        }
+                        continue;
+                    }
                     for (int cg = 0; cg < ncg; ++cg) {
                         const int col0 = cg * kCPT;
                         float* pp = &part[(size_t)row * ld + col0];
