@@ -256,81 +256,80 @@ __device__ inline unsigned form_cost(int kind, bool full)
 
 // Estimated cost of every (template, row band) workgroup of the gather: per
 // point, the rows x strips of its footprint inside each band it meets, times
-// its form's cost.  Integer atomics: the sums do not depend on the order.
+// its form's cost.  One workgroup per template, the band sums in LDS (integer
+// atomics: the sums do not depend on the order); nblk * 8 bytes of dynamic LDS.
 __global__ __launch_bounds__(256) void k_dep_cost(const PointDesc* __restrict__ desc,
-                                                  const int64_t* __restrict__ offsets, int64_t ntemplates,
-                                                  int64_t npoints, int nx, int ncg, int nblk,
+                                                  const int64_t* __restrict__ offsets, int nx, int ncg, int nblk,
                                                   unsigned long long* __restrict__ cost)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npoints) return;
-    const int4 f = *reinterpret_cast<const int4*>(desc + p);
-    if (f.x > f.y) return;
-    int64_t lo = 0, hi = ntemplates - 1;    // template t: offsets[t] <= p < offsets[t + 1]
-    while (lo < hi) {
-        const int64_t mid = (lo + hi + 1) / 2;
-        if (offsets[mid] <= p) lo = mid;
-        else hi = mid - 1;
+    extern __shared__ unsigned long long band[];
+    const int64_t t = blockIdx.x;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) band[b] = 0;
+    __syncthreads();
+    for (int64_t p = offsets[t] + threadIdx.x; p < offsets[t + 1]; p += blockDim.x) {
+        const int4 f = *reinterpret_cast<const int4*>(desc + p);
+        if (f.x > f.y) continue;
+        const bool full = f.x == 0 && f.y == nx - 1;
+        const unsigned w = form_cost(desc[p].kind, full);
+        const int nstrips = f.y / kCPT - f.x / kCPT + 1;
+        const int b0 = (int)(((int64_t)f.z * ncg) / kDepBlock);
+        int b1 = (int)(((int64_t)f.w * ncg + ncg - 1) / kDepBlock);
+        if (b1 > nblk - 1) b1 = nblk - 1;
+        for (int b = b0; b <= b1; ++b) {
+            const int rlo = (int)(((int64_t)b * kDepBlock) / ncg);
+            const int rhi = (int)(((int64_t)b * kDepBlock + kDepBlock - 1) / ncg);
+            const int r0 = f.z > rlo ? f.z : rlo, r1 = f.w < rhi ? f.w : rhi;
+            if (r1 >= r0) atomicAdd(&band[b], (unsigned long long)(r1 - r0 + 1) * nstrips * w);
+        }
     }
-    const bool full = f.x == 0 && f.y == nx - 1;
-    const unsigned w = form_cost(desc[p].kind, full);
-    const int nstrips = f.y / kCPT - f.x / kCPT + 1;
-    const int b0 = (int)(((int64_t)f.z * ncg) / kDepBlock);
-    int b1 = (int)(((int64_t)f.w * ncg + ncg - 1) / kDepBlock);
-    if (b1 > nblk - 1) b1 = nblk - 1;
-    for (int b = b0; b <= b1; ++b) {
-        const int rlo = (int)(((int64_t)b * kDepBlock) / ncg), rhi = (int)(((int64_t)b * kDepBlock + kDepBlock - 1) / ncg);
-        const int r0 = f.z > rlo ? f.z : rlo, r1 = f.w < rhi ? f.w : rhi;
-        if (r1 >= r0)
-            atomicAdd(&cost[lo * nblk + b], (unsigned long long)(r1 - r0 + 1) * nstrips * w);
-    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) cost[t * nblk + b] = band[b];
 }
 
-constexpr int kOrderMax = 16384;   // workgroups one ordering launch sorts (its LDS: 64 KiB)
+constexpr int kOrderMax = 16384;    // workgroups one ordering launch takes
+constexpr int kOrderBuckets = 1024;
 
-// order[] = the n workgroup ids by decreasing cost (ties: increasing id), by a
-// bitonic sort of 32-bit keys in one workgroup's LDS: the cost scaled to 18
-// bits in the high bits, 2^14 - 1 - id in the low 14 (n <= 2^14), so that a
-// descending sort orders ties by id; padding keys are 0 and sort last.
+// order[] = the n workgroup ids by decreasing cost, to 1/1024 of the largest:
+// a counting sort over 1024 cost buckets in one workgroup (the order inside
+// a bucket is that of the LDS atomics; it only schedules the gather, whose
+// workgroups each write their own band, so the templates do not depend on it).
 __global__ __launch_bounds__(1024) void k_dep_order(const unsigned long long* __restrict__ cost, int n,
                                                     int32_t* __restrict__ order)
 {
-    extern __shared__ uint32_t keys[];
     __shared__ unsigned long long smax[16];
-    int m = 1;
-    while (m < n) m <<= 1;
+    __shared__ int start[kOrderBuckets];
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     unsigned long long mx = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) mx = cost[i] > mx ? cost[i] : mx;
+    for (int i = tid; i < n; i += blockDim.x) mx = cost[i] > mx ? cost[i] : mx;
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_xor(mx, off, 64);
         mx = o > mx ? o : mx;
     }
-    if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = mx;
+    if (lane == 0) smax[wv] = mx;
+    start[tid] = 0;
     __syncthreads();
     mx = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) mx = smax[w] > mx ? smax[w] : mx;
+    for (int w = 0; w < 16; ++w) mx = smax[w] > mx ? smax[w] : mx;
     int shift = 0;
-    while ((mx >> shift) >= (1ull << 18)) ++shift;
-    for (int i = threadIdx.x; i < m; i += blockDim.x)
-        keys[i] = i < n ? ((uint32_t)(cost[i] >> shift) << 14) | (uint32_t)(0x3FFF - i) : 0u;
+    while ((mx >> shift) >= (unsigned long long)kOrderBuckets) ++shift;
+    auto bucket = [&](int i) { return kOrderBuckets - 1 - (int)(cost[i] >> shift); };
+    for (int i = tid; i < n; i += blockDim.x) atomicAdd(&start[bucket(i)], 1);
     __syncthreads();
-    for (int k = 2; k <= m; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < m; i += blockDim.x) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const uint32_t a = keys[i], b = keys[l];
-                    // the block of i descends when (i & k) == 0: all of it at k = m
-                    if ((i & k) == 0 ? (a < b) : (a > b)) {
-                        keys[i] = b;
-                        keys[l] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
+    // exclusive prefix sum of the 1024 counts, one per thread
+    const int c = start[tid];
+    int incl = c;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) order[i] = 0x3FFF - (int32_t)(keys[i] & 0x3FFF);
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < wv; ++w) base += wsum[w];
+    start[tid] = base + incl - c;
+    __syncthreads();
+    for (int i = tid; i < n; i += blockDim.x) order[atomicAdd(&start[bucket(i)], 1)] = i;
 }
 
 // The gather over descriptors of npoints points (template t owns
@@ -349,15 +348,13 @@ static int launch_gather(sfh_ctx* ctx, const PointDesc* desc, const int64_t* d_o
         void *pc = nullptr, *po = nullptr;
         if (int rc = scratch(ctx, "dep_cost", (size_t)grid * sizeof(unsigned long long), &pc)) return rc;
         if (int rc = scratch(ctx, "dep_order", (size_t)grid * sizeof(int32_t), &po)) return rc;
-        SFH_HIP_TRY(ctx, hipMemsetAsync(pc, 0, (size_t)grid * sizeof(unsigned long long), ctx->stream));
         {
             TimedLaunch tl(ctx, "deposit_order");
-            hipLaunchKernelGGL(k_dep_cost, dim3((unsigned)cdiv(npoints, 256)), dim3(256), 0, ctx->stream, desc,
-                               d_offsets, ntemplates, npoints, nx, ncg, nblk, (unsigned long long*)pc);
-            int m = 1;
-            while (m < grid) m <<= 1;
-            hipLaunchKernelGGL(k_dep_order, dim3(1), dim3(1024), (size_t)m * sizeof(uint32_t),
-                               ctx->stream, (const unsigned long long*)pc, (int)grid, (int32_t*)po);
+            hipLaunchKernelGGL(k_dep_cost, dim3((unsigned)ntemplates), dim3(256),
+                               (size_t)nblk * sizeof(unsigned long long), ctx->stream, desc, d_offsets, nx, ncg, nblk,
+                               (unsigned long long*)pc);
+            hipLaunchKernelGGL(k_dep_order, dim3(1), dim3(1024), 0, ctx->stream, (const unsigned long long*)pc,
+                               (int)grid, (int32_t*)po);
         }
         if (int rc = check_launch(ctx, "k_dep_order")) return rc;
         order = (const int32_t*)po;
