@@ -886,25 +886,34 @@ int sfh_fg(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_data* data, const d
     SFH_ENTER(ctx);
     SFH_ARG_CHECK(ctx, tpl && data && coeffs, "null pointer");
     SFH_ARG_CHECK(ctx, !want_F || negLL, "negLL must be given when want_F");
-    double* dc = nullptr;
-    if (int rc = upload(ctx, "h_coeffs", coeffs, tpl->ntemplates, &dc)) return rc;
-    void *pn = nullptr, *pg = nullptr, *pm = nullptr;
-    if (int rc = scratch(ctx, "h_negll", sizeof(double), &pn)) return rc;
-    if (G) {
-        if (int rc = scratch(ctx, "h_G", (size_t)tpl->ntemplates * sizeof(double), &pg)) return rc;
-    }
+    const int64_t T = tpl->ntemplates;
+    // The context's pinned host staging holds [negLL, G (T), coefficients (T)]:
+    // the coefficients go up from it in one DMA, and the final reduction
+    // writes negLL and G into it directly, so no copy comes back (the pageable
+    // copies this replaces cost ~30 us per call).  The call holds the context
+    // lock until the stream is synchronised, so the staging is not shared.
+    void* ph = nullptr;
+    if (int rc = pinned(ctx, (size_t)(1 + 2 * T) * sizeof(double), &ph)) return rc;
+    double* hout = (double*)ph;
+    double* hin = hout + 1 + T;
+    if (T > 0) std::memcpy(hin, coeffs, (size_t)T * sizeof(double));
+    void* dout = nullptr;
+    SFH_HIP_TRY(ctx, hipHostGetDevicePointer(&dout, ph, 0));
+    void *pc = nullptr, *pm = nullptr;
+    if (int rc = scratch(ctx, "h_coeffs", (size_t)(T > 0 ? T : 1) * sizeof(double), &pc)) return rc;
+    if (T > 0)
+        SFH_HIP_TRY(ctx, hipMemcpyAsync(pc, hin, (size_t)T * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     if (composite) {
         if (int rc = scratch(ctx, "h_composite", (size_t)tpl->nbins * sizeof(double), &pm)) return rc;
     }
-    if (int rc = sfh_fg_device(ctx, tpl, data, dc, want_F, eps, eps_grad, (double*)pn, (double*)pg, (double*)pm))
+    if (int rc = sfh_fg_device(ctx, tpl, data, (const double*)pc, want_F, eps, eps_grad,
+                               want_F ? (double*)dout : nullptr, G ? (double*)dout + 1 : nullptr, (double*)pm))
         return rc;
-    if (want_F)
-        if (int rc = download(ctx, negLL, (const double*)pn, 1)) return rc;
-    if (G)
-        if (int rc = download(ctx, G, (const double*)pg, tpl->ntemplates)) return rc;
     if (composite)
         if (int rc = download(ctx, composite, (const double*)pm, tpl->nbins)) return rc;
     SFH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (want_F) *negLL = hout[0];
+    if (G && T > 0) std::memcpy(G, hout + 1, (size_t)T * sizeof(double));
     return SFH_OK;
 }
 
