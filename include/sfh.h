@@ -207,9 +207,9 @@ int sfh_multifit_destroy(sfh_multifit* mf);
 
 /* Work decomposition of the grouped fg! launch: about item_elems template
  * elements per work item (whole 32-bin blocks of one fit), workgroups of
- * 64*waves threads (waves in {4, 8, 16}; 16 only when no fit has more than 512
- * templates) and wg_per_cu persistent workgroups
- * per compute unit. */
+ * 64*waves threads (waves in {4, 8, 16}: 4 for fits of up to 1024 templates,
+ * 8 up to 704, 16 up to 512; a group whose largest fit exceeds 704 starts at 4)
+ * and wg_per_cu persistent workgroups per compute unit. */
 int sfh_multifit_get_shape(const sfh_multifit* mf, int64_t* item_elems, int* waves, int* wg_per_cu);
 int sfh_multifit_set_shape(sfh_multifit* mf, int64_t item_elems, int waves, int wg_per_cu);
 /* Times every candidate shape on this group's fits (reps launches each, at

@@ -542,7 +542,22 @@ __global__ __launch_bounds__(kWG) void k_opt_propose(OptParams p)
     }
     if (threadIdx.x == 0) {
         p.active[f] = 1;
-        p.st[f] = st;
+        // only the fields this step changes: a whole-struct store would carry
+        // the state's arrays through a private-memory copy
+        OptState& o = p.st[f];
+        o.phase = st.phase;
+        o.nmem = st.nmem;
+        o.head = st.head;
+        o.ls_evals = st.ls_evals;
+        o.bracketed = st.bracketed;
+        o.alpha = st.alpha;
+        o.dphi0 = st.dphi0;
+        o.a_lo = st.a_lo;
+        o.f_lo = st.f_lo;
+        o.d_lo = st.d_lo;
+        o.a_hi = st.a_hi;
+        o.f_hi = st.f_hi;
+        o.d_hi = st.d_hi;
     }
 }
 
@@ -815,10 +830,13 @@ __global__ __launch_bounds__(kWG) void k_datasum(const FitDesc* __restrict__ fit
 int kmax_for(int64_t maxT, int nw)
 {
     const int need = (int)cdiv(maxT, 8 * nw);
-    // 16-wave workgroups (128 registers per lane) stop at 4 columns per
-    // thread: 8 columns' two register sets and gradient sums spill
-    const int cap = nw == 4 ? 32 : nw == 8 ? 16 : 4;
-    for (int v : {2, 4, 8, 11, 16, 32})
+    // The variants that hold their two register sets and gradient sums
+    // without spilling: 16-wave workgroups (128 registers per lane) stop at 4
+    // columns per thread, 8-wave ones at 11 (T <= 704); 16 columns spill at
+    // two waves per SIMD, so fits of 353-1024 templates take the 4-wave,
+    // 32-column variant (one wave per SIMD, 512 registers).
+    const int cap = nw == 4 ? 32 : nw == 8 ? 11 : 4;
+    for (int v : {2, 4, 8, 11, 32})
         if (need <= v && v <= cap) return v;
     return 0;
 }
@@ -1134,10 +1152,10 @@ void launch_fg(sfh_multifit* mf, const double* d_X, const int32_t* active, doubl
 #define SFH_L(NW, KM) launch_fg_t<NW, KM>(s, grid, mf->d_fits, mf->d_items, mf->nitems, d_X, active, eps, eps_g, want_G, mf->d_llp, mf->d_gp)
     if (mf->nw == 4) {
         if (km <= 2) SFH_L(4, 2); else if (km <= 4) SFH_L(4, 4); else if (km <= 8) SFH_L(4, 8);
-        else if (km <= 11) SFH_L(4, 11); else if (km <= 16) SFH_L(4, 16); else SFH_L(4, 32);
+        else if (km <= 11) SFH_L(4, 11); else SFH_L(4, 32);
     } else if (mf->nw == 8) {
         if (km <= 2) SFH_L(8, 2); else if (km <= 4) SFH_L(8, 4); else if (km <= 8) SFH_L(8, 8);
-        else if (km <= 11) SFH_L(8, 11); else SFH_L(8, 16);
+        else SFH_L(8, 11);
     } else {
         if (km <= 2) SFH_L(16, 2); else SFH_L(16, 4);
     }
@@ -1213,6 +1231,9 @@ int sfh_multifit_create(sfh_ctx* ctx, int64_t nfits, const sfh_templates* const*
         boff.push_back(blocked_elems);
         blocked_elems += cdiv(fd.N, kSB) * kSB * fd.T;
     }
+    // the default 8-wave workgroups hold fits of up to 704 templates; larger
+    // ones (up to 1024) take 4-wave workgroups
+    if (kmax_for(mf->maxT, mf->nw) == 0) mf->nw = 4;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) {
         delete mf;
@@ -1301,7 +1322,8 @@ int sfh_multifit_set_shape(sfh_multifit* mf, int64_t item_elems, int waves, int 
     SFH_ARG_CHECK(ctx, item_elems >= kSB && (waves == 4 || waves == 8 || waves == 16) && wg_per_cu >= 1 && wg_per_cu <= 8,
                   "sfh_multifit_set_shape: item_elems >= 32, waves in {4, 8, 16}, 1 <= wg_per_cu <= 8");
     SFH_ARG_CHECK(ctx, kmax_for(mf->maxT, waves) > 0,
-                  "sfh_multifit_set_shape: 16 waves per workgroup need at most 512 templates per fit");
+                  "sfh_multifit_set_shape: %d waves per workgroup hold at most %d templates per fit (largest: %lld)",
+                  waves, waves == 4 ? 1024 : waves == 8 ? 704 : 512, (long long)mf->maxT);
     if (int rc = bind(ctx)) return rc;
     mf->item_elems = item_elems;
     mf->nw = waves;

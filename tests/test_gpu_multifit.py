@@ -37,7 +37,8 @@ def test_grouped_fg_against_oracle(sfh, O, gpu_ctx):
     mf = MultiFit([p[0] for p in probs], [p[1] for p in probs])
     rng = np.random.default_rng(5)
     X = [p[2] * np.exp(0.3 * rng.normal(size=p[2].size)) for p in probs]
-    shapes = [(1 << 18, 8, 2), (1 << 16, 4, 3), (4096, 8, 1)]
+    # the group's largest fit (1024 templates) takes 4-wave workgroups
+    shapes = [(1 << 18, 4, 2), (1 << 16, 4, 3), (4096, 4, 1)]
     results = []
     for shp in shapes:
         mf.set_shape(*shp)
@@ -53,19 +54,22 @@ def test_grouped_fg_against_oracle(sfh, O, gpu_ctx):
     print("autotuned shape:", mf.autotune(reps=2))
     nl, _ = mf.fg(X)
     assert np.allclose(nl, results[0][0], rtol=1e-13, atol=0)
-    # 16-wave workgroups take groups whose fits have at most 512 templates
-    with pytest.raises(sfh.ArgumentError):
-        mf.set_shape(1 << 20, 16, 1)
-    small = [i for i, (_, T) in enumerate(SHAPES) if T <= 512]
-    ms = MultiFit([probs[i][0] for i in small], [probs[i][1] for i in small])
-    for shp in [(1 << 20, 16, 1), (1 << 16, 16, 2)]:
-        ms.set_shape(*shp)
-        nl, G = ms.fg([X[i] for i in small])
-        for j, i in enumerate(small):
-            rnl, rG, _ = O.fg(probs[i][0], probs[i][1], X[i], nthreads=8)
-            assert abs(nl[j] - rnl) <= 1e-12 * abs(rnl), (shp, i)
-            assert np.linalg.norm(G[j] - rG) <= 1e-10 * np.linalg.norm(rG), (shp, i)
-    ms.close()
+    # 8-wave workgroups take groups whose fits have at most 704 templates,
+    # 16-wave ones at most 512
+    for waves in (8, 16):
+        with pytest.raises(sfh.ArgumentError):
+            mf.set_shape(1 << 20, waves, 1)
+    for limit, waves_shapes in ((704, [(1 << 18, 8, 2), (4096, 8, 1)]), (512, [(1 << 20, 16, 1), (1 << 16, 16, 2)])):
+        small = [i for i, (_, T) in enumerate(SHAPES) if T <= limit]
+        ms = MultiFit([probs[i][0] for i in small], [probs[i][1] for i in small])
+        for shp in waves_shapes:
+            ms.set_shape(*shp)
+            nl, G = ms.fg([X[i] for i in small])
+            for j, i in enumerate(small):
+                rnl, rG, _ = O.fg(probs[i][0], probs[i][1], X[i], nthreads=8)
+                assert abs(nl[j] - rnl) <= 1e-12 * abs(rnl), (shp, i)
+                assert np.linalg.norm(G[j] - rG) <= 1e-10 * np.linalg.norm(rG), (shp, i)
+        ms.close()
 
 
 def test_grouped_fg_repeatable_and_without_gradient(sfh, gpu_ctx):
