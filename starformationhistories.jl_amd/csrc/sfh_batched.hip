@@ -391,6 +391,7 @@ __device__ inline void static_for(F&& f)
 // reused by its ~32 resident workgroups.
 struct Swizzle {
     int nrt, nct, gr, gc, nsr, nsc;
+    int xsplit;     // split-K launches: 1-D grid, XCD x runs splits x, x + 8, ... (nsplit % 8 == 0)
 };
 
 __device__ inline bool swizzle(const Swizzle& s, int64_t id, int* rt, int* ct)
@@ -468,8 +469,22 @@ __global__ __launch_bounds__(WR * WC * 64, 1) void k_gemm(const uint4* __restric
     const u32x4* Bm = reinterpret_cast<const u32x4*>(B_);
 
     int rt, ct;
-    const int64_t split = blockIdx.y;
-    if (!swizzle(sw, blockIdx.x, &rt, &ct)) return;
+    int64_t split;
+    if (sw.xsplit) {
+        // consecutive workgroups go round-robin over the 8 XCDs, so workgroup
+        // id runs on XCD id % 8: that XCD takes whole splits, its tiles in
+        // column-major order, so the workgroups resident together share each
+        // R column tile (3 row tiles) and each A row tile (~10 column tiles)
+        // through the XCD's L2
+        const int64_t id = blockIdx.x, tiles = (int64_t)sw.nrt * sw.nct;
+        const int64_t local = id >> 3, tile = local % tiles;
+        split = (local / tiles) * 8 + (id & 7);
+        ct = (int)(tile / sw.nrt);
+        rt = (int)(tile % sw.nrt);
+    } else {
+        split = blockIdx.y;
+        if (!swizzle(sw, blockIdx.x, &rt, &ct)) return;
+    }
     const int64_t kb0 = split * kb_per_split;
     int64_t kb1 = kb0 + kb_per_split;
     if (kb1 > nkb_total) kb1 = nkb_total;
@@ -983,6 +998,7 @@ static Swizzle make_swizzle(int nrt, int nct, int gr, int gc)
     s.gc = std::max(1, std::min(gc, nct));
     s.nsr = (int)cdiv(nrt, s.gr);
     s.nsc = (int)cdiv(nct, s.gc);
+    s.xsplit = 0;
     return s;
 }
 
@@ -1074,10 +1090,14 @@ static int fg_batched_impl(sfh_ctx* ctx, const sfh_templates* tpl, const sfh_dat
         const int64_t kps = cdiv(nkb2, best);
         nsplit = (int)cdiv(nkb2, kps);
         if (int rc = scratch(ctx, "bat_g", (size_t)nsplit * Bpad * ops->tpad2 * sizeof(float), &pg)) return rc;
-        const Swizzle sw = make_swizzle(nrt, nct2, nrt, std::max(1, 8 / nrt));
+        Swizzle sw = make_swizzle(nrt, nct2, nrt, std::max(1, 8 / nrt));
+        dim3 grid(swizzle_grid(sw), (unsigned)nsplit);
+        if (nsplit % 8 == 0) {
+            sw.xsplit = 1;
+            grid = dim3((unsigned)(tiles * nsplit), 1);
+        }
         EpiLL el{};
         EpiGrad eg{(float*)pg, Bpad, ops->tpad2};
-        const dim3 grid(swizzle_grid(sw), (unsigned)nsplit);
         TimedLaunch tl(ctx, "batched_gradient");
         if (mode == FP32)
             hipLaunchKernelGGL((k_gemm<GRAD_FP32, false, GWR, GWC>), grid, dim3(GWR * GWC * 64), 0, ctx->stream,
