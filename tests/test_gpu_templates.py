@@ -65,6 +65,30 @@ def test_batch_200x300_shape(sfh, O, gpu_ctx):
     assert pairs == counts
 
 
+@pytest.mark.parametrize("cov_mult", [1.0, 0.0])
+def test_batch_launch_order_does_not_change_templates(sfh, O, gpu_ctx, cov_mult):
+    """Batches of up to 16,384 (template, row band) workgroups run in the
+    order of their estimated cost, larger ones in the plain rotation; every
+    workgroup writes its own band, so both give the same templates bit for
+    bit: 6 templates alone (ordered) and the same 6 repeated to 1,030 templates
+    (16,480 workgroups on the 200 x 300 grid, unordered); the covariant and the
+    asymmetric kernel."""
+    import torch
+    from sfh_amd import synth
+    edges = synth.edges_200x300()
+    specs = synth.specs_70x10()[::131][:6]
+    pts = [synth.template_points(s, 2000, edges, 120.0)[0] for s in specs]
+    small = sfh.deposit_batch(sfh.PointBatch(pts, edges, cov_mult=cov_mult))
+    big = sfh.deposit_batch(sfh.PointBatch([pts[i % 6] for i in range(1030)], edges, cov_mult=cov_mult))
+    torch.cuda.synchronize()
+    for i in range(1030):
+        assert torch.equal(big[i], small[i % 6]), i
+    ex, ey = oracle_range(O, edges[0]), oracle_range(O, edges[1])
+    sp = pts[0]
+    ref, _ = O.bin_cmd_smooth(sp.colors, sp.mags, sp.color_err, sp.mag_err, cov_mult, sp.weights, ex, ey, nthreads=8)
+    assert max_norm_err(small[0].cpu().numpy().reshape(ref.shape, order="F"), ref) < TOL
+
+
 def test_batch_config1_exact_combination(sfh, O, gpu_ctx):
     """The 700-template workload's exact combination: 200 x 300 grid,
     2000-point isochrones, IMF 0.08-120 Msun, colour-magnitude correlation 0.6
