@@ -686,12 +686,14 @@ class Tabulated:
         return fn
 
 
-def _fn_of(f, lo, hi, keep):
-    """The sfh_fn of callable ``f`` over arguments in [lo, hi]: its own form
-    when it has one (``as_fn``), else a table (kept alive in ``keep``)."""
+def _fn_of(f, arg_range, keep):
+    """The sfh_fn of callable ``f``: its own form when it has one
+    (``as_fn``), else a table over the arguments it will see, [lo, hi] =
+    ``arg_range()`` (called only then), kept alive in ``keep``."""
     if hasattr(f, "as_fn"):
         keep.append(f)
-        return f.as_fn(lo, hi)
+        return f.as_fn()
+    lo, hi = arg_range()
     if not hi > lo:
         hi = lo + 1.0
     # exactly over [lo, hi]: arguments are clamped to the table, and a padded
@@ -702,15 +704,33 @@ def _fn_of(f, lo, hi, keep):
     return t.as_fn()
 
 
+def _pinned_f64(n):
+    """A float64 host array of n elements in page-locked memory when a GPU is
+    present (the library's copies from it are direct DMA at the full bus
+    rate, where pageable memory goes through a staging copy), else an
+    ordinary array.  Returns (array, owner): keep the owner alive with it."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            t = torch.empty(max(int(n), 1), dtype=torch.float64, pin_memory=True)
+            return t.numpy()[:n], t
+    except (ImportError, RuntimeError):
+        pass
+    return np.empty(n, dtype=np.float64), None
+
+
 class IsochroneBatch:
     """Many isochrones in the packed form sfh_partial_cmd_smooth_batch takes:
     isochrone t owns entries offsets[t] .. offsets[t+1]-1 of m_ini and of each
     filter's magnitude array.  Building it once lets repeated batch calls skip
-    the per-call concatenation of per-isochrone arrays."""
+    the per-call concatenation of per-isochrone arrays; the packed arrays live
+    in page-locked host memory when a GPU is present, so each call's upload of
+    them is a direct DMA."""
 
     def __init__(self, m_ini, mags, offsets):
-        self.m_ini = np.ascontiguousarray(m_ini, dtype=np.float64).ravel()
-        self.mags = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in mags]
+        self._owners = []
+        self.m_ini = self._pack(np.asarray(m_ini, dtype=np.float64).ravel())
+        self.mags = [self._pack(np.asarray(a, dtype=np.float64).ravel()) for a in mags]
         self.offsets = np.ascontiguousarray(offsets, dtype=np.int64).ravel()
         n = self.m_ini.size
         if self.offsets.size < 2 or self.offsets[0] != 0 or self.offsets[-1] != n or np.any(np.diff(self.offsets) < 0):
@@ -718,23 +738,50 @@ class IsochroneBatch:
         if any(a.size != n for a in self.mags):
             raise ArgumentError("axes(m_ini) == axes(mags[i])")
 
+    def _pack(self, a):
+        out, owner = _pinned_f64(a.size)
+        out[:] = a
+        if owner is not None:
+            self._owners.append(owner)
+        return out
+
     @classmethod
     def from_list(cls, isochrones):
         """From a sequence of (m_ini, mags) pairs."""
         m_cat = [np.asarray(m, dtype=np.float64).ravel() for m, _ in isochrones]
         counts = np.array([m.size for m in m_cat], dtype=np.int64)
         nf = len(isochrones[0][1]) if len(isochrones) else 0
-        mags = []
-        for f in range(nf):
-            arrs = [np.asarray(mg[f], dtype=np.float64).ravel() for _, mg in isochrones]
-            if any(a.size != n for a, n in zip(arrs, counts)):
+        arrs = [[np.asarray(mg[f], dtype=np.float64).ravel() for _, mg in isochrones] for f in range(nf)]
+        for per_filter in arrs:
+            if any(a.size != n for a, n in zip(per_filter, counts)):
                 raise ArgumentError("axes(m_ini) == axes(mags[i])")
-            mags.append(np.concatenate(arrs))
-        return cls(np.concatenate(m_cat) if m_cat else np.zeros(0), mags,
-                   np.concatenate([[0], np.cumsum(counts)]))
+        # concatenated straight into the packed (page-locked) arrays
+        self = cls.__new__(cls)
+        self._owners = []
+        n = int(counts.sum())
+
+        def packed(parts):
+            out, owner = _pinned_f64(n)
+            if owner is not None:
+                self._owners.append(owner)
+            if parts:
+                np.concatenate(parts, out=out)
+            return out
+        self.m_ini = packed(m_cat)
+        self.mags = [packed(per_filter) for per_filter in arrs]
+        self.offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        return self
 
     def __len__(self):
         return self.offsets.size - 1
+
+    def ranges(self):
+        """(min, max) of m_ini and of each filter's magnitudes, computed once
+        (for tabulating callables over the arguments they will see)."""
+        if getattr(self, "_ranges", None) is None:
+            mm = lambda a: (float(a.min()), float(a.max())) if a.size else (0.0, 1.0)  # noqa: E731
+            self._ranges = (mm(self.m_ini), [mm(a) for a in self.mags])
+        return self._ranges
 
     @property
     def nfilters(self):
@@ -788,16 +835,16 @@ def partial_cmd_smooth_batch(isochrones, mag_err_funcs, y_index, color_indices, 
         mean_mass = imf_mean(imf)
     nv, mm = per(normalize_value), per(mean_mass)
     keep = [m_all, mags_all, dm, nv, mm, offsets]
-    # argument ranges the callables will see: the filters' magnitudes plus dmod
-    lo = [float(a.min() + dm.min()) if a.size else 0.0 for a in mags_all]
-    hi = [float(a.max() + dm.max()) if a.size else 1.0 for a in mags_all]
+    # argument ranges the callables will see (the filters' magnitudes plus
+    # dmod; the masses), needed only for callables that are tabulated
+    mag_range = lambda f: (batch.ranges()[1][f][0] + float(dm.min()), batch.ranges()[1][f][1] + float(dm.max()))  # noqa: E731,E501
     FnArr = _lib.sfh_fn * nf
-    err = FnArr(*[_fn_of(mag_err_funcs[f], lo[f], hi[f], keep) for f in range(nf)])
-    comp = FnArr(*[(_fn_of(completeness_funcs[f], lo[f], hi[f], keep) if completeness_funcs[f] is not None
+    err = FnArr(*[_fn_of(mag_err_funcs[f], lambda f=f: mag_range(f), keep) for f in range(nf)])
+    comp = FnArr(*[(_fn_of(completeness_funcs[f], lambda f=f: mag_range(f), keep) if completeness_funcs[f] is not None
                     else Constant(1.0).as_fn()) for f in range(nf)])
-    bias = FnArr(*[(_fn_of(bias_funcs[f], lo[f], hi[f], keep) if bias_funcs[f] is not None
+    bias = FnArr(*[(_fn_of(bias_funcs[f], lambda f=f: mag_range(f), keep) if bias_funcs[f] is not None
                     else Constant(0.0).as_fn()) for f in range(nf)])
-    imf_fn = _fn_of(imf, float(m_all.min()) if m_all.size else 0.0, float(m_all.max()) if m_all.size else 1.0, keep)
+    imf_fn = _fn_of(imf, lambda: batch.ranges()[0], keep)
     mag_ptrs = (C.c_void_p * nf)(*[a.ctypes.data for a in mags_all])
     nb = (ex.length - 1) * (ey.length - 1)
     npts, pairs = C.c_int64(), C.c_int64()
