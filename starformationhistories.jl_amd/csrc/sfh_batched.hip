@@ -771,19 +771,29 @@ __global__ __launch_bounds__(kWG, 1) void k_gemm_f64_ll(const uint4* __restrict_
     auto compute = [&](int st) {
         const float* cA = reinterpret_cast<const float*>(lds + st * STAGE_U4);
         const double* cX = reinterpret_cast<const double*>(lds + st * STAGE_U4 + CA * 64);
+        // fragments of step ks + 1 are read while step ks's products run
+        float ga[2][4];
+        double fb[2][4];
+        auto rd = [&](int ks, int s) {
+            const int k = 4 * ks + lk;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) ga[s][a] = cA[a_elem(wr * 64 + a * 16 + lr, k)];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) fb[s][b] = cX[k * kBN + wc * 64 + b * 16 + lr];
+        };
+        rd(0, 0);
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
-            const int k = 4 * ks + lk;
-            double fa[4], fb[4];
+            const int s = ks & 1;
+            if (ks < 3) rd(ks + 1, s ^ 1);
+            double fa[4];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) fa[a] = (double)cA[a_elem(wr * 64 + a * 16 + lr, k)];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) fb[b] = cX[k * kBN + wc * 64 + b * 16 + lr];
+            for (int a = 0; a < 4; ++a) fa[a] = (double)ga[s][a];
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[s][b], acc[a][b], 0, 0, 0);
         }
     };
 
