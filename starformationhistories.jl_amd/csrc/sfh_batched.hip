@@ -27,7 +27,7 @@
 //           MFMA sums its 16 products aligned to the largest one and drops what
 //           falls below ~2^-27 of it, a bias no FP64 promotion removes.
 // The gradient GEMM uses hi+lo BF16 (3 products; FP32 mode: FP32), FP32
-// accumulation folded into FP64 every 32 K-blocks (512 bins).
+// accumulation folded into FP64 every 32 K-blocks (512 bins; FP32 mode: 8).
 //
 // Operand layout ("fragment tiles"): for a 32-row x 16-k block, 64 lanes x
 // 8 values, lane l holding row l%32 and k = 8*(l/32) + 0..7 (the lane layout of
@@ -460,7 +460,7 @@ __global__ __launch_bounds__(WR * WC * 64, 1) void k_gemm(const uint4* __restric
     // ring depth: the split-K gradient streams R from HBM and wants a deep ring;
     // the composite GEMMs hit L2 and prefer the occupancy of a shallow one
     constexpr int S = (M == GRAD_BF16 || M == GRAD_FP32) ? 4 : 2;
-    constexpr int KP = (M == GRAD_BF16 || M == GRAD_FP32) ? 32 : 0;
+    constexpr int KP = M == GRAD_BF16 ? 32 : M == GRAD_FP32 ? 8 : 0;
     static_assert(CH % NW == 0, "stage size");
 
     __shared__ u32x4 ring[S][CH * 64];

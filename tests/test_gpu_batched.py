@@ -4,7 +4,7 @@ them).  The log-likelihood bound of the f64 form is the 1e-9 target; the other
 bounds are 3-6x the largest error measured over these cases on an MI355X, so
 that a regression of the GEMM kernels shows:
   form     LL rel-err   gradient norm-rel-err   (extreme-value case)   measured max
-  f64      1e-9         2e-6                     1e-5                 2.1e-14 / 3.7e-7 (2.1e-6)
+  f64      1e-9         2e-6                     1e-5                 2.1e-14 / 7.1e-7 (2.3e-6)
   bf16x2   5e-5         1e-4                     2e-5                 1.4e-5 / 3.3e-5 (4.5e-6)
   fp32     1e-5         2e-5                     2e-6                 3.1e-6 / 4.3e-6 (3.5e-7)
 """
