@@ -87,6 +87,12 @@ int sfh_graph_destroy(sfh_graph* graph);
 int sfh_ctx_kernel_stats(sfh_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
 int sfh_ctx_reset_stats(sfh_ctx* ctx);
 int sfh_version(void);
+/* No reference counterpart: a measurement helper for the deposition's
+ * roofline.  Runs the row strip's instruction mix (per step one FMA reading a
+ * scalar register and one multiply of two vector registers, 8 independent
+ * chains) at four waves per SIMD on every CU for `iters` iterations
+ * (1 .. 2^24) and returns the VALU lane-instructions per second it reached. */
+int sfh_probe_valu_issue(sfh_ctx* ctx, int64_t iters, double* lane_instr_per_s);
 
 /* ------------------------------------------------------------- templates */
 

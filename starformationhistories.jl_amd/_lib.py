@@ -108,6 +108,7 @@ SIGNATURES = [
     ("sfh_ctx_synchronize", _int, [_vp]),
     ("sfh_ctx_set_timing", _int, [_vp, _int]),
     ("sfh_ctx_kernel_stats", _int, [_vp, C.c_char_p, _pd, _p64]),
+    ("sfh_probe_valu_issue", _int, [_vp, C.c_int64, _pd]),
     ("sfh_ctx_reset_stats", _int, [_vp]),
     ("sfh_templates_create", _int, [_vp, _vp, _i64, _i64, C.POINTER(_vp)]),
     ("sfh_templates_create_device", _int, [_vp, _vp, _i64, _i64, C.POINTER(_vp)]),

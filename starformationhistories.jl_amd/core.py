@@ -76,6 +76,13 @@ class Context:
         self.check(self.lib.sfh_ctx_kernel_stats(self.handle, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def valu_issue_rate(self, iters: int = 20000) -> float:
+        """VALU lane-instructions per second that the deposition strip's
+        instruction mix reaches on this device (four waves per SIMD)."""
+        r = C.c_double()
+        self.check(self.lib.sfh_probe_valu_issue(self.handle, int(iters), C.byref(r)))
+        return r.value
+
     def reset_stats(self):
         self.check(self.lib.sfh_ctx_reset_stats(self.handle))
 

@@ -517,6 +517,36 @@ void ctx_release(sfh_ctx* ctx)
     if (last) ctx_teardown(ctx);
 }
 
+// The deposition strip's instruction mix as independent streams, four waves
+// per SIMD: per step, each of 8 chains takes one FMA that reads a scalar
+// register (the strip's accumulate with a column factor) and one multiply of
+// two vector registers (the strip's recurrence), 64 VALU instructions per
+// iteration and lane.  Measures the VALU issue rate such code reaches.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_valu_issue_probe(float s, int iters,
+                                                                                                   float* out)
+{
+    float acc[8], f[8], g[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        acc[i] = 0.0f;
+        f[i] = 1.0f + 1e-3f * (float)i + 1e-7f * (float)threadIdx.x;
+        g[i] = 1.0f - 1e-9f * (float)(threadIdx.x + i);
+    }
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                acc[i] = fmaf(f[i], s, acc[i]);
+                f[i] *= g[i];
+            }
+    }
+    float t = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) t += acc[i] + f[i];
+    out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = t;
+}
+
 }  // namespace sfh
 
 extern "C" {
@@ -627,6 +657,39 @@ int sfh_ctx_kernel_stats(sfh_ctx* ctx, const char* kernel, double* total_ms, int
     auto it = ctx->stats.find(kernel);
     if (total_ms) *total_ms = it == ctx->stats.end() ? 0.0 : it->second.total_ms;
     if (launches) *launches = it == ctx->stats.end() ? 0 : it->second.launches;
+    return SFH_OK;
+}
+
+int sfh_probe_valu_issue(sfh_ctx* ctx, int64_t iters, double* lane_instr_per_s)
+{
+    if (!ctx || !lane_instr_per_s || iters < 1 || iters > (1 << 24)) return SFH_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (int rc = bind(ctx)) return rc;
+    int cus = 0;
+    SFH_HIP_TRY(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const int blocks = 4 * cus;     // 4 workgroups of 4 waves per CU: 4 waves per SIMD
+    void* out = nullptr;
+    if (int rc = scratch(ctx, "valu_probe", (size_t)blocks * 256 * sizeof(float), &out)) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SFH_HIP_TRY(ctx, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return set_error(ctx, SFH_ERR_DEVICE, "hipEventCreate failed");
+    }
+    hipLaunchKernelGGL(k_valu_issue_probe, dim3(blocks), dim3(256), 0, ctx->stream, 0.5f, 16, (float*)out);
+    (void)hipEventRecord(e0, ctx->stream);
+    hipLaunchKernelGGL(k_valu_issue_probe, dim3(blocks), dim3(256), 0, ctx->stream, 0.5f, (int)iters, (float*)out);
+    (void)hipEventRecord(e1, ctx->stream);
+    int rc = check_launch(ctx, "k_valu_issue_probe");
+    float ms = 0.0f;
+    if (rc == SFH_OK) {
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || !(ms > 0.0f))
+            rc = set_error(ctx, SFH_ERR_DEVICE, "probe timing failed");
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *lane_instr_per_s = (double)blocks * 256.0 * 64.0 * (double)iters / ((double)ms * 1e-3);
     return SFH_OK;
 }
 
