@@ -19,19 +19,19 @@
 // For e < kSeriesEps, D is evaluated without the erf-difference cancellation:
 //   D = (4 e / sqrt(pi)) exp(-u^2) S(u^2),  S(w) = sum_n H_2n(sqrt w) e^2n / (2n+1)!
 // (the mean of exp(-(u+s)^2) over |s| < e; Hermite polynomials H_2n), whose
-// coefficients in powers of w are kept up to w^DEG.  Relative to the local
-// value at |u| <= 4 the dropped terms are below ~1e-7 for DEG = 2 when
-// e < 0.026 and DEG = 3 when e < 0.1 (the terms of order e^2n |H_2n(4)| /
-// (2n+1)! are ~10 e^2, 28 e^4, 30 e^6, 15 e^8).
-// For e < 0.016 the factor is kept in the exponent instead (the Gaussian
+// coefficients in powers of w are kept up to w^3.  Relative to the local
+// value at |u| <= 4 the dropped terms are below ~1e-7 when e < 0.1 (the
+// terms of order e^2n |H_2n(4)| / (2n+1)! are ~10 e^2, 28 e^4, 30 e^6, 15 e^8).
+// For e < 0.026 the factor is kept in the exponent instead (the Gaussian
 // form): s0 + s1 w = s0 exp(b w) (1 + O(b^2 w^2)), b = s1 / s0 = 2e^2/3 +
 // O(e^4), so D = (4 e / sqrt(pi)) s0 exp(-(1 - b) w), one Gaussian of a
 // rescaled coordinate.  ln S(w) = ln s0 + b w - (4/45) e^4 w^2 + ..., so the
-// dropped term is (4/45) e^4 w^2 relative to the local value: <= 1.5e-6 at
-// |u| = 4 for e < 0.016 (below 8e-8 for e < 0.0077, and always below the
+// dropped term is (4/45) e^4 w^2 relative to the local value: <= 1.1e-5 at
+// |u| = 4 for e < 0.026 (below 8e-8 for e < 0.0077, and always below the
 // degree-1 polynomial's own truncation error (2/15) e^4 w^2); relative to
-// the kernel's peak it is below 4e-9.  On the 700-template workload 94% of
-// the pairs take this form.
+// the kernel's peak it is at most (4/45) e^4 max(w^2 e^-w) = 0.048 e^4,
+// below 2.2e-8, under the FP32 rounding of the sums.  On the 700-template
+// workload 97% of the pairs take this form.
 //
 // The series forms run in scaled coordinates a' = a sqrt(log2 e) (the
 // Gaussian form: a sqrt((1 - b) log2 e)), so exp(-a^2) = exp2(-a'^2) is one
@@ -49,8 +49,8 @@ namespace sfh {
 namespace psf {
 
 constexpr float kSeriesEps = 0.1f;
-constexpr float kDeg1Eps = 0.016f;    // the Gaussian form below this half width
-constexpr float kDeg2Eps = 0.026f;
+constexpr float kDeg1Eps = 0.026f;    // the Gaussian form below this half width
+constexpr float kDeg2Eps = 0.026f;    // (no degree-2 series: its range is the Gaussian form's)
 constexpr double kGLNode = 0.7745966692414834;   // :303
 constexpr float kW0 = 0.5555555555555556f;      // :304
 constexpr float kW1 = 0.8888888888888888f;
@@ -434,8 +434,8 @@ __host__ __device__ __forceinline__ void static_for(F&& f)
 // asymmetric kernel) instead of one per node and column (an exp2 takes two
 // VALU issue slots, an FMA or multiply one), and a column costs one FMA and
 // one multiply per node.  The multiplier's rounding grows over at most 8
-// steps.  The Gaussian form has du < 0.039 (half bin width e < 0.016), so B
-// stays within [2^-2, 2^2] for |m| < 25.  Each column adds the nodes in order
+// steps.  The Gaussian form has du < 0.064 (half bin width e < 0.026), so B
+// stays within [2^-2, 2^2] for |m| < 15.  Each column adds the nodes in order
 // k = 0, 1, 2.
 template <int NODES, bool MASK>
 __host__ __device__ __forceinline__ void gauss_strip(const PointDesc& d, const float* M, const float* L,
