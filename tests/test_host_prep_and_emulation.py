@@ -63,7 +63,7 @@ def test_emulation_config1_templates(sfh, emu, O, la, mh, cov_mult):
     """The kernel arithmetic at the 700-template workload's shape: a 200 x 300
     grid, 2000-point isochrones, IMF 0.08-120 Msun, colour-magnitude
     correlation 0.6 (covariant) or the pixel-space asymmetric kernel; every
-    x form (degree-1/2/3 series, erfc differences) occurs.  Max-normalised
+    x form (the Gaussian form, the degree-3 series, erfc differences) occurs.  Max-normalised
     error <= 1e-5 and per-bin relative error <= 2e-5 on bins above 1e-6 of the
     maximum."""
     from sfh_amd import synth
